@@ -222,7 +222,8 @@ def test_cartesian_signal_is_centred_dft(oracle):
     # sum_ix line[ix] exp(2 pi i (s - n/2)(ix - n/2)/n) via an inverse FFT with centring shifts
     want = np.fft.fftshift(np.fft.ifft(np.fft.ifftshift(line))) * n
     for r in range(traj.nreadouts):
-        assert rel(S[0, r], want) < 1e-9
+        # float32 dk and x carry ~1.2e-7 relative error, on phases up to ~25 rad
+        assert rel(S[0, r], want) < 1e-5
 
 
 def test_cartesian_signal_is_matmul_of_factors(oracle):
