@@ -1,0 +1,417 @@
+"""Python host interface of libcompas_amd.so, mirroring CompasToolkit.jl.
+
+The names, argument order and meaning follow the Julia package the
+reference ships (CompasToolkit.jl/src/CompasToolkit.jl:98-1036), so tests
+read like the reference's Julia examples.  Arrays are numpy arrays in C
+order: echos [nreadouts][nvoxels], coils [ncoils][nvoxels], signal
+[ncoils][nreadouts][nsamples] (Julia passes the same memory with reversed
+dimensions).
+
+Everything goes through the C-ABI declared in include/compas_amd.h; there is
+no fallback: if the library is missing or no GPU is present the calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libcompas_amd.so")
+
+SIGNAL_METHODS = {"naive": 0, "direct": 1, "matmul_pedantic": 2, "matmul": 3, "matmul_fast": 4, "matmul_low": 5}
+JACOBIAN_METHODS = {"naive": 0, "direct": 1, "gemm": 2, "gemm_fast": 3, "gemm_low": 4}
+
+# TissueParameterField (tissue_view.cuh:13-26)
+T1, T2, B1, B0, RHO_X, RHO_Y, X, Y, Z = range(9)
+
+
+class CompasError(RuntimeError):
+    pass
+
+
+_lib = None
+_vp = C.c_void_p
+_f = C.c_float
+_i = C.c_int
+_i32 = C.c_int32
+_i64p = C.POINTER(C.c_int64)
+
+_SIGNATURES = {
+    "compas_version": (C.c_char_p, []),
+    "compas_make_context": (_vp, []),
+    "compas_copy_context_for_device": (_vp, [_vp, _i]),
+    "compas_destroy_context": (None, [_vp]),
+    "compas_destroy_object": (None, [_vp]),
+    "compas_synchronize": (None, [_vp]),
+    "compas_make_array_float": (_vp, [_vp, _vp, _i, _i64p]),
+    "compas_read_array_float": (None, [_vp, _vp, _vp, C.c_int64]),
+    "compas_make_array_complex": (_vp, [_vp, _vp, _i, _i64p]),
+    "compas_read_array_complex": (None, [_vp, _vp, _vp, C.c_int64]),
+    "compas_destroy_array": (None, [_vp]),
+    "compas_make_tissue_parameters": (_vp, [_vp, _i] + [_vp] * 9),
+    "compas_make_fisp_sequence": (_vp, [_vp, _vp, _f, _f, _f, _i, _f, _i, _i]),
+    "compas_make_pssfp_sequence": (_vp, [_vp, _f, _vp, _f, _f, _f, _f, _f, _f, _vp]),
+    "compas_make_cartesian_trajectory": (_vp, [_i, _i, _f, _vp, _f]),
+    "compas_make_spiral_trajectory": (_vp, [_i, _i, _f, _vp, _vp]),
+    "compas_simulate_magnetization_fisp": (_vp, [_vp, _vp, _vp]),
+    "compas_simulate_magnetization_pssfp": (_vp, [_vp, _vp, _vp]),
+    "compas_magnetization_to_signal_cartesian": (_vp, [_vp, _i, _vp, _vp, _vp, _vp, _i32]),
+    "compas_magnetization_to_signal_spiral": (_vp, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "compas_compute_jacobian": (_vp, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]),
+    "compas_compute_jacobian_hermitian": (_vp, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]),
+    "phase_encoding": (_vp, [_vp, _vp, _vp, _vp]),
+    "compas_compute_residual": (_vp, [_vp, _vp, _vp, C.POINTER(C.c_float)]),
+    "compas_simulate_magnetization_derivative_pssfp": (_vp, [_vp, _i, _vp, _vp, _f, _vp]),
+    "compas_simulate_magnetization_derivative_fisp": (_vp, [_vp, _i, _vp, _vp, _f, _vp]),
+    # extensions
+    "compas_last_error": (C.c_char_p, []),
+    "compas_clear_error": (None, []),
+    "compas_array_info": (_i, [_vp, C.POINTER(C.c_int), _i64p]),
+    "compas_magnetization_to_signal_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _i]),
+    "compas_compute_jacobian_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "compas_compute_jacobian_hermitian_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "compas_set_kernel_timing": (None, [_vp, _i]),
+    "compas_read_kernel_timings": (_i, [_vp, C.c_char_p, _i]),
+    "compas_trim_memory": (None, [_vp]),
+}
+
+REFERENCE_SYMBOLS = [name for name in _SIGNATURES if not name.startswith(("compas_last", "compas_clear",
+                                                                             "compas_array_info",
+                                                                             "compas_set_kernel",
+                                                                             "compas_read_kernel",
+                                                                             "compas_trim"))
+                     and not name.endswith("_method")]
+
+
+def library_path() -> str:
+    return os.environ.get("COMPAS_AMD_LIBRARY", LIB_PATH)
+
+
+def lib():
+    """Load libcompas_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise CompasError(f"{path} not found: build it with `python compas-toolkit_amd/build.py`")
+        handle = C.CDLL(path, mode=C.RTLD_GLOBAL)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(result=None):
+    err = lib().compas_last_error()
+    if err:
+        lib().compas_clear_error()
+        raise CompasError(err.decode(errors="replace"))
+    return result
+
+
+def version() -> str:
+    return lib().compas_version().decode()
+
+
+class Context:
+    """A compas context bound to one device (CompasToolkit.jl:56-91)."""
+
+    def __init__(self, ptr=None, device: int = 0):
+        if ptr is None:
+            ptr = _check(lib().compas_make_context())
+            if device != 0:
+                base = ptr
+                ptr = _check(lib().compas_copy_context_for_device(base, device))
+                lib().compas_destroy_context(base)
+        self.ptr = ptr
+
+    def with_device(self, device: int) -> "Context":
+        return Context(_check(lib().compas_copy_context_for_device(self.ptr, device)))
+
+    def synchronize(self):
+        lib().compas_synchronize(self.ptr)
+        _check()
+
+    def set_kernel_timing(self, enabled: bool):
+        lib().compas_set_kernel_timing(self.ptr, int(enabled))
+        _check()
+
+    def read_kernel_timings(self) -> dict:
+        n = lib().compas_read_kernel_timings(self.ptr, None, 0)
+        _check()
+        buf = C.create_string_buffer(max(n, 1))
+        lib().compas_read_kernel_timings(self.ptr, buf, len(buf))
+        _check()
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, count, total = line.rsplit(" ", 2)
+            out[name] = (int(count), float(total))
+        return out
+
+    def trim_memory(self):
+        lib().compas_trim_memory(self.ptr)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.compas_destroy_context(self.ptr)
+            self.ptr = None
+
+
+_CTX: Optional[Context] = None
+
+
+def init_context(device: int = 0) -> Context:
+    """CompasToolkit.init_context: create and install the current context."""
+    global _CTX
+    _CTX = Context(device=device)
+    return _CTX
+
+
+def get_context() -> Context:
+    if _CTX is None:
+        raise CompasError("COMPAS toolkit has not been initialized, use `init_context` before usage")
+    return _CTX
+
+
+class CompasArray:
+    """Device array owned by the library (CompasToolkit.jl:140-256)."""
+
+    def __init__(self, ptr, context: Context):
+        self.ptr = ptr
+        self.context = context
+        dtype = C.c_int(0)
+        sizes = (C.c_int64 * 3)()
+        rank = _check(lib().compas_array_info(ptr, C.byref(dtype), sizes))
+        self.dtype = np.complex64 if dtype.value == 1 else np.float32
+        self.shape = tuple(int(sizes[i]) for i in range(rank))
+
+    @classmethod
+    def from_numpy(cls, a, context: Optional[Context] = None) -> "CompasArray":
+        ctx = context or get_context()
+        a = np.asarray(a)
+        if np.iscomplexobj(a):
+            a = np.ascontiguousarray(a, dtype=np.complex64)
+            fn = lib().compas_make_array_complex
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            fn = lib().compas_make_array_float
+        sizes = (C.c_int64 * a.ndim)(*a.shape)
+        ptr = _check(fn(ctx.ptr, a.ctypes.data_as(C.c_void_p), a.ndim, sizes))
+        return cls(ptr, ctx)
+
+    def collect(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        n = out.size
+        if self.dtype == np.complex64:
+            lib().compas_read_array_complex(self.context.ptr, self.ptr, out.ctypes.data_as(C.c_void_p), n)
+        else:
+            lib().compas_read_array_float(self.context.ptr, self.ptr, out.ctypes.data_as(C.c_void_p), n)
+        _check()
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.compas_destroy_array(self.ptr)
+            self.ptr = None
+
+    def __repr__(self):
+        return f"CompasArray({np.dtype(self.dtype).name}, {self.shape})"
+
+
+def as_array(a, dtype=None, context: Optional[Context] = None) -> CompasArray:
+    if isinstance(a, CompasArray):
+        return a
+    if dtype is not None:
+        a = np.asarray(a, dtype=dtype)
+    return CompasArray.from_numpy(a, context)
+
+
+class _Object:
+    def __init__(self, ptr, keep=()):
+        self.ptr = ptr
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.compas_destroy_object(self.ptr)
+            self.ptr = None
+
+
+class TissueParameters(_Object):
+    """CompasToolkit.jl:437-505; every field is given (z defaults to zeros)."""
+
+    def __init__(self, nvoxels, T1, T2, B1, B0, rho_x, rho_y, x, y, z=None):
+        ctx = get_context()
+        fields = [T1, T2, B1, B0, rho_x, rho_y, x, y, np.zeros(nvoxels) if z is None else z]
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(f, dtype=np.float32), (nvoxels,))) for f in fields]
+        ptr = _check(lib().compas_make_tissue_parameters(ctx.ptr, int(nvoxels), *[a.ctypes.data_as(C.c_void_p) for a in arrs]))
+        super().__init__(ptr)
+        self.nvoxels = int(nvoxels)
+
+    @classmethod
+    def from_tissue(cls, t) -> "TissueParameters":
+        return cls(t.nvoxels, t.T1, t.T2, t.B1, t.B0, t.rho.real, t.rho.imag, t.x, t.y, t.z)
+
+
+class FispSequence(_Object):
+    """CompasToolkit.jl:542-575 (slice_profiles in C order: [nslices][nTR])."""
+
+    def __init__(self, RF_train, slice_profiles, TR, TE, max_state, TI, undersampling_factor=1, repetitions=1,
+                 TW=0.0):
+        rf = as_array(np.asarray(RF_train, dtype=np.complex64))
+        prof = as_array(np.asarray(slice_profiles, dtype=np.complex64).reshape(-1, rf.shape[0]))
+        ptr = _check(lib().compas_make_fisp_sequence(rf.ptr, prof.ptr, float(TR), float(TE), float(TW),
+                                                     int(max_state), float(TI), int(undersampling_factor),
+                                                     int(repetitions)))
+        super().__init__(ptr, keep=(rf, prof))
+        self.nreadouts = rf.shape[0] * int(undersampling_factor)
+
+    @classmethod
+    def from_sequence(cls, s) -> "FispSequence":
+        return cls(s.RF, s.profiles, s.TR, s.TE, s.max_state, s.TI, s.undersampling, s.repetitions, s.TW)
+
+
+class pSSFPSequence(_Object):
+    """CompasToolkit.jl:594-631 (z is passed as float32)."""
+
+    def __init__(self, RF_train, TR, gamma_dt_RF, dt, gamma_dt_GRz, z):
+        rf = as_array(np.asarray(RF_train, dtype=np.complex64))
+        g = as_array(np.asarray(gamma_dt_RF, dtype=np.complex64))
+        zz = as_array(np.asarray(z, dtype=np.float32))
+        ptr = _check(lib().compas_make_pssfp_sequence(rf.ptr, float(TR), g.ptr, *[float(t) for t in dt],
+                                                      *[float(t) for t in gamma_dt_GRz], zz.ptr))
+        super().__init__(ptr, keep=(rf, g, zz))
+        self.nreadouts = rf.shape[0]
+
+    @classmethod
+    def from_sequence(cls, s) -> "pSSFPSequence":
+        return cls(s.RF, s.TR, s.gamma_dt_RF, s.dt, s.gamma_dt_GRz, s.z)
+
+
+class CartesianTrajectory(_Object):
+    """CompasToolkit.jl:280-320 (real delta_k)."""
+
+    def __init__(self, nreadouts, samples_per_readout, delta_t, k_start, delta_k):
+        ks = as_array(np.asarray(k_start, dtype=np.complex64))
+        ptr = _check(lib().compas_make_cartesian_trajectory(int(nreadouts), int(samples_per_readout),
+                                                            float(delta_t), ks.ptr, float(np.real(delta_k))))
+        super().__init__(ptr, keep=(ks,))
+        self.nreadouts = int(nreadouts)
+        self.samples_per_readout = int(samples_per_readout)
+
+    @classmethod
+    def from_trajectory(cls, t) -> "CartesianTrajectory":
+        return cls(t.nreadouts, t.samples_per_readout, t.delta_t, t.k_start, t.delta_k)
+
+
+class SpiralTrajectory(_Object):
+    """CompasToolkit.jl:333-375."""
+
+    def __init__(self, nreadouts, samples_per_readout, delta_t, k_start, delta_k):
+        ks = as_array(np.asarray(k_start, dtype=np.complex64))
+        dk = as_array(np.asarray(delta_k, dtype=np.complex64))
+        ptr = _check(lib().compas_make_spiral_trajectory(int(nreadouts), int(samples_per_readout), float(delta_t),
+                                                         ks.ptr, dk.ptr))
+        super().__init__(ptr, keep=(ks, dk))
+        self.nreadouts = int(nreadouts)
+        self.samples_per_readout = int(samples_per_readout)
+
+    @classmethod
+    def from_trajectory(cls, t) -> "SpiralTrajectory":
+        return cls(t.nreadouts, t.samples_per_readout, t.delta_t, t.k_start, t.delta_k)
+
+
+def _wrap(ptr) -> CompasArray:
+    _check()
+    if not ptr:
+        raise CompasError("library returned no array")
+    return CompasArray(ptr, get_context())
+
+
+def simulate_magnetization(parameters: TissueParameters, sequence) -> CompasArray:
+    ctx = get_context()
+    if isinstance(sequence, FispSequence):
+        return _wrap(lib().compas_simulate_magnetization_fisp(ctx.ptr, parameters.ptr, sequence.ptr))
+    if isinstance(sequence, pSSFPSequence):
+        return _wrap(lib().compas_simulate_magnetization_pssfp(ctx.ptr, parameters.ptr, sequence.ptr))
+    raise TypeError("sequence must be a FispSequence or pSSFPSequence")
+
+
+def simulate_magnetization_derivative(field, echos, parameters, sequence, delta=1e-4) -> CompasArray:
+    ctx = get_context()
+    e = as_array(echos, np.complex64)
+    fn = (lib().compas_simulate_magnetization_derivative_fisp if isinstance(sequence, FispSequence)
+          else lib().compas_simulate_magnetization_derivative_pssfp)
+    return _wrap(fn(ctx.ptr, int(field), e.ptr, parameters.ptr, float(delta), sequence.ptr))
+
+
+def simulate_magnetization_derivatives(echos, parameters, sequence, delta=1e-4):
+    return {"T1": simulate_magnetization_derivative(T1, echos, parameters, sequence, delta),
+            "T2": simulate_magnetization_derivative(T2, echos, parameters, sequence, delta)}
+
+
+def magnetization_to_signal(echos, parameters, trajectory, coils, low_precision=False, method=None) -> CompasArray:
+    ctx = get_context()
+    e = as_array(echos, np.complex64)
+    c = as_array(coils, np.complex64)
+    if method is not None:
+        m = SIGNAL_METHODS[method] if isinstance(method, str) else int(method)
+        return _wrap(lib().compas_magnetization_to_signal_method(ctx.ptr, e.ptr, parameters.ptr, c.ptr,
+                                                                 trajectory.ptr, m))
+    nc = c.shape[0]
+    if isinstance(trajectory, CartesianTrajectory):
+        return _wrap(lib().compas_magnetization_to_signal_cartesian(ctx.ptr, nc, e.ptr, parameters.ptr, c.ptr,
+                                                                    trajectory.ptr, int(bool(low_precision))))
+    return _wrap(lib().compas_magnetization_to_signal_spiral(ctx.ptr, nc, e.ptr, parameters.ptr, c.ptr,
+                                                             trajectory.ptr))
+
+
+def compute_jacobian(echos, dechos, parameters, trajectory, coils, v, low_precision=False, method=None):
+    ctx = get_context()
+    e = as_array(echos, np.complex64)
+    d1 = as_array(dechos["T1"] if isinstance(dechos, dict) else dechos[0], np.complex64)
+    d2 = as_array(dechos["T2"] if isinstance(dechos, dict) else dechos[1], np.complex64)
+    c = as_array(coils, np.complex64)
+    vv = as_array(v, np.complex64)
+    if method is not None:
+        m = JACOBIAN_METHODS[method] if isinstance(method, str) else int(method)
+        return _wrap(lib().compas_compute_jacobian_method(ctx.ptr, e.ptr, d1.ptr, d2.ptr, parameters.ptr, c.ptr,
+                                                          trajectory.ptr, vv.ptr, m))
+    return _wrap(lib().compas_compute_jacobian(ctx.ptr, c.shape[0], e.ptr, d1.ptr, d2.ptr, parameters.ptr, c.ptr,
+                                               trajectory.ptr, vv.ptr, int(bool(low_precision))))
+
+
+def compute_jacobian_hermitian(echos, dechos, parameters, trajectory, coils, y, low_precision=False, method=None):
+    ctx = get_context()
+    e = as_array(echos, np.complex64)
+    d1 = as_array(dechos["T1"] if isinstance(dechos, dict) else dechos[0], np.complex64)
+    d2 = as_array(dechos["T2"] if isinstance(dechos, dict) else dechos[1], np.complex64)
+    c = as_array(coils, np.complex64)
+    yy = as_array(y, np.complex64)
+    if method is not None:
+        m = JACOBIAN_METHODS[method] if isinstance(method, str) else int(method)
+        return _wrap(lib().compas_compute_jacobian_hermitian_method(ctx.ptr, e.ptr, d1.ptr, d2.ptr, parameters.ptr,
+                                                                    trajectory.ptr, c.ptr, yy.ptr, m))
+    return _wrap(lib().compas_compute_jacobian_hermitian(ctx.ptr, c.shape[0], e.ptr, d1.ptr, d2.ptr, parameters.ptr,
+                                                         trajectory.ptr, c.ptr, yy.ptr, int(bool(low_precision))))
+
+
+def phase_encoding(echos, parameters, trajectory) -> CompasArray:
+    ctx = get_context()
+    e = as_array(echos, np.complex64)
+    return _wrap(lib().phase_encoding(ctx.ptr, e.ptr, parameters.ptr, trajectory.ptr))
+
+
+def compute_residual(lhs, rhs):
+    ctx = get_context()
+    a = as_array(lhs, np.complex64)
+    b = as_array(rhs, np.complex64)
+    obj = C.c_float(0.0)
+    diff = _wrap(lib().compas_compute_residual(ctx.ptr, a.ptr, b.ptr, C.byref(obj)))
+    return float(obj.value), diff

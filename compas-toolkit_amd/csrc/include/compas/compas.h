@@ -1,0 +1,310 @@
+#pragma once
+// compas:: C++ API of the simulation hot path.
+//
+// Names, argument order and meaning follow the reference's public headers:
+//   include/compas/parameters/tissue.h:16-67, tissue_view.cuh:13-26
+//   include/compas/sequences/fisp.h:10-60, pssfp.h:9-72
+//   include/compas/trajectories/{trajectory,cartesian,spiral,signal,phase_encoding}.h
+//   include/compas/simulate/{fisp,pssfp,derivative}.h
+//   include/compas/jacobian/{product,hermitian}.h
+//   include/compas/utils/residual.h
+// Every compute function enqueues work on the context's stream and returns an
+// Array whose contents are valid once the stream reaches that point
+// (synchronize(), or any copy_to()).
+
+#include <vector>
+
+#include "compas/core/complex.h"
+#include "compas/core/context.h"
+
+namespace compas {
+
+struct Object {
+    virtual ~Object() = default;
+};
+
+// ---------------------------------------------------------------- tissue --
+
+namespace TissueParameterField {
+enum {
+    T1 = 0,
+    T2 = 1,
+    B1 = 2,
+    B0 = 3,
+    RHO_X = 4,
+    RHO_Y = 5,
+    X = 6,
+    Y = 7,
+    Z = 8,
+    NUM_FIELDS = 9,
+};
+}
+
+// Structure-of-arrays [NUM_FIELDS][nvoxels] float32 in HBM: each field is a
+// contiguous row, so kernels read every field with coalesced loads.
+struct TissueParameters: Object {
+    Array<float, 2> data;
+    int nvoxels = 0;
+    int chunk_size = 0;
+    bool has_z = true;
+    bool has_b0 = true;
+    bool has_b1 = true;
+
+    TissueParameters() = default;
+    TissueParameters(Array<float, 2> d, bool has_z, bool has_b0, bool has_b1) :
+        data(d),
+        nvoxels(int(d.size(1))),
+        chunk_size(int(d.size(1))),
+        has_z(has_z),
+        has_b0(has_b0),
+        has_b1(has_b1) {}
+};
+
+TissueParameters make_tissue_parameters(
+    const CompasContext& ctx,
+    int num_voxels,
+    int chunk_size,
+    View<float> T1,
+    View<float> T2,
+    View<float> B1,
+    View<float> B0,
+    View<float> rho_x,
+    View<float> rho_y,
+    View<float> x,
+    View<float> y,
+    View<float> z = {});
+
+// ------------------------------------------------------------- sequences --
+
+struct FISPSequence: Object {
+    Array<cfloat> RF_train;  // [nTR], |RF| = flip angle in degrees, arg = phase
+    Array<cfloat, 2> sliceprofiles;  // [nslices][nTR]
+    float TR = 0;
+    float TE = 0;
+    float TW = 0;
+    int max_state = 0;
+    float TI = 0;
+    int undersampling_factor = 1;
+    int repetitions = 1;
+    bool inversion_prepulse = true;
+    bool wait_spoiling = true;
+
+    FISPSequence() = default;
+    FISPSequence(
+        Array<cfloat> RF,
+        Array<cfloat, 2> profiles,
+        float TR,
+        float TE,
+        float TW,
+        int max_state,
+        float TI,
+        int undersampling_factor = 1,
+        int repetitions = 1,
+        bool inversion_prepulse = true,
+        bool wait_spoiling = true) :
+        RF_train(RF),
+        sliceprofiles(profiles),
+        TR(TR),
+        TE(TE),
+        TW(TW),
+        max_state(max_state),
+        TI(TI),
+        undersampling_factor(undersampling_factor),
+        repetitions(repetitions),
+        inversion_prepulse(inversion_prepulse),
+        wait_spoiling(wait_spoiling) {}
+};
+
+FISPSequence make_fisp_sequence(
+    const CompasContext& context,
+    View<cfloat> RF_train,
+    View<cfloat, 2> sliceprofiles,
+    float TR,
+    float TE,
+    int max_state,
+    float TI);
+
+struct RepetitionData {
+    float ex;
+    float inv;
+    float pr;
+};
+
+struct pSSFPSequence: Object {
+    Array<cfloat> RF_train;  // [nTR]
+    float TR = 0;  // stored, not used by the simulation (as in the reference)
+    int nRF = 0;
+    int nTR = 0;
+    Array<cfloat> gamma_dt_RF;  // [nRF], waveform normalised to 1 degree
+    RepetitionData dt {};
+    RepetitionData gamma_dt_GRz {};
+    int nz = 0;
+    Array<float> z;  // [nz] slice positions
+
+    pSSFPSequence() = default;
+    pSSFPSequence(
+        Array<cfloat> RF_train,
+        float TR,
+        Array<cfloat> gamma_dt_RF,
+        RepetitionData dt,
+        RepetitionData gamma_dt_GRz,
+        Array<float> z) :
+        RF_train(RF_train),
+        TR(TR),
+        nRF(int(gamma_dt_RF.size(0))),
+        nTR(int(RF_train.size(0))),
+        gamma_dt_RF(gamma_dt_RF),
+        dt(dt),
+        gamma_dt_GRz(gamma_dt_GRz),
+        nz(int(z.size(0))),
+        z(z) {}
+};
+
+pSSFPSequence make_pssfp_sequence(
+    const CompasContext& context,
+    View<cfloat> RF_train,
+    float TR,
+    View<cfloat> gamma_dt_RF,
+    RepetitionData dt,
+    RepetitionData gamma_dt_GRz,
+    View<float> z);
+
+// ---------------------------------------------------------- trajectories --
+
+struct Trajectory: Object {
+    int nreadouts = 0;
+    int samples_per_readout = 0;
+    float delta_t = 0;
+
+    Trajectory(int nr, int ns, float dt) : nreadouts(nr), samples_per_readout(ns), delta_t(dt) {}
+};
+
+struct CartesianTrajectory: Trajectory {
+    Array<cfloat> k_start;  // [nreadouts]; used by phase_encoding only
+    cfloat delta_k;
+
+    CartesianTrajectory(int nr, int ns, float dt, Array<cfloat> k_start, cfloat delta_k) :
+        Trajectory(nr, ns, dt),
+        k_start(k_start),
+        delta_k(delta_k) {}
+};
+
+struct SpiralTrajectory: Trajectory {
+    Array<cfloat> k_start;  // [nreadouts]
+    Array<cfloat> delta_k;  // [nreadouts]
+
+    SpiralTrajectory(int nr, int ns, float dt, Array<cfloat> k_start, Array<cfloat> delta_k) :
+        Trajectory(nr, ns, dt),
+        k_start(k_start),
+        delta_k(delta_k) {}
+};
+
+CartesianTrajectory make_cartesian_trajectory(
+    const CompasContext& context,
+    int nreadouts,
+    int samples_per_readout,
+    float delta_t,
+    HostView<cfloat> k_start,
+    cfloat delta_k);
+
+SpiralTrajectory make_spiral_trajectory(
+    const CompasContext& context,
+    int nreadouts,
+    int samples_per_readout,
+    float delta_t,
+    View<cfloat> k_start,
+    View<cfloat> delta_k);
+
+// -------------------------------------------------------------- simulate --
+
+Array<cfloat, 2> simulate_magnetization(
+    const CompasContext& context,
+    TissueParameters parameters,
+    FISPSequence sequence);
+
+Array<cfloat, 2> simulate_magnetization(
+    const CompasContext& context,
+    TissueParameters parameters,
+    pSSFPSequence sequence);
+
+static constexpr float DEFAULT_FINITE_DIFFERENCE_DELTA = 1e-4F;
+
+Array<cfloat, 2> simulate_magnetization_derivative(
+    const CompasContext& context,
+    int field,
+    Array<cfloat, 2> echos,
+    TissueParameters parameters,
+    FISPSequence sequence,
+    float delta = DEFAULT_FINITE_DIFFERENCE_DELTA);
+
+Array<cfloat, 2> simulate_magnetization_derivative(
+    const CompasContext& context,
+    int field,
+    Array<cfloat, 2> echos,
+    TissueParameters parameters,
+    pSSFPSequence sequence,
+    float delta = DEFAULT_FINITE_DIFFERENCE_DELTA);
+
+// ---------------------------------------------------------------- signal --
+
+// Same enumerators as the reference (signal.h:14-44).  On MI355X:
+//   Naive          one output per thread, voxel loop (reference semantics check)
+//   Direct         FP32 VALU contraction (the plain, MFMA-free path)
+//   MatmulPedantic FP16 hi+lo split operands on MFMA, FP32 accumulate, scaled
+//   Matmul         BF16 hi+lo split operands on MFMA (3 products)
+//   MatmulFast     same as Matmul (default)
+//   MatmulLow      BF16 single product on MFMA (low accuracy, fastest)
+enum struct SimulateSignalMethod { Naive, Direct, MatmulPedantic, Matmul, MatmulFast, MatmulLow };
+
+Array<cfloat, 3> magnetization_to_signal(
+    const CompasContext& context,
+    Array<cfloat, 2> echos,
+    TissueParameters parameters,
+    const Trajectory& trajectory,
+    Array<cfloat, 2> coil_sensitivities,
+    SimulateSignalMethod method = SimulateSignalMethod::MatmulFast);
+
+Array<cfloat, 2> phase_encoding(
+    const CompasContext& ctx,
+    const Array<cfloat, 2>& echos,
+    const TissueParameters& parameters,
+    const CartesianTrajectory& trajectory);
+
+// -------------------------------------------------------------- jacobian --
+
+// Same enumerators as the reference (product.h:10).  Naive is the per-output
+// loop; Direct and Gemm run the FP32 VALU contraction; GemmFast the BF16
+// hi+lo MFMA contraction; GemmLow single-product BF16.
+enum struct JacobianComputeMethod { Naive, Direct, Gemm, GemmFast, GemmLow };
+
+Array<cfloat, 3> compute_jacobian(
+    const CompasContext& ctx,
+    Array<cfloat, 2> echos,
+    Array<cfloat, 2> delta_echos_T1,
+    Array<cfloat, 2> delta_echos_T2,
+    TissueParameters parameters,
+    CartesianTrajectory trajectory,
+    Array<cfloat, 2> coil_sensitivities,
+    Array<cfloat, 2> vector,
+    JacobianComputeMethod kind = JacobianComputeMethod::Direct);
+
+Array<cfloat, 2> compute_jacobian_hermitian(
+    const CompasContext& ctx,
+    Array<cfloat, 2> echos,
+    Array<cfloat, 2> delta_echos_T1,
+    Array<cfloat, 2> delta_echos_T2,
+    TissueParameters parameters,
+    CartesianTrajectory trajectory,
+    Array<cfloat, 2> coil_sensitivities,
+    Array<cfloat, 3> vector,
+    JacobianComputeMethod kind = JacobianComputeMethod::Direct);
+
+// ---------------------------------------------------------------- utils --
+
+Array<cfloat, 3> compute_residual(
+    const CompasContext& ctx,
+    Array<cfloat, 3> lhs,
+    Array<cfloat, 3> rhs,
+    float* objective_out = nullptr);
+
+}  // namespace compas

@@ -1,0 +1,75 @@
+#pragma once
+// Device helpers shared by the HIP kernels (gfx950, wave64).
+
+#include <hip/hip_runtime.h>
+
+#include "compas/compas.h"
+#include "compas/core/complex.h"
+
+namespace compas {
+namespace dev {
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+constexpr float TWO_PI_F = 6.2831853071795864769f;
+constexpr float DEG2RAD_F = 3.14159265358979323846f / 180.0f;
+
+// Structure-of-arrays tissue map [NUM_FIELDS][nvoxels]; the defaults for absent
+// B1 / B0 / z follow TissueParametersView::get (tissue_view.cuh:53-71).
+struct TissueView {
+    const float* data;
+    int nvoxels;
+    bool has_z, has_b0, has_b1;
+
+    __device__ float field(int f, int v) const {
+        return data[size_t(f) * size_t(nvoxels) + size_t(v)];
+    }
+    __device__ float T1(int v) const { return field(TissueParameterField::T1, v); }
+    __device__ float T2(int v) const { return field(TissueParameterField::T2, v); }
+    __device__ float B1(int v) const { return has_b1 ? field(TissueParameterField::B1, v) : 1.0f; }
+    __device__ float B0(int v) const { return has_b0 ? field(TissueParameterField::B0, v) : 0.0f; }
+    __device__ cfloat rho(int v) const {
+        return {field(TissueParameterField::RHO_X, v), field(TissueParameterField::RHO_Y, v)};
+    }
+    __device__ float x(int v) const { return field(TissueParameterField::X, v); }
+    __device__ float y(int v) const { return field(TissueParameterField::Y, v); }
+    __device__ float z(int v) const { return has_z ? field(TissueParameterField::Z, v) : 0.0f; }
+};
+
+inline TissueView tissue_view(const TissueParameters& p) {
+    return {p.data.data(), p.nvoxels, p.has_z, p.has_b0, p.has_b1};
+}
+
+// Reduce an FP64 phase to [-pi, pi) and return it as float.  The argument is
+// formed in FP64 from the float32 inputs, so large k.r products (hundreds of
+// radians) keep full precision before the float sincos.
+__device__ inline float reduce_phase(double phi) {
+    double k = rint(phi * (1.0 / TWO_PI));
+    return float(fma(-k, TWO_PI, phi));
+}
+
+__device__ inline cfloat polar_f(float mag, float phase) {
+    float s, c;
+    sincosf(phase, &s, &c);
+    return {mag * c, mag * s};
+}
+
+__device__ inline cfloat cmul(cfloat a, cfloat b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// a + b * c  with fused multiply-adds
+__device__ inline cfloat cfma(cfloat b, cfloat c, cfloat a) {
+    return {fmaf(b.re, c.re, fmaf(-b.im, c.im, a.re)), fmaf(b.re, c.im, fmaf(b.im, c.re, a.im))};
+}
+
+// a + b * conj(c)
+__device__ inline cfloat cfma_conj(cfloat b, cfloat c, cfloat a) {
+    return {fmaf(b.re, c.re, fmaf(b.im, c.im, a.re)), fmaf(b.im, c.re, fmaf(-b.re, c.im, a.im))};
+}
+
+inline unsigned div_ceil(long long a, long long b) {
+    return unsigned((a + b - 1) / b);
+}
+
+}  // namespace dev
+}  // namespace compas

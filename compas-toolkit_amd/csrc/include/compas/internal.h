@@ -1,0 +1,48 @@
+#pragma once
+// Internal building blocks shared by the compute entry points.
+
+#include <algorithm>
+#include <cstdint>
+
+#include "compas/compas.h"
+
+namespace compas {
+
+// dst[i] += src[i] for i < n (complex)
+void add_into(const CompasContext& ctx, cfloat* dst, const cfloat* src, int64_t n);
+
+// Operand precision of the complex contraction engine.
+enum struct ContractionPrecision {
+    FP32,  // FP32 VALU (no MFMA)
+    BF16x3,  // BF16 hi+lo split, 3 MFMA products, FP32 accumulate
+    FP16x3,  // FP16 hi+lo split with per-row power-of-two scaling, 3 MFMA products
+    BF16x1,  // BF16 single product
+};
+
+// Complex contraction C = A * B^T with both operands K-contiguous:
+//   C[m][n] = sum_k A[m*lda + k] * B[n*ldb + k]      (complex FP32 operands)
+// Output element (m, n) is written to
+//   out[(n / nsub) * out_outer + m * nsub + (n % nsub)]
+// which covers both plain row-major C (nsub = N) and coil-major signal layouts
+// ([nc][nr][ns] with n = c*ns + s, nsub = ns, out_outer = M*ns).  Partial sums
+// over K splits are reduced in FP64.  `accumulate` adds into `out`.
+struct ContractionShape {
+    int M, N, K;
+    int64_t lda, ldb;
+    int nsub;
+    int64_t out_outer;
+    bool accumulate;
+};
+
+void complex_contraction(
+    const CompasContext& ctx,
+    const cfloat* A,
+    const cfloat* B,
+    cfloat* out,
+    ContractionShape shape,
+    ContractionPrecision precision);
+
+ContractionPrecision signal_precision(SimulateSignalMethod method);
+ContractionPrecision jacobian_precision(JacobianComputeMethod method);
+
+}  // namespace compas

@@ -1,0 +1,503 @@
+// C-ABI of libcompas_amd.so (declared in include/compas_amd.h).
+//
+// Mirrors the reference's julia-bindings/src/bindings.cpp:19-411 entry point
+// by entry point.  Handle objects are type-checked on every call (the
+// reference casts blindly), and errors go through jl_errorf when the host is
+// Julia, or through a per-thread message otherwise.
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <typeinfo>
+
+#include "compas/compas.h"
+#include "compas_amd.h"
+
+// Provided by libjulia when loaded from CompasToolkit.jl; absent otherwise.
+extern "C" void jl_errorf(const char* fmt, ...) __attribute__((weak, noreturn));
+
+namespace {
+
+using namespace compas;
+
+thread_local std::string g_last_error;
+
+struct ContextHandle {
+    CompasContext ctx;
+};
+
+struct Handle {
+    virtual ~Handle() = default;
+    virtual const char* name() const = 0;
+};
+
+template<typename T>
+struct ObjectHandle: Handle, T {
+    template<typename... Args>
+    explicit ObjectHandle(Args&&... args) : T(std::forward<Args>(args)...) {}
+    const char* name() const override { return typeid(T).name(); }
+};
+
+enum DType { DT_FLOAT = 0, DT_COMPLEX = 1 };
+
+struct ArrayHandle: Handle {
+    std::shared_ptr<Buffer> buf;
+    int dtype = DT_FLOAT;
+    int rank = 0;
+    int64_t sizes[3] = {0, 0, 0};
+
+    const char* name() const override { return dtype == DT_FLOAT ? "Array<float>" : "Array<cfloat>"; }
+
+    int64_t numel() const {
+        int64_t n = 1;
+        for (int i = 0; i < rank; i++) n *= sizes[i];
+        return n;
+    }
+
+    template<typename T, size_t N>
+    Array<T, N> as(const char* what) const {
+        const int want = std::is_same<T, cfloat>::value ? DT_COMPLEX : DT_FLOAT;
+        if (dtype != want || rank != int(N)) {
+            std::stringstream ss;
+            ss << "invalid array for `" << what << "`: expecting " << (want == DT_COMPLEX ? "complex" : "float")
+               << " rank " << N << ", gotten " << name() << " rank " << rank;
+            throw CompasException(ss.str());
+        }
+        std::array<int64_t, N> s {};
+        for (size_t i = 0; i < N; i++) s[i] = sizes[i];
+        return Array<T, N>(buf, s);
+    }
+
+    template<typename T, size_t N>
+    static ArrayHandle* wrap(const Array<T, N>& a) {
+        auto* h = new ArrayHandle();
+        h->buf = a.buffer();
+        h->dtype = std::is_same<T, cfloat>::value ? DT_COMPLEX : DT_FLOAT;
+        h->rank = int(N);
+        for (size_t i = 0; i < N; i++) h->sizes[i] = a.size(i);
+        return h;
+    }
+};
+
+template<typename T>
+const T& unwrap(const void* p, const char* what) {
+    if (p == nullptr) throw CompasException(std::string("null handle passed as `") + what + "`");
+    const auto* h = static_cast<const Handle*>(p);
+    if (const auto* t = dynamic_cast<const T*>(h)) return *t;
+    std::stringstream ss;
+    ss << "invalid object for `" << what << "` (expecting: " << typeid(T).name() << ", gotten: " << h->name() << ")";
+    throw CompasException(ss.str());
+}
+
+const ArrayHandle& array(const void* p, const char* what) {
+    return unwrap<ArrayHandle>(p, what);
+}
+
+const CompasContext& context(const void* p) {
+    if (p == nullptr) throw CompasException("null context");
+    return static_cast<const ContextHandle*>(p)->ctx;
+}
+
+void report(const char* msg) {
+    if (jl_errorf) jl_errorf("COMPAS error occurred: %s\n", msg);
+    g_last_error = msg;
+}
+
+template<typename R>
+R default_value() {
+    if constexpr (std::is_void<R>::value) {
+        return;
+    } else {
+        return R {};
+    }
+}
+
+template<typename F>
+auto catch_exceptions(F fun) -> decltype(fun()) {
+    using R = decltype(fun());
+    try {
+        g_last_error.clear();
+        return fun();
+    } catch (const std::exception& e) {
+        report(e.what());
+    } catch (...) {
+        report("unknown exception");
+    }
+    return default_value<R>();
+}
+
+template<typename T>
+CompasObjectHandle* new_object(T&& value) {
+    auto* h = new ObjectHandle<std::decay_t<T>>(std::forward<T>(value));
+    return reinterpret_cast<CompasObjectHandle*>(static_cast<Handle*>(h));
+}
+
+template<typename T, size_t N>
+CompasArrayHandle* new_array(const Array<T, N>& a) {
+    return reinterpret_cast<CompasArrayHandle*>(static_cast<Handle*>(ArrayHandle::wrap(a)));
+}
+
+const CompasContextHandle* new_context(const CompasContext& ctx) {
+    return reinterpret_cast<const CompasContextHandle*>(new ContextHandle {ctx});
+}
+
+View<float> host_view(const float* p, int n) {
+    if (p == nullptr) return {};
+    return View<float>(p, {n});
+}
+
+template<typename T>
+const ArrayHandle* make_array_impl(const CompasContext& ctx, const T* data, int rank, const int64_t* sizes) {
+    if (rank < 1 || rank > 3) COMPAS_ERROR("cannot support rank > 3");  // bindings.cpp:63-64
+    COMPAS_CHECK(sizes != nullptr);
+    for (int i = 0; i < rank; i++) COMPAS_CHECK(sizes[i] >= 0);
+    if (rank == 1) return ArrayHandle::wrap(ctx.allocate_from<T, 1>(data, {sizes[0]}));
+    if (rank == 2) return ArrayHandle::wrap(ctx.allocate_from<T, 2>(data, {sizes[0], sizes[1]}));
+    return ArrayHandle::wrap(ctx.allocate_from<T, 3>(data, {sizes[0], sizes[1], sizes[2]}));
+}
+
+void read_array_impl(const ArrayHandle& a, void* dest, int64_t length, size_t elem, int dtype) {
+    if (a.dtype != dtype) throw CompasException("array element type does not match the read function");
+    COMPAS_CHECK(length >= 0 && length <= a.numel());
+    if (length == 0) return;
+    COMPAS_CHECK(dest != nullptr);
+    copy_device_to_host(a.buf, dest, size_t(length) * elem);
+}
+
+SimulateSignalMethod signal_method(int m) {
+    if (m < 0 || m > 5) COMPAS_ERROR("invalid value for `SimulateSignalMethod`");
+    return SimulateSignalMethod(m);
+}
+
+JacobianComputeMethod jacobian_method(int m) {
+    if (m < 0 || m > 4) COMPAS_ERROR("invalid value for `JacobianComputeMethod`");
+    return JacobianComputeMethod(m);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* compas_version() {
+    return "compas-amd 0.1.0 (gfx950)";
+}
+
+const CompasContextHandle* compas_make_context() {
+    return catch_exceptions([&] { return new_context(make_context(0)); });
+}
+
+const CompasContextHandle* compas_copy_context_for_device(CompasContextHandle* ctx, int device) {
+    return catch_exceptions([&] { return new_context(context(ctx).with_device(device)); });
+}
+
+void compas_destroy_context(const CompasContextHandle* ctx) {
+    catch_exceptions([&] { delete reinterpret_cast<const ContextHandle*>(ctx); });
+}
+
+void compas_destroy_object(const CompasObjectHandle* obj) {
+    catch_exceptions([&] { delete reinterpret_cast<const Handle*>(obj); });
+}
+
+void compas_synchronize(const CompasContextHandle* ctx) {
+    catch_exceptions([&] { context(ctx).synchronize(); });
+}
+
+const CompasArrayHandle* compas_make_array_float(
+    const CompasContextHandle* ctx, const float* data, int rank, int64_t* sizes) {
+    return catch_exceptions([&] {
+        return reinterpret_cast<const CompasArrayHandle*>(
+            static_cast<const Handle*>(make_array_impl<float>(context(ctx), data, rank, sizes)));
+    });
+}
+
+void compas_read_array_float(const CompasContextHandle* ctx, const CompasArrayHandle* a, float* dest, int64_t length) {
+    catch_exceptions([&] {
+        (void)context(ctx);
+        read_array_impl(array(a, "array"), dest, length, sizeof(float), DT_FLOAT);
+    });
+}
+
+const CompasArrayHandle* compas_make_array_complex(
+    const CompasContextHandle* ctx, const compas_cfloat* data, int rank, int64_t* sizes) {
+    return catch_exceptions([&] {
+        return reinterpret_cast<const CompasArrayHandle*>(static_cast<const Handle*>(
+            make_array_impl<cfloat>(context(ctx), reinterpret_cast<const cfloat*>(data), rank, sizes)));
+    });
+}
+
+void compas_read_array_complex(
+    const CompasContextHandle* ctx, const CompasArrayHandle* a, compas_cfloat* dest, int64_t length) {
+    catch_exceptions([&] {
+        (void)context(ctx);
+        read_array_impl(array(a, "array"), dest, length, sizeof(cfloat), DT_COMPLEX);
+    });
+}
+
+void compas_destroy_array(const CompasArrayHandle* a) {
+    catch_exceptions([&] { delete reinterpret_cast<const Handle*>(a); });
+}
+
+const CompasObjectHandle* compas_make_tissue_parameters(
+    const CompasContextHandle* ctx, int nvoxels, const float* T1, const float* T2, const float* B1,
+    const float* B0, const float* rho_x, const float* rho_y, const float* x, const float* y, const float* z) {
+    return catch_exceptions([&]() -> const CompasObjectHandle* {
+        COMPAS_CHECK(nvoxels >= 0);
+        // bindings.cpp:131 -- one chunk holding every voxel
+        return new_object(make_tissue_parameters(
+            context(ctx), nvoxels, nvoxels, host_view(T1, nvoxels), host_view(T2, nvoxels), host_view(B1, nvoxels),
+            host_view(B0, nvoxels), host_view(rho_x, nvoxels), host_view(rho_y, nvoxels), host_view(x, nvoxels),
+            host_view(y, nvoxels), host_view(z, nvoxels)));
+    });
+}
+
+CompasObjectHandle* compas_make_fisp_sequence(
+    const CompasArrayHandle* RF_train, const CompasArrayHandle* sliceprofiles, float TR, float TE, float TW,
+    int max_state, float TI, int undersampling_factor, int repetitions) {
+    return catch_exceptions([&] {
+        auto rf = array(RF_train, "RF_train").as<cfloat, 1>("RF_train");
+        auto prof = array(sliceprofiles, "sliceprofiles").as<cfloat, 2>("sliceprofiles");
+        return new_object(FISPSequence(rf, prof, TR, TE, TW, max_state, TI, undersampling_factor, repetitions));
+    });
+}
+
+CompasObjectHandle* compas_make_pssfp_sequence(
+    const CompasArrayHandle* RF_train, float TR, const CompasArrayHandle* gamma_dt_RF, float dt_ex, float dt_inv,
+    float dt_pr, float g_ex, float g_inv, float g_pr, const CompasArrayHandle* z) {
+    return catch_exceptions([&] {
+        auto rf = array(RF_train, "RF_train").as<cfloat, 1>("RF_train");
+        auto g = array(gamma_dt_RF, "gamma_dt_RF").as<cfloat, 1>("gamma_dt_RF");
+        auto zz = array(z, "z").as<float, 1>("z");
+        COMPAS_CHECK(rf.size(0) > 0);
+        return new_object(pSSFPSequence(rf, TR, g, RepetitionData {dt_ex, dt_inv, dt_pr},
+                                        RepetitionData {g_ex, g_inv, g_pr}, zz));
+    });
+}
+
+CompasObjectHandle* compas_make_cartesian_trajectory(
+    int nreadouts, int samples_per_readout, float delta_t, CompasArrayHandle* k_start, float delta_k) {
+    return catch_exceptions([&] {
+        auto ks = array(k_start, "k_start").as<cfloat, 1>("k_start");
+        COMPAS_CHECK(ks.size(0) == nreadouts);
+        return new_object(CartesianTrajectory(nreadouts, samples_per_readout, delta_t, ks, cfloat(delta_k, 0.0f)));
+    });
+}
+
+CompasObjectHandle* compas_make_spiral_trajectory(
+    int nreadouts, int samples_per_readout, float delta_t, CompasArrayHandle* k_start, CompasArrayHandle* delta_k) {
+    return catch_exceptions([&] {
+        auto ks = array(k_start, "k_start").as<cfloat, 1>("k_start");
+        auto dk = array(delta_k, "delta_k").as<cfloat, 1>("delta_k");
+        COMPAS_CHECK(ks.size(0) == nreadouts && dk.size(0) == nreadouts);
+        return new_object(SpiralTrajectory(nreadouts, samples_per_readout, delta_t, ks, dk));
+    });
+}
+
+CompasArrayHandle* compas_simulate_magnetization_fisp(
+    const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence) {
+    return catch_exceptions([&] {
+        return new_array(simulate_magnetization(
+            context(ctx), unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<FISPSequence>(sequence, "sequence")));
+    });
+}
+
+CompasArrayHandle* compas_simulate_magnetization_pssfp(
+    const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence) {
+    return catch_exceptions([&] {
+        return new_array(simulate_magnetization(
+            context(ctx), unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<pSSFPSequence>(sequence, "sequence")));
+    });
+}
+
+CompasArrayHandle* compas_magnetization_to_signal_cartesian(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasArrayHandle* coils, const CompasObjectHandle* trajectory, int32_t low_precision) {
+    (void)ncoils;  // unused, as in the reference (the coil count comes from `coils`)
+    return catch_exceptions([&] {
+        auto method = low_precision == 0 ? SimulateSignalMethod::MatmulFast : SimulateSignalMethod::MatmulLow;
+        return new_array(magnetization_to_signal(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<CartesianTrajectory>(trajectory, "trajectory"), array(coils, "coils").as<cfloat, 2>("coils"),
+            method));
+    });
+}
+
+CompasArrayHandle* compas_magnetization_to_signal_spiral(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasArrayHandle* coils, const CompasObjectHandle* trajectory) {
+    (void)ncoils;
+    return catch_exceptions([&] {
+        return new_array(magnetization_to_signal(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            unwrap<TissueParameters>(parameters, "parameters"), unwrap<SpiralTrajectory>(trajectory, "trajectory"),
+            array(coils, "coils").as<cfloat, 2>("coils")));
+    });
+}
+
+CompasArrayHandle* compas_compute_jacobian(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasObjectHandle* parameters, const CompasArrayHandle* coils,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* vector, int32_t low_precision) {
+    (void)ncoils;
+    return catch_exceptions([&] {
+        auto method = low_precision == 0 ? JacobianComputeMethod::GemmFast : JacobianComputeMethod::GemmLow;
+        return new_array(compute_jacobian(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
+            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
+            unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<CartesianTrajectory>(trajectory, "trajectory"), array(coils, "coils").as<cfloat, 2>("coils"),
+            array(vector, "vector").as<cfloat, 2>("vector"), method));
+    });
+}
+
+CompasArrayHandle* compas_compute_jacobian_hermitian(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory,
+    const CompasArrayHandle* coils, const CompasArrayHandle* vector, int32_t low_precision) {
+    (void)ncoils;
+    return catch_exceptions([&] {
+        auto method = low_precision == 0 ? JacobianComputeMethod::GemmFast : JacobianComputeMethod::GemmLow;
+        return new_array(compute_jacobian_hermitian(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
+            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
+            unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<CartesianTrajectory>(trajectory, "trajectory"), array(coils, "coils").as<cfloat, 2>("coils"),
+            array(vector, "vector").as<cfloat, 3>("vector"), method));
+    });
+}
+
+CompasArrayHandle* phase_encoding(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* trajectory) {
+    return catch_exceptions([&] {
+        return new_array(compas::phase_encoding(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<CartesianTrajectory>(trajectory, "trajectory")));
+    });
+}
+
+CompasArrayHandle* compas_compute_residual(
+    const CompasContextHandle* ctx, const CompasArrayHandle* lhs, const CompasArrayHandle* rhs, float* sum) {
+    return catch_exceptions([&] {
+        return new_array(compute_residual(
+            context(ctx), array(lhs, "lhs").as<cfloat, 3>("lhs"), array(rhs, "rhs").as<cfloat, 3>("rhs"), sum));
+    });
+}
+
+CompasArrayHandle* compas_simulate_magnetization_derivative_pssfp(
+    const CompasContextHandle* ctx, int field, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    float delta, const CompasObjectHandle* sequence) {
+    return catch_exceptions([&] {
+        return new_array(simulate_magnetization_derivative(
+            context(ctx), field, array(echos, "echos").as<cfloat, 2>("echos"),
+            unwrap<TissueParameters>(parameters, "parameters"), unwrap<pSSFPSequence>(sequence, "sequence"), delta));
+    });
+}
+
+CompasArrayHandle* compas_simulate_magnetization_derivative_fisp(
+    const CompasContextHandle* ctx, int field, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    float delta, const CompasObjectHandle* sequence) {
+    return catch_exceptions([&] {
+        return new_array(simulate_magnetization_derivative(
+            context(ctx), field, array(echos, "echos").as<cfloat, 2>("echos"),
+            unwrap<TissueParameters>(parameters, "parameters"), unwrap<FISPSequence>(sequence, "sequence"), delta));
+    });
+}
+
+// ---- extensions ----
+
+const char* compas_last_error() {
+    return g_last_error.empty() ? nullptr : g_last_error.c_str();
+}
+
+void compas_clear_error() {
+    g_last_error.clear();
+}
+
+int compas_array_info(const CompasArrayHandle* a, int* dtype, int64_t* sizes3) {
+    return catch_exceptions([&] {
+        const auto& h = array(a, "array");
+        if (dtype) *dtype = h.dtype;
+        if (sizes3)
+            for (int i = 0; i < 3; i++) sizes3[i] = i < h.rank ? h.sizes[i] : 0;
+        return h.rank;
+    });
+}
+
+CompasArrayHandle* compas_magnetization_to_signal_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasArrayHandle* coils, const CompasObjectHandle* trajectory, int method) {
+    return catch_exceptions([&] {
+        return new_array(magnetization_to_signal(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            unwrap<TissueParameters>(parameters, "parameters"), unwrap<Trajectory>(trajectory, "trajectory"),
+            array(coils, "coils").as<cfloat, 2>("coils"), signal_method(method)));
+    });
+}
+
+CompasArrayHandle* compas_compute_jacobian_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasObjectHandle* parameters, const CompasArrayHandle* coils,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* vector, int method) {
+    return catch_exceptions([&] {
+        return new_array(compute_jacobian(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
+            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
+            unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<CartesianTrajectory>(trajectory, "trajectory"), array(coils, "coils").as<cfloat, 2>("coils"),
+            array(vector, "vector").as<cfloat, 2>("vector"), jacobian_method(method)));
+    });
+}
+
+CompasArrayHandle* compas_compute_jacobian_hermitian_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory,
+    const CompasArrayHandle* coils, const CompasArrayHandle* vector, int method) {
+    return catch_exceptions([&] {
+        return new_array(compute_jacobian_hermitian(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
+            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
+            unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<CartesianTrajectory>(trajectory, "trajectory"), array(coils, "coils").as<cfloat, 2>("coils"),
+            array(vector, "vector").as<cfloat, 3>("vector"), jacobian_method(method)));
+    });
+}
+
+void compas_set_kernel_timing(const CompasContextHandle* ctx, int enabled) {
+    catch_exceptions([&] { context(ctx).runtime()->set_timing(enabled != 0); });
+}
+
+int compas_read_kernel_timings(const CompasContextHandle* ctx, char* buf, int buflen) {
+    return catch_exceptions([&] {
+        auto stats = context(ctx).runtime()->collect_timings();
+        std::stringstream ss;
+        for (auto& kv : stats) {
+            char line[256];
+            std::snprintf(line, sizeof(line), "%s %lld %.6f\n", kv.first.c_str(), (long long)kv.second.count,
+                          kv.second.total_ms);
+            ss << line;
+        }
+        std::string s = ss.str();
+        if (buf && buflen > 0) {
+            size_t n = std::min(size_t(buflen - 1), s.size());
+            std::memcpy(buf, s.data(), n);
+            buf[n] = '\0';
+        }
+        return int(s.size() + 1);
+    });
+}
+
+void compas_trim_memory(const CompasContextHandle* ctx) {
+    catch_exceptions([&] { context(ctx).runtime()->trim(); });
+}
+
+}  // extern "C"

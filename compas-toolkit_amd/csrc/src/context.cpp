@@ -1,0 +1,215 @@
+#include "compas/core/context.h"
+
+#include <sstream>
+
+namespace compas {
+
+void raise_error(const char* message, const char* file, int line, const char* func) {
+    std::stringstream ss;
+    ss << "COMPAS error occurred (" << file << ":" << line << ":" << func << "): " << message;
+    throw CompasException(ss.str());
+}
+
+void check_launch(const char* name) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::stringstream ss;
+        ss << "kernel launch failed (" << name << "): " << hipGetErrorString(e);
+        throw CompasException(ss.str());
+    }
+}
+
+Runtime::Runtime(int device) : device_(device) {
+    COMPAS_HIP_CHECK(hipSetDevice(device));
+    // default priority on purpose: streams of different priorities are not mixed
+    COMPAS_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Runtime::~Runtime() {
+    // Runtimes live until process exit; the driver reclaims memory then.
+}
+
+void Runtime::activate() const {
+    COMPAS_HIP_CHECK(hipSetDevice(device_));
+}
+
+static size_t round_block(size_t n) {
+    // 256-byte granularity below 1 MiB, 2 MiB above: limits fragmentation
+    // while keeping recycled blocks close in size.
+    if (n == 0) return 256;
+    if (n < (size_t(1) << 20)) return (n + 255) & ~size_t(255);
+    size_t g = size_t(2) << 20;
+    return (n + g - 1) / g * g;
+}
+
+void* Runtime::allocate(size_t nbytes) {
+    size_t n = round_block(nbytes);
+    {
+        std::lock_guard<std::mutex> lock(mutex_);
+        auto it = free_blocks_.lower_bound(n);
+        // accept a cached block up to 25% larger than requested
+        if (it != free_blocks_.end() && it->first <= n + n / 4) {
+            void* p = it->second;
+            size_t sz = it->first;
+            free_blocks_.erase(it);
+            cached_ -= sz;
+            in_use_ += sz;
+            return p;
+        }
+    }
+    activate();
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        // out of memory: give the cache back and retry once
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(&p, n);
+    }
+    if (e != hipSuccess) {
+        std::stringstream ss;
+        ss << "device allocation of " << nbytes << " bytes failed: " << hipGetErrorString(e);
+        throw CompasException(ss.str());
+    }
+    std::lock_guard<std::mutex> lock(mutex_);
+    in_use_ += n;
+    return p;
+}
+
+void Runtime::release(void* ptr, size_t nbytes) {
+    if (!ptr) return;
+    size_t n = round_block(nbytes);
+    std::lock_guard<std::mutex> lock(mutex_);
+    in_use_ -= n;
+    cached_ += n;
+    free_blocks_.emplace(n, ptr);
+}
+
+void Runtime::trim() {
+    std::multimap<size_t, void*> blocks;
+    {
+        std::lock_guard<std::mutex> lock(mutex_);
+        blocks.swap(free_blocks_);
+        cached_ = 0;
+    }
+    if (blocks.empty()) return;
+    activate();
+    // pending kernels may still read recycled memory of other arrays; freeing
+    // cached blocks is only safe once the stream has drained
+    (void)hipStreamSynchronize(stream_);
+    for (auto& kv : blocks) (void)hipFree(kv.second);
+}
+
+void Runtime::synchronize() {
+    activate();
+    COMPAS_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Runtime::set_timing(bool on) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    timing_ = on;
+}
+
+hipEvent_t Runtime::take_event() {
+    if (!event_pool_.empty()) {
+        hipEvent_t e = event_pool_.back();
+        event_pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    COMPAS_HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+int Runtime::begin_kernel(const char* name) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    Pending p;
+    p.name = name;
+    p.start = take_event();
+    p.stop = take_event();
+    COMPAS_HIP_CHECK(hipEventRecord(p.start, stream_));
+    pending_.push_back(p);
+    return int(pending_.size() - 1);
+}
+
+void Runtime::end_kernel(int token) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    COMPAS_HIP_CHECK(hipEventRecord(pending_[size_t(token)].stop, stream_));
+}
+
+std::map<std::string, KernelStats> Runtime::collect_timings() {
+    synchronize();
+    std::lock_guard<std::mutex> lock(mutex_);
+    for (auto& p : pending_) {
+        float ms = 0;
+        COMPAS_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+        auto& s = stats_[p.name];
+        s.total_ms += ms;
+        s.count += 1;
+        event_pool_.push_back(p.start);
+        event_pool_.push_back(p.stop);
+    }
+    pending_.clear();
+    auto out = stats_;
+    stats_.clear();
+    return out;
+}
+
+static std::mutex g_registry_mutex;
+static std::map<int, std::shared_ptr<Runtime>>& registry() {
+    static auto* r = new std::map<int, std::shared_ptr<Runtime>>();  // never destroyed
+    return *r;
+}
+
+std::shared_ptr<Runtime> runtime_for_device(int device) {
+    std::lock_guard<std::mutex> lock(g_registry_mutex);
+    auto& reg = registry();
+    auto it = reg.find(device);
+    if (it != reg.end()) return it->second;
+    auto rt = std::make_shared<Runtime>(device);
+    reg[device] = rt;
+    return rt;
+}
+
+Buffer::Buffer(std::shared_ptr<Runtime> rt, size_t nbytes) : rt_(std::move(rt)), nbytes_(nbytes) {
+    ptr_ = rt_->allocate(nbytes);
+}
+
+Buffer::~Buffer() {
+    rt_->release(ptr_, nbytes_);
+}
+
+void copy_device_to_host(const std::shared_ptr<Buffer>& buf, void* host, size_t nbytes) {
+    COMPAS_CHECK(nbytes <= buf->nbytes());
+    auto& rt = buf->runtime();
+    rt->activate();
+    COMPAS_HIP_CHECK(hipMemcpyAsync(host, buf->data(), nbytes, hipMemcpyDeviceToHost, rt->stream()));
+    COMPAS_HIP_CHECK(hipStreamSynchronize(rt->stream()));
+}
+
+void CompasContext::upload(const std::shared_ptr<Buffer>& buf, const void* host, size_t nbytes) const {
+    COMPAS_CHECK(host != nullptr);
+    activate();
+    COMPAS_HIP_CHECK(hipMemcpyAsync(buf->data(), host, nbytes, hipMemcpyHostToDevice, stream()));
+    // the caller may free or reuse the host memory as soon as this returns
+    COMPAS_HIP_CHECK(hipStreamSynchronize(stream()));
+}
+
+CompasContext CompasContext::with_device(int index) const {
+    int count = 0;
+    COMPAS_HIP_CHECK(hipGetDeviceCount(&count));
+    COMPAS_CHECK(count > 0);
+    int dev = ((index % count) + count) % count;
+    return CompasContext(runtime_for_device(dev));
+}
+
+CompasContext make_context(int device) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        throw CompasException("no HIP device available (compas requires an AMD GPU)");
+    }
+    return CompasContext(runtime_for_device(((device % count) + count) % count));
+}
+
+}  // namespace compas

@@ -1,0 +1,160 @@
+// Element-wise kernels: zero fill, accumulation, finite-difference
+// derivative (reference src/simulate/derivative.cu:11-94,
+// derivative_kernels.cuh:6-35) and the residual of the reconstruction
+// objective (src/utils/residual.cu:8-46, residual_kernels.cuh:8-64).
+
+#include "compas/device/common.h"
+#include "compas/internal.h"
+
+namespace compas {
+namespace kernels {
+
+// Zero fill by a kernel (not hipMemsetAsync) so callers may capture it in graphs.
+__global__ void fill_zero_u32(uint32_t* p, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void add_into_kernel(cfloat* dst, const cfloat* src, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] += src[i];
+}
+
+__global__ void add_difference_to_parameters(
+    const float* __restrict__ in, float* __restrict__ out, int nv, int field, float delta) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    for (int f = 0; f < TissueParameterField::NUM_FIELDS; f++) {
+        out[size_t(f) * nv + v] = in[size_t(f) * nv + v];
+    }
+    out[size_t(field) * nv + v] += delta;
+}
+
+__global__ void finite_difference(
+    const cfloat* __restrict__ e0, const cfloat* __restrict__ e1, int64_t n, float inv_delta, cfloat* __restrict__ out) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const cfloat a = e0[i], b = e1[i];
+        out[i] = {(b.re - a.re) * inv_delta, (b.im - a.im) * inv_delta};
+    }
+}
+
+// diff = lhs - rhs; per-block partial sums of |diff|^2 (FP64)
+__global__ void __launch_bounds__(256) residual_kernel(
+    const cfloat* __restrict__ lhs, const cfloat* __restrict__ rhs, int64_t n, cfloat* __restrict__ diff,
+    double* __restrict__ partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const cfloat d = {lhs[i].re - rhs[i].re, lhs[i].im - rhs[i].im};
+        diff[i] = d;
+        acc += double(d.re) * d.re + double(d.im) * d.im;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) sum_partials(const double* __restrict__ partial, int n, float* out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = float(red[0]);
+}
+
+}  // namespace kernels
+
+static unsigned grid_for(int64_t n) {
+    return unsigned(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096));
+}
+
+void CompasContext::fill_zero(void* ptr, size_t nbytes) const {
+    if (nbytes == 0) return;
+    COMPAS_CHECK(nbytes % 4 == 0);
+    activate();
+    const int64_t n = int64_t(nbytes / 4);
+    KernelScope ks(*this, "fill_zero");
+    kernels::fill_zero_u32<<<grid_for(n), 256, 0, stream()>>>(static_cast<uint32_t*>(ptr), n);
+    check_launch("fill_zero");
+}
+
+void add_into(const CompasContext& ctx, cfloat* dst, const cfloat* src, int64_t n) {
+    if (n == 0) return;
+    KernelScope ks(ctx, "add_into");
+    kernels::add_into_kernel<<<grid_for(n), 256, 0, ctx.stream()>>>(dst, src, n);
+    check_launch("add_into");
+}
+
+template<typename Sequence>
+static Array<cfloat, 2> derivative_impl(
+    const CompasContext& ctx, int field, Array<cfloat, 2> echos, TissueParameters p, Sequence seq, float delta) {
+    ctx.activate();
+    const int nv = p.nvoxels;
+    COMPAS_CHECK(echos.size(1) == nv);  // derivative.cu:24
+    COMPAS_CHECK(field >= 0 && field < TissueParameterField::NUM_FIELDS);  // derivative.cu:25
+    auto data = ctx.empty<float, 2>({TissueParameterField::NUM_FIELDS, nv});
+    if (nv > 0) {
+        KernelScope ks(ctx, "add_difference_to_parameters");
+        kernels::add_difference_to_parameters<<<dev::div_ceil(nv, 256), 256, 0, ctx.stream()>>>(
+            p.data.data(), data.data(), nv, field, delta);
+        check_launch("add_difference_to_parameters");
+    }
+    // the perturbed field is present even where the original map defaulted it
+    TissueParameters q = p;
+    q.data = data;
+    if (field == TissueParameterField::B1 && !q.has_b1) COMPAS_ERROR("B1 is not part of these tissue parameters");
+    if (field == TissueParameterField::B0 && !q.has_b0) COMPAS_ERROR("B0 is not part of these tissue parameters");
+    if (field == TissueParameterField::Z && !q.has_z) COMPAS_ERROR("z is not part of these tissue parameters");
+    auto next = simulate_magnetization(ctx, q, seq);
+    COMPAS_CHECK(next.size(0) == echos.size(0));
+    auto out = ctx.empty<cfloat, 2>({echos.size(0), echos.size(1)});
+    const int64_t n = out.numel();
+    if (n > 0) {
+        KernelScope ks(ctx, "finite_difference");
+        kernels::finite_difference<<<grid_for(n), 256, 0, ctx.stream()>>>(
+            echos.data(), next.data(), n, 1.0f / delta, out.data());
+        check_launch("finite_difference");
+    }
+    return out;
+}
+
+Array<cfloat, 2> simulate_magnetization_derivative(
+    const CompasContext& ctx, int field, Array<cfloat, 2> echos, TissueParameters p, FISPSequence seq, float delta) {
+    return derivative_impl(ctx, field, echos, p, seq, delta);
+}
+
+Array<cfloat, 2> simulate_magnetization_derivative(
+    const CompasContext& ctx, int field, Array<cfloat, 2> echos, TissueParameters p, pSSFPSequence seq, float delta) {
+    return derivative_impl(ctx, field, echos, p, seq, delta);
+}
+
+Array<cfloat, 3> compute_residual(
+    const CompasContext& ctx, Array<cfloat, 3> lhs, Array<cfloat, 3> rhs, float* objective_out) {
+    ctx.activate();
+    COMPAS_CHECK(lhs.sizes() == rhs.sizes());  // residual.cu:14
+    auto diff = ctx.empty<cfloat, 3>(lhs.sizes());
+    const int64_t n = lhs.numel();
+    const unsigned blocks = unsigned(std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256)));
+    auto partial = ctx.empty<double, 1>({int64_t(blocks)});
+    auto objective = ctx.empty<float, 1>({1});
+    {
+        KernelScope ks(ctx, "residual");
+        kernels::residual_kernel<<<blocks, 256, 0, ctx.stream()>>>(lhs.data(), rhs.data(), n, diff.data(), partial.data());
+        check_launch("residual");
+        kernels::sum_partials<<<1, 256, 0, ctx.stream()>>>(partial.data(), int(blocks), objective.data());
+        check_launch("sum_partials");
+    }
+    if (objective_out) objective.copy_to(objective_out);
+    return diff;
+}
+
+}  // namespace compas

@@ -1,0 +1,248 @@
+// magnetization_to_signal and phase_encoding on gfx950.
+//
+// Semantics: reference src/trajectories/signal.cu:13-356,
+// signal_kernels.cuh:12-345, include/compas/trajectories/cartesian_view.cuh:28-63,
+// spiral_view.cuh:13-73, src/trajectories/phase_encoding_kernels.cuh:10-25.
+//
+// Cartesian:  S[c][r][s] = sum_v m[r][v] * W[c*ns + s][v] with
+//   W = rho_v * C[c][v] * exp((ns/2 - s) dt R2) * exp(i phi),
+//   phi = s*Theta_v - ns/2 (dt 2 pi B0 + dk_re x)  (readout factor folded in),
+//   Theta_v = dk_re x + dk_im y + dt 2 pi B0,
+// so the raw echoes are the A operand of one contraction (M = nr, N = nc*ns,
+// K = nv) and the coil dimension is folded into N.  phi is formed in FP64 and
+// reduced to [-pi, pi) before the float sincos.
+//
+// Spiral:  S[c][r][s] = sum_v A[r][v] C[c][v] E_rv^s with per-readout
+// exponents, evaluated by a direct kernel: FP64-anchored powers E^s0 every 8
+// samples, complex recurrence in between.
+
+#include "compas/device/common.h"
+#include "compas/internal.h"
+
+namespace compas {
+namespace kernels {
+
+__global__ void signal_cartesian_weights(
+    dev::TissueView tissue, const cfloat* __restrict__ coils, int nc, int ns,
+    float dt, float dk_re, float dk_im, cfloat* __restrict__ W) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    const int nv = tissue.nvoxels;
+    if (v >= nv) return;
+    const float R2 = 1.0f / tissue.T2(v);
+    const double B0 = tissue.B0(v), x = tissue.x(v), y = tissue.y(v);
+    const double off = double(dt) * dev::TWO_PI * B0;
+    const double theta = double(dk_re) * x + double(dk_im) * y + off;
+    const double phi = double(s) * theta - 0.5 * double(ns) * (off + double(dk_re) * x);
+    const float mag = expf((0.5f * float(ns) - float(s)) * dt * R2);
+    const cfloat e = dev::cmul(dev::polar_f(mag, dev::reduce_phase(phi)), tissue.rho(v));
+    for (int c = 0; c < nc; c++) {
+        W[(size_t(c) * ns + s) * nv + v] = dev::cmul(e, coils[size_t(c) * nv + v]);
+    }
+}
+
+// A[r][v] = m[r][v] * rewind * prephaser * rho (spiral_view.cuh:13-57, signal_kernels.cuh:12-30)
+__global__ void signal_spiral_factors(
+    dev::TissueView tissue, const cfloat* __restrict__ echos, const cfloat* __restrict__ k_start,
+    int nr, int ns, float dt, cfloat* __restrict__ A) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    const int nv = tissue.nvoxels;
+    if (v >= nv) return;
+    const float R2 = 1.0f / tissue.T2(v);
+    const double B0 = tissue.B0(v), x = tissue.x(v);
+    const double half = 0.5 * double(ns) * double(dt);
+    const double phi = -half * dev::TWO_PI * B0 + double(k_start[r].re) * x;
+    const float mag = expf(float(half) * R2);
+    const cfloat f = dev::cmul(dev::polar_f(mag, dev::reduce_phase(phi)), tissue.rho(v));
+    A[size_t(r) * nv + v] = dev::cmul(echos[size_t(r) * nv + v], f);
+}
+
+constexpr int SP_SAMPLES = 8;  // samples per thread (recurrence length)
+constexpr int SP_COILS = 4;  // coils per block
+
+// grid: (nr, ceil(ns / 64), ceil(nc / SP_COILS)); block 256 = 4 waves.
+// lane % 32 strides over voxels; (wave, lane / 32) selects 8 consecutive samples.
+__global__ void __launch_bounds__(256) signal_spiral_direct(
+    dev::TissueView tissue, const cfloat* __restrict__ A, const cfloat* __restrict__ coils, int nc,
+    const cfloat* __restrict__ delta_k, int nr, int ns, float dt, cfloat* __restrict__ signal) {
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int vl = lane & 31;
+    const int sg = wave * 2 + (lane >> 5);
+    const int s0 = blockIdx.y * 64 + sg * SP_SAMPLES;
+    const int c0 = blockIdx.z * SP_COILS;
+    const int nv = tissue.nvoxels;
+    const cfloat dk = delta_k[r];
+
+    float acc_re[SP_COILS][SP_SAMPLES], acc_im[SP_COILS][SP_SAMPLES];
+#pragma unroll
+    for (int c = 0; c < SP_COILS; c++)
+#pragma unroll
+        for (int i = 0; i < SP_SAMPLES; i++) {
+            acc_re[c][i] = 0.0f;
+            acc_im[c][i] = 0.0f;
+        }
+
+    if (s0 < ns) {
+        for (int v = vl; v < nv; v += 32) {
+            const float R2 = 1.0f / tissue.T2(v);
+            const double theta = double(dk.re) * double(tissue.x(v)) + double(dk.im) * double(tissue.y(v))
+                + double(dt) * dev::TWO_PI * double(tissue.B0(v));
+            const float decay = -dt * R2;
+            const cfloat step = dev::polar_f(expf(decay), dev::reduce_phase(theta));
+            cfloat cur = dev::polar_f(expf(float(s0) * decay), dev::reduce_phase(double(s0) * theta));
+            cur = dev::cmul(cur, A[size_t(r) * nv + v]);
+            cfloat cv[SP_COILS];
+#pragma unroll
+            for (int c = 0; c < SP_COILS; c++) cv[c] = (c0 + c < nc) ? coils[size_t(c0 + c) * nv + v] : cfloat {0, 0};
+#pragma unroll
+            for (int i = 0; i < SP_SAMPLES; i++) {
+#pragma unroll
+                for (int c = 0; c < SP_COILS; c++) {
+                    acc_re[c][i] = fmaf(cur.re, cv[c].re, fmaf(-cur.im, cv[c].im, acc_re[c][i]));
+                    acc_im[c][i] = fmaf(cur.re, cv[c].im, fmaf(cur.im, cv[c].re, acc_im[c][i]));
+                }
+                cur = dev::cmul(cur, step);
+            }
+        }
+    }
+    // reduce over the 32 voxel lanes of each half-wave
+#pragma unroll
+    for (int c = 0; c < SP_COILS; c++)
+#pragma unroll
+        for (int i = 0; i < SP_SAMPLES; i++) {
+            float re = acc_re[c][i], im = acc_im[c][i];
+#pragma unroll
+            for (int off = 16; off > 0; off >>= 1) {
+                re += __shfl_xor(re, off, 32);
+                im += __shfl_xor(im, off, 32);
+            }
+            acc_re[c][i] = re;
+            acc_im[c][i] = im;
+        }
+    if (vl == 0 && s0 < ns) {
+#pragma unroll
+        for (int c = 0; c < SP_COILS; c++) {
+            if (c0 + c >= nc) continue;
+#pragma unroll
+            for (int i = 0; i < SP_SAMPLES; i++) {
+                if (s0 + i < ns) signal[(size_t(c0 + c) * nr + r) * ns + s0 + i] = {acc_re[c][i], acc_im[c][i]};
+            }
+        }
+    }
+}
+
+__global__ void phase_encoding_kernel(
+    dev::TissueView tissue, const cfloat* __restrict__ echos, const cfloat* __restrict__ k_start, int nr,
+    cfloat* __restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    const int nv = tissue.nvoxels;
+    if (v >= nv) return;
+    const float ky = k_start[r].im;
+    const double phi = double(tissue.y(v)) * double(ky);
+    out[size_t(r) * nv + v] = dev::cmul(echos[size_t(r) * nv + v], dev::polar_f(1.0f, dev::reduce_phase(phi)));
+}
+
+}  // namespace kernels
+
+static void check_signal_shapes(const Array<cfloat, 2>& echos, const TissueParameters& p, const Trajectory& t,
+                                const Array<cfloat, 2>& coils) {
+    COMPAS_CHECK(echos.size(0) == t.nreadouts);
+    COMPAS_CHECK(echos.size(1) == p.nvoxels);
+    COMPAS_CHECK(coils.size(1) == p.nvoxels);
+    COMPAS_CHECK(t.samples_per_readout >= 0);
+}
+
+static Array<cfloat, 3> signal_cartesian(
+    const CompasContext& ctx, const Array<cfloat, 2>& echos, const TissueParameters& p,
+    const CartesianTrajectory& t, const Array<cfloat, 2>& coils, SimulateSignalMethod method) {
+    const int nc = int(coils.size(0)), nr = t.nreadouts, ns = t.samples_per_readout, nv = p.nvoxels;
+    auto signal = ctx.empty<cfloat, 3>({nc, nr, ns});
+    if (signal.numel() == 0) return signal;
+    if (nv == 0) {
+        ctx.fill_zero(signal.data(), signal.nbytes());
+        return signal;
+    }
+    auto W = ctx.empty<cfloat, 2>({int64_t(nc) * ns, nv});
+    {
+        KernelScope ks(ctx, "signal_cartesian_weights");
+        dim3 grid(dev::div_ceil(nv, 256), ns);
+        kernels::signal_cartesian_weights<<<grid, 256, 0, ctx.stream()>>>(
+            dev::tissue_view(p), coils.data(), nc, ns, t.delta_t, t.delta_k.re, t.delta_k.im, W.data());
+        check_launch("signal_cartesian_weights");
+    }
+    ContractionShape s {nr, nc * ns, nv, nv, nv, ns, int64_t(nr) * ns, false};
+    complex_contraction(ctx, echos.data(), W.data(), signal.data(), s, signal_precision(method));
+    return signal;
+}
+
+static Array<cfloat, 3> signal_spiral(
+    const CompasContext& ctx, const Array<cfloat, 2>& echos, const TissueParameters& p,
+    const SpiralTrajectory& t, const Array<cfloat, 2>& coils) {
+    const int nc = int(coils.size(0)), nr = t.nreadouts, ns = t.samples_per_readout, nv = p.nvoxels;
+    COMPAS_CHECK(t.k_start.size(0) == nr && t.delta_k.size(0) == nr);
+    auto signal = ctx.empty<cfloat, 3>({nc, nr, ns});
+    if (signal.numel() == 0) return signal;
+    if (nv == 0) {
+        ctx.fill_zero(signal.data(), signal.nbytes());
+        return signal;
+    }
+    auto A = ctx.empty<cfloat, 2>({nr, nv});
+    {
+        KernelScope ks(ctx, "signal_spiral_factors");
+        dim3 grid(dev::div_ceil(nv, 256), nr);
+        kernels::signal_spiral_factors<<<grid, 256, 0, ctx.stream()>>>(
+            dev::tissue_view(p), echos.data(), t.k_start.data(), nr, ns, t.delta_t, A.data());
+        check_launch("signal_spiral_factors");
+    }
+    {
+        KernelScope ks(ctx, "signal_spiral_direct");
+        dim3 grid(nr, dev::div_ceil(ns, 64), dev::div_ceil(nc, kernels::SP_COILS));
+        kernels::signal_spiral_direct<<<grid, 256, 0, ctx.stream()>>>(
+            dev::tissue_view(p), A.data(), coils.data(), nc, t.delta_k.data(), nr, ns, t.delta_t, signal.data());
+        check_launch("signal_spiral_direct");
+    }
+    return signal;
+}
+
+Array<cfloat, 3> magnetization_to_signal(
+    const CompasContext& ctx,
+    Array<cfloat, 2> echos,
+    TissueParameters parameters,
+    const Trajectory& trajectory,
+    Array<cfloat, 2> coils,
+    SimulateSignalMethod method) {
+    ctx.activate();
+    check_signal_shapes(echos, parameters, trajectory, coils);
+    if (const auto* cart = dynamic_cast<const CartesianTrajectory*>(&trajectory)) {
+        (void)signal_precision(method);  // validates the enumerator
+        return signal_cartesian(ctx, echos, parameters, *cart, coils, method);
+    } else if (const auto* sp = dynamic_cast<const SpiralTrajectory*>(&trajectory)) {
+        return signal_spiral(ctx, echos, parameters, *sp, coils);
+    }
+    COMPAS_ERROR("invalid trajectory type");
+}
+
+Array<cfloat, 2> phase_encoding(
+    const CompasContext& ctx,
+    const Array<cfloat, 2>& echos,
+    const TissueParameters& parameters,
+    const CartesianTrajectory& trajectory) {
+    ctx.activate();
+    const int nr = trajectory.nreadouts, nv = parameters.nvoxels;
+    COMPAS_CHECK(echos.size(0) == nr && echos.size(1) == nv);
+    COMPAS_CHECK(trajectory.k_start.size(0) == nr);
+    auto out = ctx.empty<cfloat, 2>({nr, nv});
+    if (out.numel() == 0) return out;
+    KernelScope ks(ctx, "phase_encoding");
+    dim3 grid(dev::div_ceil(nv, 256), nr);
+    kernels::phase_encoding_kernel<<<grid, 256, 0, ctx.stream()>>>(
+        dev::tissue_view(parameters), echos.data(), trajectory.k_start.data(), nr, out.data());
+    check_launch("phase_encoding");
+    return out;
+}
+
+}  // namespace compas

@@ -1,0 +1,167 @@
+/*
+ * C-ABI of the MI355X compas simulation hot path (libcompas_amd.so).
+ *
+ * The first 26 entry points are the exports of the reference's Julia
+ * bindings, with the same names, argument order and meaning
+ * (julia-bindings/src/bindings.cpp, line of each cited below), so the
+ * @ccall wrappers of CompasToolkit.jl bind to this library unchanged.
+ * Handles are opaque pointers owned by the caller and released with the
+ * matching destroy function.  Array layout is row-major (C order, last index
+ * fastest), complex values are interleaved float pairs.
+ *
+ * Errors: every entry point catches exceptions.  When the process provides
+ * `jl_errorf` (a Julia host), it is called with the message and does not
+ * return, exactly as in the reference (julia-bindings/src/common.h:50-63).
+ * Otherwise the message is stored per thread, the call returns NULL (or
+ * nothing), and compas_last_error() returns the message.
+ */
+#ifndef COMPAS_AMD_H
+#define COMPAS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct CompasContextHandle CompasContextHandle; /* compas::CompasContext */
+typedef struct CompasObjectHandle CompasObjectHandle;   /* parameters, sequences, trajectories */
+typedef struct CompasArrayHandle CompasArrayHandle;     /* device array */
+
+typedef struct {
+    float re;
+    float im;
+} compas_cfloat;
+
+/* ---- context and arrays ---------------------------------------------- */
+
+/* bindings.cpp:19 */
+const char* compas_version(void);
+/* bindings.cpp:23 -- context on device 0 */
+const CompasContextHandle* compas_make_context(void);
+/* bindings.cpp:30 -- copy bound to `device` (modulo the device count) */
+const CompasContextHandle* compas_copy_context_for_device(CompasContextHandle* ctx, int device);
+/* bindings.cpp:38 */
+void compas_destroy_context(const CompasContextHandle* ctx);
+/* bindings.cpp:42 -- parameters, sequences and trajectories */
+void compas_destroy_object(const CompasObjectHandle* obj);
+/* bindings.cpp:46 -- waits for all work queued on the context */
+void compas_synchronize(const CompasContextHandle* ctx);
+/* bindings.cpp:50 -- copies `data` (rank <= 3, sizes in C order) to the device */
+const CompasArrayHandle* compas_make_array_float(
+    const CompasContextHandle* ctx, const float* data, int rank, int64_t* sizes);
+/* bindings.cpp:69 -- blocking copy of `length` elements to `dest` */
+void compas_read_array_float(
+    const CompasContextHandle* ctx, const CompasArrayHandle* array, float* dest, int64_t length);
+/* bindings.cpp:80 */
+const CompasArrayHandle* compas_make_array_complex(
+    const CompasContextHandle* ctx, const compas_cfloat* data, int rank, int64_t* sizes);
+/* bindings.cpp:98 */
+void compas_read_array_complex(
+    const CompasContextHandle* ctx, const CompasArrayHandle* array, compas_cfloat* dest, int64_t length);
+/* bindings.cpp:109 */
+void compas_destroy_array(const CompasArrayHandle* array);
+
+/* ---- data model --------------------------------------------------------- */
+
+/* bindings.cpp:117 -- every field present; nvoxels floats each */
+const CompasObjectHandle* compas_make_tissue_parameters(
+    const CompasContextHandle* ctx, int nvoxels, const float* T1, const float* T2, const float* B1,
+    const float* B0, const float* rho_x, const float* rho_y, const float* x, const float* y, const float* z);
+/* bindings.cpp:151 -- RF_train [nTR] complex, sliceprofiles [nslices][nTR] complex */
+CompasObjectHandle* compas_make_fisp_sequence(
+    const CompasArrayHandle* RF_train, const CompasArrayHandle* sliceprofiles, float TR, float TE, float TW,
+    int max_state, float TI, int undersampling_factor, int repetitions);
+/* bindings.cpp:176 -- z is a float array [nz] */
+CompasObjectHandle* compas_make_pssfp_sequence(
+    const CompasArrayHandle* RF_train, float TR, const CompasArrayHandle* gamma_dt_RF, float dt_ex, float dt_inv,
+    float dt_pr, float gamma_dt_GRz_ex, float gamma_dt_GRz_inv, float gamma_dt_GRz_pr, const CompasArrayHandle* z);
+/* bindings.cpp:198 -- real delta_k, as passed by CompasToolkit.jl */
+CompasObjectHandle* compas_make_cartesian_trajectory(
+    int nreadouts, int samples_per_readout, float delta_t, CompasArrayHandle* k_start, float delta_k);
+/* bindings.cpp:214 */
+CompasObjectHandle* compas_make_spiral_trajectory(
+    int nreadouts, int samples_per_readout, float delta_t, CompasArrayHandle* k_start, CompasArrayHandle* delta_k);
+
+/* ---- compute ------------------------------------------------------------ */
+
+/* bindings.cpp:230 -- returns echos [nreadouts][nvoxels] complex */
+CompasArrayHandle* compas_simulate_magnetization_fisp(
+    const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence);
+/* bindings.cpp:243 */
+CompasArrayHandle* compas_simulate_magnetization_pssfp(
+    const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence);
+/* bindings.cpp:256 -- signal [ncoils][nreadouts][nsamples]; low_precision 0 = MatmulFast, else MatmulLow */
+CompasArrayHandle* compas_magnetization_to_signal_cartesian(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasArrayHandle* coils, const CompasObjectHandle* trajectory, int32_t low_precision);
+/* bindings.cpp:278 */
+CompasArrayHandle* compas_magnetization_to_signal_spiral(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasArrayHandle* coils, const CompasObjectHandle* trajectory);
+/* bindings.cpp:296 -- vector [4][nvoxels]; low_precision 0 = GemmFast, else GemmLow */
+CompasArrayHandle* compas_compute_jacobian(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasObjectHandle* parameters, const CompasArrayHandle* coils,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* vector, int32_t low_precision);
+/* bindings.cpp:324 -- note: trajectory before coils; vector [ncoils][nreadouts][nsamples] */
+CompasArrayHandle* compas_compute_jacobian_hermitian(
+    const CompasContextHandle* ctx, int ncoils, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory,
+    const CompasArrayHandle* coils, const CompasArrayHandle* vector, int32_t low_precision);
+/* bindings.cpp:352 -- exported without the compas_ prefix, as in the reference */
+CompasArrayHandle* phase_encoding(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* trajectory);
+/* bindings.cpp:367 -- returns lhs - rhs, writes sum |lhs - rhs|^2 to *sum */
+CompasArrayHandle* compas_compute_residual(
+    const CompasContextHandle* ctx, const CompasArrayHandle* lhs, const CompasArrayHandle* rhs, float* sum);
+/* bindings.cpp:376 */
+CompasArrayHandle* compas_simulate_magnetization_derivative_pssfp(
+    const CompasContextHandle* ctx, int field, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    float delta, const CompasObjectHandle* sequence);
+/* bindings.cpp:395 */
+CompasArrayHandle* compas_simulate_magnetization_derivative_fisp(
+    const CompasContextHandle* ctx, int field, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    float delta, const CompasObjectHandle* sequence);
+
+/* ---- extensions (no counterpart in the reference) ------------------------ */
+
+/* Error channel for hosts without jl_errorf (Python ctypes, C, Go, ...). */
+const char* compas_last_error(void);
+void compas_clear_error(void);
+
+/* Shape of an array handle: dtype 0 = float32, 1 = complex64. Returns rank. */
+int compas_array_info(const CompasArrayHandle* array, int* dtype, int64_t* sizes3);
+
+/* Method-selecting variants of the signal and Jacobian entry points, exposing
+ * the C++ enums SimulateSignalMethod {Naive, Direct, MatmulPedantic, Matmul,
+ * MatmulFast, MatmulLow} and JacobianComputeMethod {Naive, Direct, Gemm,
+ * GemmFast, GemmLow} as integers 0..5 / 0..4. */
+CompasArrayHandle* compas_magnetization_to_signal_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasObjectHandle* parameters,
+    const CompasArrayHandle* coils, const CompasObjectHandle* trajectory, int method);
+CompasArrayHandle* compas_compute_jacobian_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasObjectHandle* parameters, const CompasArrayHandle* coils,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* vector, int method);
+CompasArrayHandle* compas_compute_jacobian_hermitian_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory,
+    const CompasArrayHandle* coils, const CompasArrayHandle* vector, int method);
+
+/* Per-kernel timing with HIP events on the context's stream.  While enabled,
+ * every library launch is bracketed by events; compas_read_kernel_timings
+ * synchronizes, writes "name count total_ms\n" lines into buf (NUL-terminated,
+ * truncated to buflen) and resets the counters.  Returns bytes needed. */
+void compas_set_kernel_timing(const CompasContextHandle* ctx, int enabled);
+int compas_read_kernel_timings(const CompasContextHandle* ctx, char* buf, int buflen);
+
+/* Release cached device memory held by the context's pool. */
+void compas_trim_memory(const CompasContextHandle* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COMPAS_AMD_H */

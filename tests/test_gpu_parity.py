@@ -1,0 +1,291 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle on seeded inputs.
+
+Tolerances (relative L2 over the whole output, complex as two reals):
+  magnetisation  <= 1e-5   against the FP64 oracle and the FP32 port
+  signal, J.v, J^H.y <= 1e-4 against the FP64 oracle
+"""
+
+import numpy as np
+import pytest
+
+import compas_amd as ca
+from compas_amd import workloads as W
+from gpu_helpers import banner, context, objects, rel
+
+pytestmark = pytest.mark.gpu
+
+MAG_TOL = 1e-5
+SIG_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return context()
+
+
+def small_fisp(nv=700, nTR=96, N=16, b0=30.0, seed=3, **kw):
+    n = int(np.ceil(np.sqrt(nv)))
+    t = W.make_tissue(W.SplitMix64(seed), n, 20.0, b0).slice(0, nv)
+    rng = W.SplitMix64(seed + 100)
+    RF = (rng.uniform(nTR, 5, 70) * np.exp(1j * rng.uniform(nTR, -0.5, 0.5))).astype(np.complex64)
+    seq = W.FispSeq(RF=RF, profiles=np.ones((1, nTR), np.complex64), TR=0.009, TE=0.004, max_state=N, TI=0.02)
+    for k, v in kw.items():
+        setattr(seq, k, v)
+    return t, seq
+
+
+def run_fisp(t, seq):
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    s = ca.FispSequence.from_sequence(seq)
+    return ca.simulate_magnetization(p, s).collect()
+
+
+# ---------------------------------------------------------------- FISP ---
+
+@pytest.mark.parametrize("N", [1, 2, 5, 8, 16, 25, 32, 40, 64, 100, 128])
+def test_fisp_state_counts(ctx, oracle, N):
+    banner(f"fisp N={N}")
+    t, seq = small_fisp(nv=300, nTR=70, N=N)
+    got = run_fisp(t, seq)
+    want = oracle.fisp(t, seq, "f64")
+    assert rel(got, want) < MAG_TOL
+    assert rel(got, oracle.fisp(t, seq, "f32")) < MAG_TOL
+
+
+@pytest.mark.parametrize("nv", [1, 63, 64, 65, 4096, 20000])
+def test_fisp_voxel_counts_and_layouts(ctx, oracle, nv):
+    banner(f"fisp nv={nv}")
+    t, seq = small_fisp(nv=nv, nTR=40, N=12)
+    got = run_fisp(t, seq)
+    assert got.shape == (40, nv)
+    assert rel(got, oracle.fisp(t, seq, "f64")) < MAG_TOL
+
+
+def test_fisp_sequence_options(ctx, oracle):
+    banner("fisp usf/reps/TW/no-inversion/slices")
+    t, seq = small_fisp(nv=257, nTR=50, N=20)
+    seq.undersampling = 3
+    seq.repetitions = 3
+    seq.TW = 0.15
+    prof = np.stack([np.linspace(0.6, 1.1, 50), np.full(50, 0.9 - 0.2j)]).astype(np.complex64)
+    seq.profiles = prof
+    got = run_fisp(t, seq)
+    assert got.shape == (150, 257)
+    assert rel(got, oracle.fisp(t, seq, "f64")) < MAG_TOL
+    seq.inversion = False
+    seq.wait_spoiling = False
+    assert rel(run_fisp(t, seq), oracle.fisp(t, seq, "f64")) < MAG_TOL
+
+
+def test_fisp_b0_zero_branch(ctx, oracle):
+    t, seq = small_fisp(nv=200, nTR=60, N=16, b0=None)
+    assert np.all(t.B0 == 0)
+    assert rel(run_fisp(t, seq), oracle.fisp(t, seq, "f64")) < MAG_TOL
+
+
+@pytest.mark.parametrize("index,scale", [(1, 1), (2, 4)])
+def test_fisp_workloads(ctx, oracle, index, scale):
+    banner(f"fisp workload {index} scale {scale}")
+    wl = W.config(index, scale=scale)
+    params, seq, _ = objects(wl)
+    got = ca.simulate_magnetization(params, seq).collect()
+    want = oracle.fisp(wl.tissue, wl.sequence, "f64")
+    assert rel(got, want) < MAG_TOL
+    assert rel(got, oracle.fisp(wl.tissue, wl.sequence, "f32")) < MAG_TOL
+
+
+def test_fisp_full_size_voxel_sample(ctx, oracle):
+    # 50176 voxels x 1120 TRs x 25 states: every voxel is independent, so a
+    # contiguous slice of voxels checked against the oracle checks the full run
+    banner("fisp full size 50176 x 1120")
+    wl = W.config(2)
+    params, seq, _ = objects(wl)
+    got = ca.simulate_magnetization(params, seq).collect()
+    assert got.shape == (1120, 50176)
+    for begin in (0, 25000, 50176 - 300):
+        sub = wl.tissue.slice(begin, begin + 300)
+        assert rel(got[:, begin:begin + 300], oracle.fisp(sub, wl.sequence, "f64")) < MAG_TOL
+
+
+def test_fisp_max_state_limit(ctx):
+    t, seq = small_fisp(nv=10, nTR=5, N=129)
+    with pytest.raises(ca.CompasError, match="max_state"):
+        run_fisp(t, seq)
+
+
+# --------------------------------------------------------------- pSSFP ---
+
+def test_pssfp_workload(ctx, oracle):
+    banner("pssfp workload 3 scale 8")
+    wl = W.config(3, scale=8)
+    params, seq, _ = objects(wl)
+    got = ca.simulate_magnetization(params, seq).collect()
+    want = oracle.pssfp(wl.tissue, wl.sequence, "f64")
+    assert rel(got, want) < MAG_TOL
+
+
+@pytest.mark.parametrize("nz,nRF", [(1, 1), (3, 5), (35, 25), (60, 7), (130, 3)])
+def test_pssfp_shapes(ctx, oracle, nz, nRF):
+    banner(f"pssfp nz={nz} nRF={nRF}")
+    n = 12
+    t = W.make_tissue(W.SplitMix64(5), n, 20.0, 40.0)
+    seq = W.make_pssfp(30, 60.0, nz=nz, nRF=nRF)
+    context()
+    got = ca.simulate_magnetization(ca.TissueParameters.from_tissue(t), ca.pSSFPSequence.from_sequence(seq)).collect()
+    assert rel(got, oracle.pssfp(t, seq, "f64")) < MAG_TOL
+
+
+def test_pssfp_large_angles(ctx, oracle):
+    # few RF sub-steps with large flip angles exercise the sincos branch of the rotation
+    t = W.make_tissue(W.SplitMix64(8), 10, 20.0, 20.0)
+    seq = W.make_pssfp(20, 150.0, nz=4, nRF=1)
+    context()
+    got = ca.simulate_magnetization(ca.TissueParameters.from_tissue(t), ca.pSSFPSequence.from_sequence(seq)).collect()
+    assert rel(got, oracle.pssfp(t, seq, "f64")) < MAG_TOL
+
+
+# -------------------------------------------------------------- signal ---
+
+@pytest.mark.parametrize("method", ["direct", "matmul_fast", "matmul", "matmul_pedantic"])
+def test_cartesian_signal_methods(ctx, oracle, method):
+    banner(f"cartesian signal {method}")
+    wl = W.config(2, scale=8)
+    params, seq, traj = objects(wl)
+    echos = oracle.fisp(wl.tissue, wl.sequence, "f64").astype(np.complex64)
+    got = ca.magnetization_to_signal(echos, params, traj, wl.coils, method=method).collect()
+    want = oracle.signal_cartesian(wl.tissue, wl.trajectory, wl.coils, echos, "f64")
+    assert got.shape == want.shape
+    assert rel(got, want) < SIG_TOL
+
+
+def test_cartesian_signal_default_abi_call(ctx, oracle):
+    wl = W.config(1, scale=2)
+    params, seq, traj = objects(wl)
+    echos = oracle.fisp(wl.tissue, wl.sequence, "f64").astype(np.complex64)
+    got = ca.magnetization_to_signal(echos, params, traj, wl.coils).collect()
+    assert rel(got, oracle.signal_cartesian(wl.tissue, wl.trajectory, wl.coils, echos, "f64")) < SIG_TOL
+
+
+def test_cartesian_signal_low_precision_is_coarse(ctx, oracle):
+    wl = W.config(1, scale=2)
+    params, seq, traj = objects(wl)
+    echos = oracle.fisp(wl.tissue, wl.sequence, "f64").astype(np.complex64)
+    got = ca.magnetization_to_signal(echos, params, traj, wl.coils, low_precision=True).collect()
+    assert rel(got, oracle.signal_cartesian(wl.tissue, wl.trajectory, wl.coils, echos, "f64")) < 2e-2
+
+
+def test_spiral_signal(ctx, oracle):
+    banner("spiral signal")
+    wl = W.config(3, scale=8)
+    params, seq, traj = objects(wl)
+    rng = W.SplitMix64(42)
+    nv, nr = wl.tissue.nvoxels, wl.trajectory.nreadouts
+    echos = ((rng.normal(nr * nv) + 1j * rng.normal(nr * nv)) * 0.1).reshape(nr, nv).astype(np.complex64)
+    got = ca.magnetization_to_signal(echos, params, traj, wl.coils).collect()
+    want = oracle.signal_spiral(wl.tissue, wl.trajectory, wl.coils, echos, "f64")
+    assert rel(got, want) < SIG_TOL
+
+
+def test_phase_encoding(ctx, oracle):
+    wl = W.config(1, scale=2)
+    params, seq, traj = objects(wl)
+    echos = oracle.fisp(wl.tissue, wl.sequence, "f64").astype(np.complex64)
+    got = ca.phase_encoding(echos, params, traj).collect()
+    assert rel(got, oracle.phase_encoding(wl.tissue, wl.trajectory, echos, "f64")) < 1e-6
+
+
+def test_signal_shape_errors(ctx):
+    wl = W.config(1, scale=4)
+    params, seq, traj = objects(wl)
+    bad = np.zeros((3, wl.tissue.nvoxels), np.complex64)
+    with pytest.raises(ca.CompasError):
+        ca.magnetization_to_signal(bad, params, traj, wl.coils)
+
+
+# ------------------------------------------------------------ jacobian ---
+
+def jac_inputs(oracle, scale=8):
+    wl = W.config(2, scale=scale)
+    rng = W.SplitMix64(9)
+    nv, nr, ns = wl.tissue.nvoxels, wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    echos = oracle.fisp(wl.tissue, wl.sequence, "f64").astype(np.complex64)
+    d1 = (rng.normal(nr * nv) * 0.01 + 1j * rng.normal(nr * nv) * 0.01).reshape(nr, nv).astype(np.complex64)
+    d2 = (rng.normal(nr * nv) * 0.1 + 1j * rng.normal(nr * nv) * 0.1).reshape(nr, nv).astype(np.complex64)
+    v = (rng.normal(4 * nv) + 1j * rng.normal(4 * nv)).reshape(4, nv).astype(np.complex64)
+    nc = wl.coils.shape[0]
+    y = (rng.normal(nc * nr * ns) + 1j * rng.normal(nc * nr * ns)).reshape(nc, nr, ns).astype(np.complex64)
+    return wl, echos, d1, d2, v, y
+
+
+@pytest.mark.parametrize("method", ["direct", "gemm_fast"])
+def test_jacobian_product(ctx, oracle, method):
+    banner(f"J.v {method}")
+    wl, echos, d1, d2, v, y = jac_inputs(oracle)
+    params, seq, traj = objects(wl)
+    got = ca.compute_jacobian(echos, (d1, d2), params, traj, wl.coils, v, method=method).collect()
+    want = oracle.jacobian(wl.tissue, wl.trajectory, wl.coils, echos, d1, d2, v, "f64")
+    assert rel(got, want) < SIG_TOL
+
+
+@pytest.mark.parametrize("method", ["direct", "gemm_fast"])
+def test_jacobian_hermitian(ctx, oracle, method):
+    banner(f"J^H.y {method}")
+    wl, echos, d1, d2, v, y = jac_inputs(oracle)
+    params, seq, traj = objects(wl)
+    got = ca.compute_jacobian_hermitian(echos, (d1, d2), params, traj, wl.coils, y, method=method).collect()
+    want = oracle.jacobian_hermitian(wl.tissue, wl.trajectory, wl.coils, echos, d1, d2, y, "f64")
+    assert rel(got, want) < SIG_TOL
+
+
+def test_jacobian_adjoint_on_gpu(ctx, oracle):
+    wl, echos, d1, d2, v, y = jac_inputs(oracle)
+    params, seq, traj = objects(wl)
+    Jv = ca.compute_jacobian(echos, (d1, d2), params, traj, wl.coils, v).collect().astype(np.complex128)
+    JHy = ca.compute_jacobian_hermitian(echos, (d1, d2), params, traj, wl.coils, y).collect().astype(np.complex128)
+    lhs = np.vdot(y.ravel().astype(np.complex128), Jv.ravel())
+    rhs = np.vdot(JHy.ravel(), v.ravel().astype(np.complex128))
+    assert abs(lhs - rhs) / abs(lhs) < SIG_TOL
+
+
+def test_jacobian_shape_errors(ctx, oracle):
+    wl, echos, d1, d2, v, y = jac_inputs(oracle, scale=16)
+    params, seq, traj = objects(wl)
+    with pytest.raises(ca.CompasError):
+        ca.compute_jacobian(echos, (d1, d2), params, traj, wl.coils, v[:3])
+    with pytest.raises(ca.CompasError):
+        ca.compute_jacobian_hermitian(echos, (d1, d2), params, traj, wl.coils, y[:, :, :-1])
+
+
+# ---------------------------------------------------- derivative, misc ---
+
+@pytest.mark.parametrize("field", [ca.T1, ca.T2, ca.B1, ca.B0])
+def test_fisp_finite_difference_derivative(ctx, oracle, field):
+    t, seq = small_fisp(nv=128, nTR=40, N=16)
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    s = ca.FispSequence.from_sequence(seq)
+    echos = ca.simulate_magnetization(p, s)
+    delta = 1e-4
+    got = ca.simulate_magnetization_derivative(field, echos, p, s, delta).collect()
+    name = ["T1", "T2", "B1", "B0"][field]
+    tp = t.replace(**{name: (getattr(t, name) + np.float32(delta)).astype(np.float32)})
+    want = (oracle.fisp(tp, seq, "f64") - oracle.fisp(t, seq, "f64")) / np.float32(delta)
+    # forward difference of FP32 magnetisation at delta = 1e-4: cancellation error ~1e-3
+    assert rel(got, want) < 5e-3
+
+
+def test_residual(ctx):
+    rng = W.SplitMix64(3)
+    a = (rng.normal(2 * 30 * 17) + 1j * rng.normal(2 * 30 * 17)).reshape(2, 30, 17).astype(np.complex64)
+    b = (rng.normal(2 * 30 * 17) + 1j * rng.normal(2 * 30 * 17)).reshape(2, 30, 17).astype(np.complex64)
+    obj, diff = ca.compute_residual(a, b)
+    np.testing.assert_allclose(diff.collect(), a - b, rtol=0, atol=1e-6)
+    assert abs(obj - np.sum(np.abs(a - b) ** 2)) / obj < 1e-5
+
+
+def test_arrays_round_trip(ctx):
+    x = (np.arange(24).reshape(2, 3, 4) + 0.5j).astype(np.complex64)
+    assert np.array_equal(ca.CompasArray.from_numpy(x).collect(), x)
+    f = np.linspace(0, 1, 7, dtype=np.float32)
+    assert np.array_equal(ca.CompasArray.from_numpy(f).collect(), f)
