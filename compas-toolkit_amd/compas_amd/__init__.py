@@ -70,6 +70,7 @@ _SIGNATURES = {
     # extensions
     "compas_last_error": (C.c_char_p, []),
     "compas_clear_error": (None, []),
+    "compas_make_fisp_sequence_ex": (_vp, [_vp, _vp, _f, _f, _f, _i, _f, _i, _i, _i, _i]),
     "compas_array_info": (_i, [_vp, C.POINTER(C.c_int), _i64p]),
     "compas_magnetization_to_signal_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _i]),
     "compas_compute_jacobian_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
@@ -84,7 +85,7 @@ REFERENCE_SYMBOLS = [name for name in _SIGNATURES if not name.startswith(("compa
                                                                              "compas_set_kernel",
                                                                              "compas_read_kernel",
                                                                              "compas_trim"))
-                     and not name.endswith("_method")]
+                     and not name.endswith(("_method", "_ex"))]
 
 
 def library_path() -> str:
@@ -263,18 +264,23 @@ class FispSequence(_Object):
     """CompasToolkit.jl:542-575 (slice_profiles in C order: [nslices][nTR])."""
 
     def __init__(self, RF_train, slice_profiles, TR, TE, max_state, TI, undersampling_factor=1, repetitions=1,
-                 TW=0.0):
+                 TW=0.0, inversion_prepulse=True, wait_spoiling=True):
         rf = as_array(np.asarray(RF_train, dtype=np.complex64))
         prof = as_array(np.asarray(slice_profiles, dtype=np.complex64).reshape(-1, rf.shape[0]))
-        ptr = _check(lib().compas_make_fisp_sequence(rf.ptr, prof.ptr, float(TR), float(TE), float(TW),
-                                                     int(max_state), float(TI), int(undersampling_factor),
-                                                     int(repetitions)))
+        args = (rf.ptr, prof.ptr, float(TR), float(TE), float(TW), int(max_state), float(TI),
+                int(undersampling_factor), int(repetitions))
+        if inversion_prepulse and wait_spoiling:
+            ptr = _check(lib().compas_make_fisp_sequence(*args))  # the reference's entry point
+        else:
+            ptr = _check(lib().compas_make_fisp_sequence_ex(*args, int(bool(inversion_prepulse)),
+                                                            int(bool(wait_spoiling))))
         super().__init__(ptr, keep=(rf, prof))
         self.nreadouts = rf.shape[0] * int(undersampling_factor)
 
     @classmethod
     def from_sequence(cls, s) -> "FispSequence":
-        return cls(s.RF, s.profiles, s.TR, s.TE, s.max_state, s.TI, s.undersampling, s.repetitions, s.TW)
+        return cls(s.RF, s.profiles, s.TR, s.TE, s.max_state, s.TI, s.undersampling, s.repetitions, s.TW,
+                   s.inversion, s.wait_spoiling)
 
 
 class pSSFPSequence(_Object):

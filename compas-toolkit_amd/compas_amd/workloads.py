@@ -214,7 +214,7 @@ def make_coils(n: int, fov: float, ncoils: int) -> np.ndarray:
 
 
 def make_pssfp(nTR: int, max_deg: float, nz: int = 35, nRF: int = 25) -> PssfpSeq:
-    t = -1.0 + 2.0 * np.arange(nRF) / (nRF - 1)
+    t = -1.0 + 2.0 * np.arange(nRF) / (nRF - 1) if nRF > 1 else np.zeros(1)
     w = np.sinc(3 * t) * np.exp(-t ** 2 / (2 * 0.5 ** 2))
     g = (np.pi / 180) * w / w.sum()
     durRF, TR, TI = 1e-3, 10e-3, 20e-3

@@ -421,6 +421,17 @@ void compas_clear_error() {
     g_last_error.clear();
 }
 
+CompasObjectHandle* compas_make_fisp_sequence_ex(
+    const CompasArrayHandle* RF_train, const CompasArrayHandle* sliceprofiles, float TR, float TE, float TW,
+    int max_state, float TI, int undersampling_factor, int repetitions, int inversion_prepulse, int wait_spoiling) {
+    return catch_exceptions([&] {
+        auto rf = array(RF_train, "RF_train").as<cfloat, 1>("RF_train");
+        auto prof = array(sliceprofiles, "sliceprofiles").as<cfloat, 2>("sliceprofiles");
+        return new_object(FISPSequence(rf, prof, TR, TE, TW, max_state, TI, undersampling_factor, repetitions,
+                                       inversion_prepulse != 0, wait_spoiling != 0));
+    });
+}
+
 int compas_array_info(const CompasArrayHandle* a, int* dtype, int64_t* sizes3) {
     return catch_exceptions([&] {
         const auto& h = array(a, "array");

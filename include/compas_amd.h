@@ -131,6 +131,13 @@ CompasArrayHandle* compas_simulate_magnetization_derivative_fisp(
 const char* compas_last_error(void);
 void compas_clear_error(void);
 
+/* compas_make_fisp_sequence with the two FISPSequence flags of the C++ API
+ * (include/compas/sequences/fisp.h:37-41) that the reference's C-ABI fixes
+ * to true: the inversion prepulse and spoiling before the next repetition. */
+CompasObjectHandle* compas_make_fisp_sequence_ex(
+    const CompasArrayHandle* RF_train, const CompasArrayHandle* sliceprofiles, float TR, float TE, float TW,
+    int max_state, float TI, int undersampling_factor, int repetitions, int inversion_prepulse, int wait_spoiling);
+
 /* Shape of an array handle: dtype 0 = float32, 1 = complex64. Returns rank. */
 int compas_array_info(const CompasArrayHandle* array, int* dtype, int64_t* sizes3);
 
