@@ -1,0 +1,223 @@
+"""Benchmark of the compas simulation hot path on one MI355X.
+
+Workload (default): FISP EPG simulation of 50176 voxels (224 x 224) x 1120
+TRs with 25 states, then the Cartesian k-space signal of 4 coils x 1120
+readouts x 224 samples -- one "step" is simulate_magnetization followed by
+magnetization_to_signal with all inputs resident in HBM, called through the
+C-ABI of libcompas_amd.so.
+
+Prints one JSON line (rank 0).  `value` is voxel.TR/s of the whole step;
+k-space samples/s, per-step latency percentiles, per-kernel times (HIP events
+on the library's stream), the roofline of the dominant kernel and the CPU
+baseline (the FP32 port of the reference algorithm on the host cores, on a
+bounded sample) are reported beside it.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "compas-toolkit_amd"))
+
+import compas_amd as ca  # noqa: E402
+from compas_amd import workloads as W  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (published)
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16/FP16 MFMA (published)
+PEAK_HBM_GBS = 8000.0
+
+
+def algorithmic_work(kernel: str, wl, method: str):
+    """(amount per launch, unit, bound, peak) of a kernel of this workload."""
+    nv = wl.tissue.nvoxels
+    nTR = int(wl.sequence.RF.shape[0])
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+    if kernel == "fisp_epg":
+        # 82 N flop per voxel.TR (excite 54 N + two rotate_decay 14 N)
+        return 82.0 * wl.sequence.max_state * nv * nTR / 1e12, "TFLOP/s", "valu", PEAK_FP32_TFLOPS
+    if kernel in ("contraction_fp32", "contraction_mfma"):
+        flops = 8.0 * nc * nr * ns * nv / 1e12
+        if kernel == "contraction_mfma":
+            return flops, "TFLOP/s", "mfma", PEAK_BF16_TFLOPS
+        return flops, "TFLOP/s", "valu", PEAK_FP32_TFLOPS
+    return None
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def cpu_baseline(wl, gpu_echos, gpu_signal, seconds_budget=20.0):
+    """FP32 port of the reference algorithm (oracle/, test infrastructure) timed on
+    the host cores on a bounded sample, plus the accuracy of the GPU outputs
+    against the FP64 oracle on the same sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    pyoracle.build()
+    threads = int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    nv = wl.tissue.nvoxels
+    nTR = int(wl.sequence.RF.shape[0])
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+
+    # simulation: contiguous voxel slice (voxels are independent)
+    nsim = max(64, nv // 16)
+    sub = wl.tissue.slice(0, nsim)
+    t0 = time.perf_counter()
+    e32 = pyoracle.fisp(sub, wl.sequence, "f32", threads)
+    t_sim = time.perf_counter() - t0
+    e64 = pyoracle.fisp(sub, wl.sequence, "f64", threads)
+    mag_err64 = pyoracle.rel_l2(gpu_echos[:, :nsim], e64)
+    mag_err32 = pyoracle.rel_l2(gpu_echos[:, :nsim], e32)
+    port_err = pyoracle.rel_l2(e32, e64)
+
+    # signal: a subset of readouts (Cartesian rows are independent), full voxel sum
+    rows = max(1, min(nr, int(round(8 * 50176 / max(nv, 1)))))
+    rows = min(rows, 16)
+    ridx = np.linspace(0, nr - 1, rows).astype(int)
+    traj = W.CartesianTraj(rows, ns, wl.trajectory.delta_t, wl.trajectory.k_start[ridx], wl.trajectory.delta_k)
+    echos_rows = gpu_echos[ridx]
+    t0 = time.perf_counter()
+    s32 = pyoracle.signal_cartesian(wl.tissue, traj, wl.coils, echos_rows.astype(np.complex64), "f32", threads)
+    t_sig = time.perf_counter() - t0
+    s64 = pyoracle.signal_cartesian(wl.tissue, traj, wl.coils, echos_rows.astype(np.complex128), "f64", threads)
+    sig_err64 = pyoracle.rel_l2(gpu_signal[:, ridx, :], s64)
+
+    sim_rate = nsim * nTR / t_sim  # voxel.TR/s
+    sig_rate = nc * rows * ns / t_sig  # k-space samples/s
+    t_step = nv * nTR / sim_rate + nc * nr * ns / sig_rate
+    base = {
+        "value": nv * nTR / t_step,
+        "unit": "voxel*TR/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP32 port of the reference algorithm: FISP on voxels [0, {nsim}) of {nv} "
+                   f"({t_sim:.2f} s), Cartesian signal on {rows} of {nr} readouts x all {nv} voxels "
+                   f"({t_sig:.2f} s); full-step time extrapolated linearly"),
+        "sim_voxel_tr_per_s": sim_rate,
+        "signal_samples_per_s": sig_rate,
+    }
+    accuracy = {
+        "magnetization_rel_l2_vs_fp64": mag_err64,
+        "magnetization_rel_l2_vs_fp32_port": mag_err32,
+        "fp32_port_rel_l2_vs_fp64": port_err,
+        "signal_rel_l2_vs_fp64": sig_err64,
+        "sample": f"voxels [0, {nsim}) for magnetization; readouts {ridx.tolist()} for signal",
+    }
+    return base, accuracy
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--method", default="matmul_fast", choices=list(ca.SIGNAL_METHODS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    wl = W.config(args.workload)
+    nv = wl.tissue.nvoxels
+    nTR = int(wl.sequence.RF.shape[0])
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+
+    ctx = ca.init_context(0)
+    params = ca.TissueParameters.from_tissue(wl.tissue)
+    seq = ca.FispSequence.from_sequence(wl.sequence)
+    traj = ca.CartesianTrajectory.from_trajectory(wl.trajectory)
+    coils = ca.CompasArray.from_numpy(wl.coils)
+    ctx.synchronize()
+
+    def step():
+        echos = ca.simulate_magnetization(params, seq)
+        signal = ca.magnetization_to_signal(echos, params, traj, coils, method=args.method)
+        return echos, signal
+
+    for _ in range(args.warmup):
+        out = step()
+    ctx.synchronize()
+    del out
+
+    # timed region: HIP events around every launch on the library's stream
+    ctx.read_kernel_timings()  # reset
+    ctx.set_kernel_timing(True)
+    lat = []
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = step()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+    t_total = time.perf_counter() - t_start
+    ctx.set_kernel_timing(False)
+    timings = ctx.read_kernel_timings()
+
+    ms_per_step = 1e3 * t_total / args.steps
+    value = nv * nTR * args.steps / t_total
+    kernels = {k: {"launches": c, "avg_ms": tot / max(c, 1), "total_ms": tot} for k, (c, tot) in timings.items()}
+    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
+
+    roofline = None
+    if dominant is not None:
+        work = algorithmic_work(dominant, wl, args.method)
+        if work is not None:
+            amount, unit, bound, peak = work
+            avg_s = kernels[dominant]["avg_ms"] / 1e3
+            achieved = amount / avg_s
+            roofline = {"kernel": dominant, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+                        "frac": achieved / peak, "traffic": None}
+
+    echos, signal = out
+    line = {
+        "metric": "voxel·TR/s sim + k-space samples/s; p50/p99 latency at ≤1e-4 rel-L2 error",
+        "value": value,
+        "unit": "voxel*TR/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 tissue maps, sinusoidal flip train, Gaussian coil maps)",
+        "config": {
+            "workload": f"FISP {nv} voxels x {nTR} TRs x {wl.sequence.max_state} states + Cartesian signal "
+                        f"{nc} coils x {nr} readouts x {ns} samples",
+            "signal_method": args.method,
+        },
+        "kspace_samples_per_s": nc * nr * ns * args.steps / t_total,
+        "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99)},
+        "kernels": kernels,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        e = echos.collect()
+        s = signal.collect()
+        base, acc = cpu_baseline(wl, e, s)
+        line["cpu_baseline"] = base
+        line["accuracy"] = acc
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
