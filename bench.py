@@ -69,8 +69,13 @@ def cpu_baseline(wl, gpu_echos, gpu_signal, seconds_budget=20.0):
     nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
     nc = wl.coils.shape[0]
 
-    # simulation: contiguous voxel slice (voxels are independent)
-    nsim = max(64, nv // 16)
+    # simulation: contiguous voxel slice (voxels are independent), sized from a short
+    # calibration run to take about a third of the time budget
+    ncal = max(64, nv // 64)
+    t0 = time.perf_counter()
+    pyoracle.fisp(wl.tissue.slice(0, ncal), wl.sequence, "f32", threads)
+    t_cal = max(time.perf_counter() - t0, 1e-6)
+    nsim = int(min(nv, max(ncal, ncal * (seconds_budget / 3) / t_cal)))
     sub = wl.tissue.slice(0, nsim)
     t0 = time.perf_counter()
     e32 = pyoracle.fisp(sub, wl.sequence, "f32", threads)
@@ -80,11 +85,19 @@ def cpu_baseline(wl, gpu_echos, gpu_signal, seconds_budget=20.0):
     mag_err32 = pyoracle.rel_l2(gpu_echos[:, :nsim], e32)
     port_err = pyoracle.rel_l2(e32, e64)
 
-    # signal: a subset of readouts (Cartesian rows are independent), full voxel sum
-    rows = max(1, min(nr, int(round(8 * 50176 / max(nv, 1)))))
-    rows = min(rows, 16)
+    # signal: a subset of readouts (Cartesian rows are independent), full voxel sum,
+    # sized from a 2-readout calibration to take about two thirds of the budget
+    def rows_traj(ridx):
+        return W.CartesianTraj(len(ridx), ns, wl.trajectory.delta_t, wl.trajectory.k_start[ridx],
+                               wl.trajectory.delta_k)
+
+    cal = np.array([0, nr - 1])
+    t0 = time.perf_counter()
+    pyoracle.signal_cartesian(wl.tissue, rows_traj(cal), wl.coils, gpu_echos[cal].astype(np.complex64), "f32", threads)
+    t_cal = max(time.perf_counter() - t0, 1e-6)
+    rows = int(min(nr, max(2, 2 * (2 * seconds_budget / 3) / t_cal)))
     ridx = np.linspace(0, nr - 1, rows).astype(int)
-    traj = W.CartesianTraj(rows, ns, wl.trajectory.delta_t, wl.trajectory.k_start[ridx], wl.trajectory.delta_k)
+    traj = rows_traj(ridx)
     echos_rows = gpu_echos[ridx]
     t0 = time.perf_counter()
     s32 = pyoracle.signal_cartesian(wl.tissue, traj, wl.coils, echos_rows.astype(np.complex64), "f32", threads)

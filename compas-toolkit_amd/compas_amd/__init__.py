@@ -144,11 +144,12 @@ class Context:
         _check()
 
     def read_kernel_timings(self) -> dict:
-        n = lib().compas_read_kernel_timings(self.ptr, None, 0)
+        """Per-kernel (launches, total ms) since the last read; resets the counters."""
+        buf = C.create_string_buffer(1 << 20)  # one call: reading resets the counters
+        n = lib().compas_read_kernel_timings(self.ptr, buf, len(buf))
         _check()
-        buf = C.create_string_buffer(max(n, 1))
-        lib().compas_read_kernel_timings(self.ptr, buf, len(buf))
-        _check()
+        if n > len(buf):
+            raise CompasError("kernel timing report truncated")
         out = {}
         for line in buf.value.decode().splitlines():
             name, count, total = line.rsplit(" ", 2)
