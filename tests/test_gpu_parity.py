@@ -159,6 +159,45 @@ def test_cartesian_signal_methods(ctx, oracle, method):
     assert rel(got, want) < SIG_TOL
 
 
+@pytest.mark.parametrize("method", ["matmul_fast", "matmul_pedantic"])
+def test_cartesian_signal_full_size(ctx, oracle, method):
+    # 4 coils x 1120 readouts x 224 samples over 50176 voxels: the FP64 oracle on a
+    # subset of readouts (rows are independent), and the FP32 VALU path on all of them
+    banner(f"cartesian signal full size {method}")
+    wl = W.config(2)
+    params, seq, traj = objects(wl)
+    echos = ca.simulate_magnetization(params, seq)
+    got = ca.magnetization_to_signal(echos, params, traj, wl.coils, method=method).collect()
+    plain = ca.magnetization_to_signal(echos, params, traj, wl.coils, method="direct").collect()
+    assert rel(got, plain) < SIG_TOL
+    e = echos.collect()
+    ridx = np.array([0, 1, 333, 777, 1119])
+    sub = W.CartesianTraj(len(ridx), traj.samples_per_readout, wl.trajectory.delta_t,
+                          wl.trajectory.k_start[ridx], wl.trajectory.delta_k)
+    want = oracle.signal_cartesian(wl.tissue, sub, wl.coils, e[ridx].astype(np.complex128), "f64")
+    assert rel(got[:, ridx, :], want) < SIG_TOL
+    assert rel(plain[:, ridx, :], want) < SIG_TOL
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (5, 3, 7), (129, 65, 33), (200, 300, 1000)])
+def test_contraction_ragged_shapes(ctx, oracle, M, N, K):
+    # the signal entry point is a contraction echos[M][K] x W[N][K]; tiny grids of voxels
+    # with one readout sample make M, N, K arbitrary: compare every method against FP64
+    rng = W.SplitMix64(M * 1000 + N * 10 + K)
+    n = int(np.ceil(np.sqrt(K)))
+    t = W.make_tissue(rng, n, 20.0, 30.0).slice(0, K)
+    traj = W.CartesianTraj(M, 1, 5e-6, np.zeros(M, np.complex64), float(np.float32(2 * np.pi / 20.0)))
+    coils = ((rng.normal(N * K) + 1j * rng.normal(N * K)) / 3).reshape(N, K).astype(np.complex64)
+    echos = ((rng.normal(M * K) + 1j * rng.normal(M * K)) / 3).reshape(M, K).astype(np.complex64)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    ctraj = ca.CartesianTrajectory.from_trajectory(traj)
+    want = oracle.signal_cartesian(t, traj, coils, echos, "f64")
+    for method, tol in (("direct", 1e-5), ("matmul_fast", 1e-4), ("matmul_pedantic", 1e-5), ("matmul_low", 2e-2)):
+        got = ca.magnetization_to_signal(echos, params, ctraj, coils, method=method).collect()
+        assert rel(got, want) < tol, method
+
+
 def test_cartesian_signal_default_abi_call(ctx, oracle):
     wl = W.config(1, scale=2)
     params, seq, traj = objects(wl)
