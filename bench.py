@@ -219,6 +219,7 @@ def main():
         "kspace_samples_per_s": nc * nr * ns * args.steps / t_total,
         "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99)},
         "kernels": kernels,
+        "contraction_tuning": ca.contraction_tuning_report().strip().splitlines(),
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

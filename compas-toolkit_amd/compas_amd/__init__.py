@@ -78,14 +78,25 @@ _SIGNATURES = {
     "compas_set_kernel_timing": (None, [_vp, _i]),
     "compas_read_kernel_timings": (_i, [_vp, C.c_char_p, _i]),
     "compas_trim_memory": (None, [_vp]),
+    "compas_set_contraction_variant": (None, [_i]),
+    "compas_contraction_variant_count": (_i, []),
+    "compas_contraction_tuning_report": (_i, [C.c_char_p, _i]),
 }
 
-REFERENCE_SYMBOLS = [name for name in _SIGNATURES if not name.startswith(("compas_last", "compas_clear",
-                                                                             "compas_array_info",
-                                                                             "compas_set_kernel",
-                                                                             "compas_read_kernel",
-                                                                             "compas_trim"))
-                     and not name.endswith(("_method", "_ex"))]
+
+def set_contraction_variant(variant: int):
+    """Force an MFMA tile variant (>= 0) or restore autotuning (-1)."""
+    lib().compas_set_contraction_variant(int(variant))
+
+
+def contraction_variant_count() -> int:
+    return int(lib().compas_contraction_variant_count())
+
+
+def contraction_tuning_report() -> str:
+    buf = C.create_string_buffer(1 << 16)
+    lib().compas_contraction_tuning_report(buf, len(buf))
+    return buf.value.decode()
 
 
 def library_path() -> str:

@@ -42,6 +42,12 @@ void complex_contraction(
     ContractionShape shape,
     ContractionPrecision precision);
 
+// MFMA tile variants: force one (v < 0 restores autotuning), count them, and
+// report the variant chosen per tuned shape.
+void set_contraction_variant(int v);
+int contraction_variant_count();
+std::string contraction_tuning_report();
+
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);
 

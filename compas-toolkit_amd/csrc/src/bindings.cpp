@@ -13,6 +13,7 @@
 #include <typeinfo>
 
 #include "compas/compas.h"
+#include "compas/internal.h"
 #include "compas_amd.h"
 
 // Provided by libjulia when loaded from CompasToolkit.jl; absent otherwise.
@@ -500,6 +501,26 @@ int compas_read_kernel_timings(const CompasContextHandle* ctx, char* buf, int bu
         std::string s = ss.str();
         if (buf && buflen > 0) {
             size_t n = std::min(size_t(buflen - 1), s.size());
+            std::memcpy(buf, s.data(), n);
+            buf[n] = '\0';
+        }
+        return int(s.size() + 1);
+    });
+}
+
+void compas_set_contraction_variant(int variant) {
+    catch_exceptions([&] { set_contraction_variant(variant); });
+}
+
+int compas_contraction_variant_count() {
+    return contraction_variant_count();
+}
+
+int compas_contraction_tuning_report(char* buf, int buflen) {
+    return catch_exceptions([&] {
+        const std::string s = contraction_tuning_report();
+        if (buf && buflen > 0) {
+            const size_t n = std::min(size_t(buflen - 1), s.size());
             std::memcpy(buf, s.data(), n);
             buf[n] = '\0';
         }

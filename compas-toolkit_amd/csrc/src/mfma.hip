@@ -13,15 +13,25 @@
 // hi*hi + hi*lo + lo*hi with FP32 accumulation in the MFMA, which recovers
 // ~16 (BF16) or ~22 (FP16) bits per product.  BF16x1 uses hi*hi only.
 //
-// Tiling.  256 threads = 4 waves (2 x 2); a workgroup owns 128 rows x 128
-// real columns (64 complex) and walks a K slice in steps of 32 complex
-// elements.  Each wave computes 64 x 64 with 2 x 2 v_mfma_f32_32x32x16
-// blocks.  Workgroups are numbered so that the ones sharing A rows run on the
-// same XCD (consecutive hardware ids are spread over the 8 XCDs), which lets
-// the XCD's L2 serve the re-reads of A across column tiles.  K slices are
-// reduced in FP64 by contraction_reduce.
+// Tiling.  A workgroup of WM x WN waves owns 64 WM rows x 64 WN real columns
+// (32 WN complex) and walks a K slice in steps of BKC complex elements; each
+// wave computes 64 x 64 with 2 x 2 v_mfma_f32_32x32x16 blocks.  The next K
+// step is prefetched into registers while the MFMAs of the current one run.
+// Operands are converted to packed 16-bit pairs on the way into LDS (one
+// v_cvt_pk per pair, 128-bit stores).  Workgroups are numbered so that those
+// sharing A rows run on the same XCD (consecutive hardware ids are spread
+// over the 8 XCDs), so the XCD's L2 serves the re-reads of A across column
+// tiles.  K slices are reduced in FP64.
+//
+// Variants (WM, WN, BKC) are timed on the live shape by a small autotuner on
+// first use and the fastest is cached per shape.
 
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "compas/device/common.h"
 #include "compas/internal.h"
@@ -32,23 +42,14 @@ namespace kernels {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-constexpr int MF_BM = 128;  // A rows per workgroup
-constexpr int MF_BNC = 64;  // complex B columns per workgroup (128 real)
-constexpr int MF_BKC = 32;  // complex K per step (64 real)
-constexpr int MF_LD = 64 + 8;  // LDS row stride in 16-bit elements (padding against bank conflicts)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 enum { P_BF16X3 = 0, P_FP16X3 = 1, P_BF16X1 = 2 };
 
 template<int P>
-struct Half16;
-template<>
-struct Half16<P_BF16X3> {
-    using T = __bf16;
-    using V = bf16x8;
-};
-template<>
-struct Half16<P_BF16X1> {
+struct Half16 {
     using T = __bf16;
     using V = bf16x8;
 };
@@ -57,17 +58,6 @@ struct Half16<P_FP16X3> {
     using T = _Float16;
     using V = f16x8;
 };
-
-template<int P>
-__device__ inline void split_hilo(float x, typename Half16<P>::T& hi, typename Half16<P>::T& lo) {
-    using T = typename Half16<P>::T;
-    hi = T(x);
-    if constexpr (P != P_BF16X1) lo = T(x - float(hi));
-}
-
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // (a, b) -> packed 16-bit pair hi = round(a, b) and lo = round((a, b) - hi),
 // first element in the low half (one v_cvt_pk_* per pair)
@@ -107,12 +97,41 @@ __device__ inline int xcd_logical_id(int b, int total) {
     const int per = (total + 7) / 8;
     const int xcd = b % 8, slot = b / 8;
     const int full = total % 8 == 0 ? 8 : total % 8;  // XCDs that receive `per` ids
-    int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+    const int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
     return base + slot;
 }
 
-template<int P>
-__global__ void __launch_bounds__(256) contraction_mfma(
+template<int P, int WM, int WN, int BKC>
+struct MfmaTile {
+    static constexpr int THREADS = 64 * WM * WN;
+    static constexpr int BM = 64 * WM;  // A rows
+    static constexpr int BNR = 64 * WN;  // real B rows (2 per complex column)
+    static constexpr int BNC = 32 * WN;  // complex B columns
+    static constexpr int LD = 2 * BKC + 8;  // LDS row stride, 16-bit elements (+16 bytes of padding)
+    static constexpr bool LO = (P != P_BF16X1);
+    static constexpr int A_PER = BKC / WN;  // complex per thread per step (contiguous in K)
+    static constexpr int A_TPR = WN;  // threads per A row
+    static constexpr int B_PER = BKC / (2 * WM);
+    static constexpr int B_TPR = 2 * WM;
+    static constexpr int LDS_BYTES = (BM + BNR) * LD * 2 * (LO ? 2 : 1);
+    static_assert(A_PER >= 4 && A_PER % 4 == 0, "A staging granularity");
+    static_assert(B_PER >= 2 && B_PER % 2 == 0, "B staging granularity");
+};
+
+template<int N>
+__device__ inline void store_words(uint32_t* dst, const uint32_t* w) {
+    if constexpr (N % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 4; i++)
+            reinterpret_cast<uint4*>(dst)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N / 2; i++) reinterpret_cast<uint2*>(dst)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+    }
+}
+
+template<int P, int WM, int WN, int BKC>
+__global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const cfloat* __restrict__ A,
     const cfloat* __restrict__ B,
     int M, int N, int K, int64_t lda, int64_t ldb,
@@ -121,28 +140,30 @@ __global__ void __launch_bounds__(256) contraction_mfma(
     const float* __restrict__ row_scale_b,
     float2* __restrict__ partial,
     cfloat* __restrict__ direct, int nsub, int64_t out_outer) {
+    using Tl = MfmaTile<P, WM, WN, BKC>;
     using T = typename Half16<P>::T;
     using V = typename Half16<P>::V;
-    constexpr bool LO = (P != P_BF16X1);
+    constexpr bool LO = Tl::LO;
+    constexpr int LD = Tl::LD;
 
-    __shared__ __attribute__((aligned(16))) T sA_hi[MF_BM * MF_LD];
-    __shared__ __attribute__((aligned(16))) T sB_hi[2 * MF_BNC * MF_LD];
-    __shared__ __attribute__((aligned(16))) T sA_lo[LO ? MF_BM * MF_LD : 8];
-    __shared__ __attribute__((aligned(16))) T sB_lo[LO ? 2 * MF_BNC * MF_LD : 8];
+    __shared__ __attribute__((aligned(16))) T sA_hi[Tl::BM * LD];
+    __shared__ __attribute__((aligned(16))) T sB_hi[Tl::BNR * LD];
+    __shared__ __attribute__((aligned(16))) T sA_lo[LO ? Tl::BM * LD : 8];
+    __shared__ __attribute__((aligned(16))) T sB_lo[LO ? Tl::BNR * LD : 8];
 
     const int total = tiles_m * tiles_n * splits;
     const int lid = xcd_logical_id(blockIdx.x, total);
     if (lid >= total) return;
     const int tn = lid % tiles_n;
     const int tm = (lid / tiles_n) % tiles_m;
-    const int split = lid / (tiles_n * tiles_m);
-    const int m0 = tm * MF_BM, n0 = tn * MF_BNC;
-    const int kb = split * k_chunk;
+    const int ks = lid / (tiles_n * tiles_m);
+    const int m0 = tm * Tl::BM, n0 = tn * Tl::BNC;
+    const int kb = ks * k_chunk;
     const int ke = min(K, kb + k_chunk);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WN, wn = wave % WN;
 
     f32x16 acc[2][2];
 #pragma unroll
@@ -152,9 +173,8 @@ __global__ void __launch_bounds__(256) contraction_mfma(
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
 
-    // staging assignment: A -> row tid/2, 16 complex; B -> row tid/4, 8 complex
-    const int a_row = tid >> 1, a_k = (tid & 1) * 16;
-    const int b_row = tid >> 2, b_k = (tid & 3) * 8;
+    const int a_row = tid / Tl::A_TPR, a_k = (tid % Tl::A_TPR) * Tl::A_PER;
+    const int b_row = tid / Tl::B_TPR, b_k = (tid % Tl::B_TPR) * Tl::B_PER;
     const int am = m0 + a_row, bn = n0 + b_row;
     const float sa = (row_scale_a && am < M) ? row_scale_a[am] : 1.0f;
     const float sb = (row_scale_b && bn < N) ? row_scale_b[bn] : 1.0f;
@@ -164,56 +184,50 @@ __global__ void __launch_bounds__(256) contraction_mfma(
     const bool vec_a = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 2 == 0);
     const bool vec_b = ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % 2 == 0);
 
-    // register prefetch of one K step: 16 complex of A, 8 complex of B
-    cfloat ra[16], rb[8];
+    // register prefetch of one K step
+    cfloat ra[Tl::A_PER], rb[Tl::B_PER];
     auto gload = [&](int k0) {
         const int ka = k0 + a_k;
-        if (vec_a && am < M && ka + 16 <= ke) {
+        if (vec_a && am < M && ka + Tl::A_PER <= ke) {
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
+            for (int i = 0; i < Tl::A_PER / 2; i++) {
                 const float4 v = *reinterpret_cast<const float4*>(a_src + ka + 2 * i);
                 ra[2 * i] = {v.x, v.y};
                 ra[2 * i + 1] = {v.z, v.w};
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < 16; i++) ra[i] = (am < M && ka + i < ke) ? a_src[ka + i] : cfloat {0, 0};
+            for (int i = 0; i < Tl::A_PER; i++) ra[i] = (am < M && ka + i < ke) ? a_src[ka + i] : cfloat {0, 0};
         }
         const int kb_ = k0 + b_k;
-        if (vec_b && bn < N && kb_ + 8 <= ke) {
+        if (vec_b && bn < N && kb_ + Tl::B_PER <= ke) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+            for (int i = 0; i < Tl::B_PER / 2; i++) {
                 const float4 v = *reinterpret_cast<const float4*>(b_src + kb_ + 2 * i);
                 rb[2 * i] = {v.x, v.y};
                 rb[2 * i + 1] = {v.z, v.w};
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < 8; i++) rb[i] = (bn < N && kb_ + i < ke) ? b_src[kb_ + i] : cfloat {0, 0};
+            for (int i = 0; i < Tl::B_PER; i++) rb[i] = (bn < N && kb_ + i < ke) ? b_src[kb_ + i] : cfloat {0, 0};
         }
     };
 
     if (kb < ke) gload(kb);
-    for (int k0 = kb; k0 < ke; k0 += MF_BKC) {
-        // ---- stage A: (re, im) pairs packed hi / lo, 16-byte LDS stores
+    for (int k0 = kb; k0 < ke; k0 += BKC) {
+        // ---- stage A: (re, im) pairs packed hi / lo
         {
-            uint32_t h[16], l[16];
+            uint32_t h[Tl::A_PER], l[Tl::A_PER];
 #pragma unroll
-            for (int i = 0; i < 16; i++) split_pair<P>(ra[i].re * sa, ra[i].im * sa, h[i], l[i]);
-            uint4* dh = reinterpret_cast<uint4*>(sA_hi + a_row * MF_LD + 2 * a_k);
-#pragma unroll
-            for (int i = 0; i < 4; i++) dh[i] = make_uint4(h[4 * i], h[4 * i + 1], h[4 * i + 2], h[4 * i + 3]);
-            if constexpr (LO) {
-                uint4* dl = reinterpret_cast<uint4*>(sA_lo + a_row * MF_LD + 2 * a_k);
-#pragma unroll
-                for (int i = 0; i < 4; i++) dl[i] = make_uint4(l[4 * i], l[4 * i + 1], l[4 * i + 2], l[4 * i + 3]);
-            }
+            for (int i = 0; i < Tl::A_PER; i++) split_pair<P>(ra[i].re * sa, ra[i].im * sa, h[i], l[i]);
+            store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sA_hi + a_row * LD + 2 * a_k), h);
+            if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sA_lo + a_row * LD + 2 * a_k), l);
         }
         // ---- stage B: conj row (re, -im) and swap row (im, re)
         {
-            uint32_t ch[8], sh[8], cl[8], sl[8];
+            uint32_t ch[Tl::B_PER], sh[Tl::B_PER], cl[Tl::B_PER], sl[Tl::B_PER];
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
+            for (int i = 0; i < Tl::B_PER; i++) {
                 uint32_t h, l;
                 split_pair<P>(rb[i].re * sb, rb[i].im * sb, h, l);
                 ch[i] = h ^ 0x80000000u;  // negate the imaginary half
@@ -221,41 +235,31 @@ __global__ void __launch_bounds__(256) contraction_mfma(
                 cl[i] = l ^ 0x80000000u;
                 sl[i] = __builtin_amdgcn_alignbit(l, l, 16);
             }
-            uint4* dch = reinterpret_cast<uint4*>(sB_hi + (2 * b_row) * MF_LD + 2 * b_k);
-            uint4* dsh = reinterpret_cast<uint4*>(sB_hi + (2 * b_row + 1) * MF_LD + 2 * b_k);
-#pragma unroll
-            for (int i = 0; i < 2; i++) {
-                dch[i] = make_uint4(ch[4 * i], ch[4 * i + 1], ch[4 * i + 2], ch[4 * i + 3]);
-                dsh[i] = make_uint4(sh[4 * i], sh[4 * i + 1], sh[4 * i + 2], sh[4 * i + 3]);
-            }
+            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_hi + (2 * b_row) * LD + 2 * b_k), ch);
+            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_hi + (2 * b_row + 1) * LD + 2 * b_k), sh);
             if constexpr (LO) {
-                uint4* dcl = reinterpret_cast<uint4*>(sB_lo + (2 * b_row) * MF_LD + 2 * b_k);
-                uint4* dsl = reinterpret_cast<uint4*>(sB_lo + (2 * b_row + 1) * MF_LD + 2 * b_k);
-#pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    dcl[i] = make_uint4(cl[4 * i], cl[4 * i + 1], cl[4 * i + 2], cl[4 * i + 3]);
-                    dsl[i] = make_uint4(sl[4 * i], sl[4 * i + 1], sl[4 * i + 2], sl[4 * i + 3]);
-                }
+                store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_lo + (2 * b_row) * LD + 2 * b_k), cl);
+                store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_lo + (2 * b_row + 1) * LD + 2 * b_k), sl);
             }
         }
         __syncthreads();
-        if (k0 + MF_BKC < ke) gload(k0 + MF_BKC);  // in flight during the MFMAs below
+        if (k0 + BKC < ke) gload(k0 + BKC);  // in flight during the MFMAs below
 
 #pragma unroll
-        for (int kk = 0; kk < 4; kk++) {
+        for (int kk = 0; kk < BKC / 8; kk++) {
             const int koff = kk * 16 + (lane >> 5) * 8;
             V ah[2], al[2], bh[2], bl[2];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
                 const int r = wm * 64 + i * 32 + (lane & 31);
-                ah[i] = *reinterpret_cast<const V*>(sA_hi + r * MF_LD + koff);
-                if constexpr (LO) al[i] = *reinterpret_cast<const V*>(sA_lo + r * MF_LD + koff);
+                ah[i] = *reinterpret_cast<const V*>(sA_hi + r * LD + koff);
+                if constexpr (LO) al[i] = *reinterpret_cast<const V*>(sA_lo + r * LD + koff);
             }
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int c = wn * 64 + j * 32 + (lane & 31);
-                bh[j] = *reinterpret_cast<const V*>(sB_hi + c * MF_LD + koff);
-                if constexpr (LO) bl[j] = *reinterpret_cast<const V*>(sB_lo + c * MF_LD + koff);
+                bh[j] = *reinterpret_cast<const V*>(sB_hi + c * LD + koff);
+                if constexpr (LO) bl[j] = *reinterpret_cast<const V*>(sB_lo + c * LD + koff);
             }
 #pragma unroll
             for (int i = 0; i < 2; i++)
@@ -288,7 +292,7 @@ __global__ void __launch_bounds__(256) contraction_mfma(
                     const int64_t o = int64_t(n / nsub) * out_outer + int64_t(m) * nsub + (n % nsub);
                     reinterpret_cast<float*>(direct)[2 * o + part] = v;
                 } else {
-                    reinterpret_cast<float*>(partial)[2 * ((int64_t(split) * M + m) * N + n) + part] = v;
+                    reinterpret_cast<float*>(partial)[2 * ((int64_t(ks) * M + m) * N + n) + part] = v;
                 }
             }
 }
@@ -312,9 +316,7 @@ __global__ void __launch_bounds__(256) row_pow2_scale(const cfloat* __restrict__
     if (threadIdx.x == 0) {
         const float m = red[0];
         int e = 0;
-        if (m > 0.0f && isfinite(m)) {
-            frexpf(m, &e);  // m = f * 2^e, f in [0.5, 1)
-        }
+        if (m > 0.0f && isfinite(m)) frexpf(m, &e);  // m = f * 2^e, f in [0.5, 1)
         scale[r] = ldexpf(1.0f, 14 - e);
     }
 }
@@ -341,7 +343,11 @@ __global__ void contraction_reduce_mf(
 
 }  // namespace kernels
 
-static int cu_count(int device) {
+namespace {
+
+using namespace kernels;
+
+int cu_count(int device) {
     static int cached[64] = {0};
     if (device >= 0 && device < 64 && cached[device]) return cached[device];
     hipDeviceProp_t prop;
@@ -350,21 +356,167 @@ static int cu_count(int device) {
     return prop.multiProcessorCount;
 }
 
-template<int P>
-static void launch_mfma(const CompasContext& ctx, const cfloat* A, const cfloat* B, cfloat* out, ContractionShape s,
-                        const char* name) {
-    using namespace kernels;
-    const int tiles_m = int(dev::div_ceil(s.M, MF_BM));
-    const int tiles_n = int(dev::div_ceil(s.N, MF_BNC));
-    const int tiles = tiles_m * tiles_n;
-    // enough workgroups for ~2 per CU, K slices of at least 8 steps
-    const int target = 2 * cu_count(ctx.device());
-    int splits = std::max(1, (target + tiles - 1) / tiles);
-    splits = std::min(splits, std::max(1, s.K / (8 * MF_BKC)));
-    int k_chunk = int(dev::div_ceil(dev::div_ceil(s.K, splits), MF_BKC)) * MF_BKC;
-    splits = std::max(1, int(dev::div_ceil(s.K, k_chunk)));
-    const int total = tiles * splits;
+struct Plan {
+    int tiles_m, tiles_n, splits, k_chunk, threads;
+};
 
+template<int P, int WM, int WN, int BKC>
+Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
+    using Tl = MfmaTile<P, WM, WN, BKC>;
+    Plan p;
+    p.threads = Tl::THREADS;
+    p.tiles_m = int(dev::div_ceil(s.M, Tl::BM));
+    p.tiles_n = int(dev::div_ceil(s.N, Tl::BNC));
+    const int tiles = p.tiles_m * p.tiles_n;
+    const int per_cu = std::max(1, std::min(4, (160 * 1024) / Tl::LDS_BYTES));
+    const int target = per_cu * cu_count(ctx.device());
+    int splits = std::max(1, (target + tiles - 1) / tiles);
+    splits = std::min(splits, std::max(1, s.K / (8 * BKC)));
+    p.k_chunk = std::max(BKC, int(dev::div_ceil(dev::div_ceil(s.K, splits), BKC)) * BKC);
+    p.splits = std::max(1, int(dev::div_ceil(s.K, p.k_chunk)));
+    return p;
+}
+
+template<int P, int WM, int WN, int BKC>
+void launch_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, cfloat* out,
+                    const ContractionShape& s, const float* sa, const float* sb) {
+    const Plan p = make_plan<P, WM, WN, BKC>(ctx, s);
+    const int total = p.tiles_m * p.tiles_n * p.splits;
+    const bool direct = (p.splits == 1 && !s.accumulate);
+    Array<float2, 1> partial;
+    if (!direct) partial = ctx.empty<float2, 1>({int64_t(p.splits) * s.M * s.N});
+    {
+        KernelScope ks(ctx, "contraction_mfma");
+        contraction_mfma<P, WM, WN, BKC><<<total, p.threads, 0, ctx.stream()>>>(
+            A, B, s.M, s.N, s.K, s.lda, s.ldb, p.k_chunk, p.tiles_m, p.tiles_n, p.splits, sa, sb,
+            direct ? nullptr : partial.data(), direct ? out : nullptr, s.nsub, s.out_outer);
+        check_launch("contraction_mfma");
+    }
+    if (!direct) {
+        KernelScope ks(ctx, "contraction_reduce");
+        const int64_t n = int64_t(s.M) * s.N;
+        contraction_reduce_mf<<<dev::div_ceil(n, 256), 256, 0, ctx.stream()>>>(
+            partial.data(), p.splits, s.M, s.N, out, s.nsub, s.out_outer, s.accumulate);
+        check_launch("contraction_reduce");
+    }
+}
+
+using VariantFn = void (*)(const CompasContext&, const cfloat*, const cfloat*, cfloat*, const ContractionShape&,
+                           const float*, const float*);
+
+struct Variant {
+    const char* name;
+    VariantFn fn[3];  // indexed by P
+};
+
+#define COMPAS_MFMA_VARIANT(WM, WN, BKC)                                                          \
+    {#WM "x" #WN "x" #BKC,                                                                        \
+     {&launch_variant<P_BF16X3, WM, WN, BKC>, &launch_variant<P_FP16X3, WM, WN, BKC>,             \
+      &launch_variant<P_BF16X1, WM, WN, BKC>}}
+
+const Variant VARIANTS[] = {
+    COMPAS_MFMA_VARIANT(2, 2, 32),
+    COMPAS_MFMA_VARIANT(4, 2, 32),
+    COMPAS_MFMA_VARIANT(2, 4, 32),
+    COMPAS_MFMA_VARIANT(4, 2, 16),
+    COMPAS_MFMA_VARIANT(2, 4, 16),
+    COMPAS_MFMA_VARIANT(4, 4, 16),
+};
+constexpr int NUM_VARIANTS = int(sizeof(VARIANTS) / sizeof(VARIANTS[0]));
+constexpr int DEFAULT_VARIANT = 1;
+
+std::mutex g_tune_mutex;
+std::map<std::tuple<int, int, int, int64_t, int64_t, int, int, bool>, int> g_tuned;
+int g_forced_variant = -1;
+
+bool autotune_enabled() {
+    const char* e = std::getenv("COMPAS_AUTOTUNE");
+    return !(e && e[0] == '0');
+}
+
+int pick_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, const ContractionShape& s, int P,
+                 const float* sa, const float* sb) {
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        if (g_forced_variant >= 0) return g_forced_variant;
+    }
+    const auto key = std::make_tuple(ctx.device(), s.M, s.N, s.lda, s.ldb, s.K, P, s.accumulate);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
+    }
+    if (!autotune_enabled() || int64_t(s.M) * s.N * s.K < (int64_t(1) << 24)) return DEFAULT_VARIANT;
+    // time every variant into a scratch output (the caller's output is not touched)
+    auto scratch = ctx.empty<cfloat, 1>({int64_t(s.M) * s.N});
+    ContractionShape t = s;
+    t.nsub = s.N;
+    t.out_outer = 0;
+    t.accumulate = false;
+    hipEvent_t e0, e1;
+    COMPAS_HIP_CHECK(hipEventCreate(&e0));
+    COMPAS_HIP_CHECK(hipEventCreate(&e1));
+    int best = DEFAULT_VARIANT;
+    float best_ms = 1e30f;
+    for (int v = 0; v < NUM_VARIANTS; v++) {
+        VARIANTS[v].fn[P](ctx, A, B, scratch.data(), t, sa, sb);  // warm-up
+        COMPAS_HIP_CHECK(hipEventRecord(e0, ctx.stream()));
+        for (int r = 0; r < 2; r++) VARIANTS[v].fn[P](ctx, A, B, scratch.data(), t, sa, sb);
+        COMPAS_HIP_CHECK(hipEventRecord(e1, ctx.stream()));
+        COMPAS_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        COMPAS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[key] = best;
+    return best;
+}
+
+}  // namespace
+
+void set_contraction_variant(int v) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_forced_variant = (v >= 0 && v < NUM_VARIANTS) ? v : -1;
+}
+
+int contraction_variant_count() {
+    return NUM_VARIANTS;
+}
+
+std::string contraction_tuning_report() {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    std::string out;
+    for (auto& kv : g_tuned) {
+        char line[256];
+        std::snprintf(line, sizeof(line), "M=%d N=%d K=%d prec=%d -> %s\n", std::get<1>(kv.first),
+                      std::get<2>(kv.first), std::get<5>(kv.first), std::get<6>(kv.first), VARIANTS[kv.second].name);
+        out += line;
+    }
+    return out;
+}
+
+void complex_contraction_mfma(
+    const CompasContext& ctx,
+    const cfloat* A,
+    const cfloat* B,
+    cfloat* out,
+    ContractionShape s,
+    ContractionPrecision precision,
+    bool* handled) {
+    int P;
+    switch (precision) {
+        case ContractionPrecision::BF16x3: P = P_BF16X3; break;
+        case ContractionPrecision::FP16x3: P = P_FP16X3; break;
+        case ContractionPrecision::BF16x1: P = P_BF16X1; break;
+        default: *handled = false; return;
+    }
+    *handled = true;
     Array<float, 1> scale_a, scale_b;
     const float* sa = nullptr;
     const float* sb = nullptr;
@@ -378,48 +530,8 @@ static void launch_mfma(const CompasContext& ctx, const cfloat* A, const cfloat*
         sa = scale_a.data();
         sb = scale_b.data();
     }
-
-    const bool direct = (splits == 1 && !s.accumulate);
-    Array<float2, 1> partial;
-    if (!direct) partial = ctx.empty<float2, 1>({int64_t(splits) * s.M * s.N});
-    {
-        KernelScope ks(ctx, name);
-        contraction_mfma<P><<<total, 256, 0, ctx.stream()>>>(
-            A, B, s.M, s.N, s.K, s.lda, s.ldb, k_chunk, tiles_m, tiles_n, splits, sa, sb,
-            direct ? nullptr : partial.data(), direct ? out : nullptr, s.nsub, s.out_outer);
-        check_launch(name);
-    }
-    if (!direct) {
-        KernelScope ks(ctx, "contraction_reduce");
-        const int64_t n = int64_t(s.M) * s.N;
-        contraction_reduce_mf<<<dev::div_ceil(n, 256), 256, 0, ctx.stream()>>>(
-            partial.data(), splits, s.M, s.N, out, s.nsub, s.out_outer, s.accumulate);
-        check_launch("contraction_reduce");
-    }
-}
-
-void complex_contraction_mfma(
-    const CompasContext& ctx,
-    const cfloat* A,
-    const cfloat* B,
-    cfloat* out,
-    ContractionShape s,
-    ContractionPrecision precision,
-    bool* handled) {
-    *handled = true;
-    switch (precision) {
-        case ContractionPrecision::BF16x3:
-            launch_mfma<kernels::P_BF16X3>(ctx, A, B, out, s, "contraction_mfma");
-            return;
-        case ContractionPrecision::FP16x3:
-            launch_mfma<kernels::P_FP16X3>(ctx, A, B, out, s, "contraction_mfma");
-            return;
-        case ContractionPrecision::BF16x1:
-            launch_mfma<kernels::P_BF16X1>(ctx, A, B, out, s, "contraction_mfma");
-            return;
-        default:
-            *handled = false;
-    }
+    const int v = pick_variant(ctx, A, B, s, P, sa, sb);
+    VARIANTS[v].fn[P](ctx, A, B, out, s, sa, sb);
 }
 
 }  // namespace compas

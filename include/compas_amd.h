@@ -164,6 +164,15 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_method(
 void compas_set_kernel_timing(const CompasContextHandle* ctx, int enabled);
 int compas_read_kernel_timings(const CompasContextHandle* ctx, char* buf, int buflen);
 
+/* MFMA contraction tile variants.  By default each new contraction shape is
+ * timed once over all variants and the fastest is cached (set the environment
+ * variable COMPAS_AUTOTUNE=0 to use a fixed default).  A variant >= 0 forces
+ * that variant for every call (tests use this to check each one); -1 restores
+ * autotuning.  The report lists "M=.. N=.. K=.. prec=.. -> variant" lines. */
+void compas_set_contraction_variant(int variant);
+int compas_contraction_variant_count(void);
+int compas_contraction_tuning_report(char* buf, int buflen);
+
 /* Release cached device memory held by the context's pool. */
 void compas_trim_memory(const CompasContextHandle* ctx);
 

@@ -198,6 +198,29 @@ def test_contraction_ragged_shapes(ctx, oracle, M, N, K):
         assert rel(got, want) < tol, method
 
 
+def test_every_mfma_variant(ctx, oracle):
+    # each tile variant the autotuner may pick, on a ragged shape, every precision
+    rng = W.SplitMix64(77)
+    M, N, K = 261, 131, 700
+    n = int(np.ceil(np.sqrt(K)))
+    t = W.make_tissue(rng, n, 20.0, 30.0).slice(0, K)
+    traj = W.CartesianTraj(M, 1, 5e-6, np.zeros(M, np.complex64), float(np.float32(2 * np.pi / 20.0)))
+    coils = ((rng.normal(N * K) + 1j * rng.normal(N * K)) / 3).reshape(N, K).astype(np.complex64)
+    echos = ((rng.normal(M * K) + 1j * rng.normal(M * K)) / 3).reshape(M, K).astype(np.complex64)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    ctraj = ca.CartesianTrajectory.from_trajectory(traj)
+    want = oracle.signal_cartesian(t, traj, coils, echos, "f64")
+    try:
+        for v in range(ca.contraction_variant_count()):
+            ca.set_contraction_variant(v)
+            for method, tol in (("matmul_fast", 1e-4), ("matmul_pedantic", 1e-5), ("matmul_low", 2e-2)):
+                got = ca.magnetization_to_signal(echos, params, ctraj, coils, method=method).collect()
+                assert rel(got, want) < tol, (v, method)
+    finally:
+        ca.set_contraction_variant(-1)
+
+
 def test_cartesian_signal_default_abi_call(ctx, oracle):
     wl = W.config(1, scale=2)
     params, seq, traj = objects(wl)
