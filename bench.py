@@ -51,6 +51,22 @@ def algorithmic_work(kernel: str, wl, method: str):
     return None
 
 
+def measured_traffic(kernel: str, workload: int):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by the same
+    counters of the stream_copy dispatch of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    entry = data.get(f"workload{workload}", {}).get(kernel)
+    if not entry:
+        return None
+    return entry.get("bytes_per_launch")
+
+
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
@@ -197,6 +213,13 @@ def main():
             roofline = {"kernel": dominant, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
                         "frac": achieved / peak, "traffic": None}
 
+    # plain streaming copy of known size: achievable HBM bandwidth, and the
+    # reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters to bytes
+    probe_bytes = 1 << 30
+    probe_ms = ctx.bandwidth_probe(probe_bytes)
+    if roofline is not None:
+        roofline["traffic"] = measured_traffic(roofline["kernel"], args.workload)
+
     echos, signal = out
     line = {
         "metric": "voxel·TR/s sim + k-space samples/s; p50/p99 latency at ≤1e-4 rel-L2 error",
@@ -219,6 +242,7 @@ def main():
         "kspace_samples_per_s": nc * nr * ns * args.steps / t_total,
         "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99)},
         "kernels": kernels,
+        "hbm_copy_GBs": 2 * probe_bytes / (probe_ms * 1e-3) / 1e9,
         "contraction_tuning": ca.contraction_tuning_report().strip().splitlines(),
         "roofline": roofline,
     }

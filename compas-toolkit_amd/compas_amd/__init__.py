@@ -78,6 +78,7 @@ _SIGNATURES = {
     "compas_set_kernel_timing": (None, [_vp, _i]),
     "compas_read_kernel_timings": (_i, [_vp, C.c_char_p, _i]),
     "compas_trim_memory": (None, [_vp]),
+    "compas_bandwidth_probe": (_f, [_vp, C.c_int64]),
     "compas_set_contraction_variant": (None, [_i]),
     "compas_contraction_variant_count": (_i, []),
     "compas_contraction_tuning_report": (_i, [C.c_char_p, _i]),
@@ -169,6 +170,12 @@ class Context:
 
     def trim_memory(self):
         lib().compas_trim_memory(self.ptr)
+
+    def bandwidth_probe(self, nbytes: int) -> float:
+        """Milliseconds of one streaming copy of `nbytes` (kernel stream_copy)."""
+        ms = lib().compas_bandwidth_probe(self.ptr, int(nbytes))
+        _check()
+        return float(ms)
 
     def __del__(self):
         if getattr(self, "ptr", None) and _lib is not None:

@@ -47,6 +47,35 @@ __device__ inline float reduce_phase(double phi) {
     return float(fma(-k, TWO_PI, phi));
 }
 
+// sin and cos of a float argument: Cody-Waite reduction by pi/2 with a
+// two-constant split, then the Taylor series of sin and cos on [-pi/4, pi/4]
+// evaluated by Horner's rule (truncation below 2e-9), quadrant by the
+// reduction index.  Arguments beyond |x| = 64 use the library sincosf.
+__device__ inline void sincos_fast(float x, float* s, float* c) {
+    if (fabsf(x) > 64.0f) {
+        sincosf(x, s, c);
+        return;
+    }
+    const float kf = rintf(x * 0.63661977236758134f);
+    float r = fmaf(-kf, 1.5707963705062866f, x);  // float(pi/2)
+    r = fmaf(-kf, -4.3711388286737929e-08f, r);  // pi/2 - float(pi/2)
+    const float r2 = r * r;
+    float ps = fmaf(r2, 2.7557319e-6f, -1.9841270e-4f);  // 1/9!, -1/7!
+    ps = fmaf(r2, ps, 8.3333333e-3f);
+    ps = fmaf(r2, ps, -1.6666667e-1f);
+    const float sr = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, -2.7557319e-7f, 2.4801587e-5f);  // -1/10!, 1/8!
+    pc = fmaf(r2, pc, -1.3888889e-3f);
+    pc = fmaf(r2, pc, 4.1666668e-2f);
+    pc = fmaf(r2, pc, -0.5f);
+    const float cr = fmaf(r2, pc, 1.0f);
+    const int q = int(kf);
+    const float ss = (q & 1) ? cr : sr;
+    const float cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+
 __device__ inline cfloat polar_f(float mag, float phase) {
     float s, c;
     sincosf(phase, &s, &c);

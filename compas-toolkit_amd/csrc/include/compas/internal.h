@@ -48,6 +48,10 @@ void set_contraction_variant(int v);
 int contraction_variant_count();
 std::string contraction_tuning_report();
 
+// Copies `nbytes` between two device buffers with a plain streaming kernel
+// (after one warm-up) and returns the elapsed milliseconds.
+float bandwidth_probe(const CompasContext& ctx, int64_t nbytes);
+
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);
 

@@ -528,6 +528,10 @@ int compas_contraction_tuning_report(char* buf, int buflen) {
     });
 }
 
+float compas_bandwidth_probe(const CompasContextHandle* ctx, int64_t nbytes) {
+    return catch_exceptions([&] { return bandwidth_probe(context(ctx), nbytes); });
+}
+
 void compas_trim_memory(const CompasContextHandle* ctx) {
     catch_exceptions([&] { context(ctx).runtime()->trim(); });
 }

@@ -71,7 +71,39 @@ __global__ void __launch_bounds__(256) sum_partials(const double* __restrict__ p
     if (threadIdx.x == 0) *out = float(red[0]);
 }
 
+// Plain streaming copy (16-byte accesses): the bandwidth probe and the
+// reference point that scales hardware traffic counters to bytes.
+__global__ void __launch_bounds__(256) stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace kernels
+
+float bandwidth_probe(const CompasContext& ctx, int64_t nbytes) {
+    ctx.activate();
+    const int64_t n = std::max<int64_t>(1, nbytes / 16);
+    auto a = ctx.empty<uint4, 1>({n});
+    auto b = ctx.empty<uint4, 1>({n});
+    const unsigned blocks = unsigned(std::min<int64_t>((n + 255) / 256, 256 * 64));
+    hipEvent_t e0, e1;
+    COMPAS_HIP_CHECK(hipEventCreate(&e0));
+    COMPAS_HIP_CHECK(hipEventCreate(&e1));
+    kernels::stream_copy<<<blocks, 256, 0, ctx.stream()>>>(a.data(), b.data(), n);  // warm-up
+    COMPAS_HIP_CHECK(hipEventRecord(e0, ctx.stream()));
+    {
+        KernelScope ks(ctx, "stream_copy");
+        kernels::stream_copy<<<blocks, 256, 0, ctx.stream()>>>(a.data(), b.data(), n);
+        check_launch("stream_copy");
+    }
+    COMPAS_HIP_CHECK(hipEventRecord(e1, ctx.stream()));
+    COMPAS_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    COMPAS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
 
 static unsigned grid_for(int64_t n) {
     return unsigned(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096));

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(64) fisp_epg(FispArgs a) {
                 const float4 t = tab[n];  // wave-uniform
                 const float alpha = t.x * B1;
                 float sx, cx;
-                sincosf(0.5f * alpha, &sx, &cx);
+                dev::sincos_fast(0.5f * alpha, &sx, &cx);
                 const float sa = 2.0f * sx * cx;  // sin(alpha)
                 const float omc = 2.0f * sx * sx;  // 1 - cos(alpha), without cancellation
                 const float cp = t.y, sp = t.z;

@@ -423,7 +423,7 @@ const Variant VARIANTS[] = {
     COMPAS_MFMA_VARIANT(4, 4, 16),
 };
 constexpr int NUM_VARIANTS = int(sizeof(VARIANTS) / sizeof(VARIANTS[0]));
-constexpr int DEFAULT_VARIANT = 1;
+constexpr int DEFAULT_VARIANT = 4;  // 2x4x16: fastest at 1120 x 896 x 50176 on MI355X
 
 std::mutex g_tune_mutex;
 std::map<std::tuple<int, int, int, int64_t, int64_t, int, int, bool>, int> g_tuned;
@@ -445,6 +445,10 @@ int pick_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, con
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) return it->second;
+    }
+    if (const char* e = std::getenv("COMPAS_CONTRACTION_VARIANT")) {
+        const int v = std::atoi(e);
+        if (v >= 0 && v < NUM_VARIANTS) return v;
     }
     if (!autotune_enabled() || int64_t(s.M) * s.N * s.K < (int64_t(1) << 24)) return DEFAULT_VARIANT;
     // time every variant into a scratch output (the caller's output is not touched)

@@ -173,6 +173,11 @@ void compas_set_contraction_variant(int variant);
 int compas_contraction_variant_count(void);
 int compas_contraction_tuning_report(char* buf, int buflen);
 
+/* Streaming copy of `nbytes` between two device buffers (kernel
+ * "stream_copy"); returns milliseconds.  Measures achievable HBM bandwidth
+ * and serves as the known-size reference for hardware traffic counters. */
+float compas_bandwidth_probe(const CompasContextHandle* ctx, int64_t nbytes);
+
 /* Release cached device memory held by the context's pool. */
 void compas_trim_memory(const CompasContextHandle* ctx);
 
