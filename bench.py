@@ -145,6 +145,92 @@ def cpu_baseline(wl, gpu_echos, gpu_signal, seconds_budget=20.0):
     return base, accuracy
 
 
+def precision_study(args):
+    """Contraction methods side by side at the 4-coil Cartesian shapes (1120 readouts x 224
+    samples x 50176 voxels): signal, J.v and J^H.y per method -- time per call (HIP events
+    around the library's launches) and rel-L2 against the FP64 oracle on independent
+    subsets (readouts for signal and J.v, voxels for J^H.y)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    pyoracle.build()
+    threads = int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    wl = W.config(2)
+    nv = wl.tissue.nvoxels
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+    ctx = ca.init_context(0)
+    params = ca.TissueParameters.from_tissue(wl.tissue)
+    seq = ca.FispSequence.from_sequence(wl.sequence)
+    traj = ca.CartesianTrajectory.from_trajectory(wl.trajectory)
+    coils = ca.CompasArray.from_numpy(wl.coils)
+    echos = ca.simulate_magnetization(params, seq)
+    d = ca.simulate_magnetization_derivatives(echos, params, seq)
+    rng = W.SplitMix64(0xC0A50004)
+    v = (rng.normal(4 * nv) + 1j * rng.normal(4 * nv)).reshape(4, nv).astype(np.complex64)
+    y = ((rng.normal(nc * nr * ns) + 1j * rng.normal(nc * nr * ns)) / np.sqrt(2)).reshape(nc, nr, ns).astype(np.complex64)
+    va, ya = ca.CompasArray.from_numpy(v), ca.CompasArray.from_numpy(y)
+    e = echos.collect()
+    d1, d2 = d["T1"].collect(), d["T2"].collect()
+
+    ridx = np.linspace(0, nr - 1, 6).astype(int)
+    sub = W.CartesianTraj(len(ridx), ns, wl.trajectory.delta_t, wl.trajectory.k_start[ridx], wl.trajectory.delta_k)
+    ref_sig = pyoracle.signal_cartesian(wl.tissue, sub, wl.coils, e[ridx].astype(np.complex128), "f64", threads)
+    ref_jv = pyoracle.jacobian(wl.tissue, sub, wl.coils, e[ridx], d1[ridx], d2[ridx], v, "f64", threads)
+    vsub = np.arange(0, nv, nv // 96)[:96]
+    tsub = wl.tissue.replace(**{f: getattr(wl.tissue, f)[vsub] for f in ("T1", "T2", "B1", "B0", "rho", "x", "y", "z")})
+    ref_jh = pyoracle.jacobian_hermitian(tsub, wl.trajectory, wl.coils[:, vsub], e[:, vsub], d1[:, vsub], d2[:, vsub], y,
+                                         "f64", threads)
+
+    def timed(fn, reps=3):
+        fn()  # first call (autotuning)
+        ctx.synchronize()
+        ctx.read_kernel_timings()
+        ctx.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        ctx.set_kernel_timing(False)
+        k = ctx.read_kernel_timings()
+        gpu = sum(tot for c, tot in k.values()) / reps
+        return out.collect(), wall * 1e3, gpu
+
+    flops = 8.0 * nc * nr * ns * nv
+    res = {}
+    pairs = [("direct", "direct"), ("matmul_fast", "gemm_fast"), ("matmul_pedantic", None), ("matmul_low", "gemm_low")]
+    for sm, jm in pairs:
+        s, sw, sg = timed(lambda: ca.magnetization_to_signal(echos, params, traj, coils, method=sm))
+        entry = {"signal": {"ms": sg, "wall_ms": sw, "useful_TFLOPs": flops / (sg * 1e-3) / 1e12,
+                            "rel_l2_vs_fp64": pyoracle.rel_l2(s[:, ridx, :], ref_sig)}}
+        if jm is not None:
+            jv, jw, jg = timed(lambda: ca.compute_jacobian(echos, d, params, traj, coils, va, method=jm))
+            jh, hw, hg = timed(lambda: ca.compute_jacobian_hermitian(echos, d, params, traj, coils, ya, method=jm))
+            entry["jacobian"] = {"ms": jg, "useful_TFLOPs": 2 * flops / (jg * 1e-3) / 1e12,
+                                 "rel_l2_vs_fp64": pyoracle.rel_l2(jv[:, ridx, :], ref_jv)}
+            entry["jacobian_hermitian"] = {"ms": hg, "useful_TFLOPs": 2 * flops / (hg * 1e-3) / 1e12,
+                                           "rel_l2_vs_fp64": pyoracle.rel_l2(jh[:, vsub], ref_jh)}
+        res[sm] = entry
+    base = res["direct"]
+    fast = res["matmul_fast"]
+    t_plain = base["signal"]["ms"] + base["jacobian"]["ms"] + base["jacobian_hermitian"]["ms"]
+    t_mixed = fast["signal"]["ms"] + fast["jacobian"]["ms"] + fast["jacobian_hermitian"]["ms"]
+    line = {
+        "metric": "signal + J.v + J^H.y time per method at <=1e-4 rel-L2 vs FP64 (mixed-precision MFMA vs plain FP32)",
+        "value": t_plain / t_mixed,
+        "unit": "x speed-up (signal+J.v+J^H.y, BF16 hi+lo MFMA vs FP32 VALU)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32 (bf16/fp16 split operands on MFMA)",
+        "data": "synthetic",
+        "config": {"workload": f"Cartesian contractions {nc} coils x {nr} readouts x {ns} samples x {nv} voxels"},
+        "methods": res,
+        "contraction_tuning": ca.contraction_tuning_report().strip().splitlines(),
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,12 +239,17 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2])
     ap.add_argument("--method", default="matmul_fast", choices=list(ca.SIGNAL_METHODS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="time signal, J.v and J^H.y per contraction method with their error vs FP64")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.precision_study:
+        precision_study(args)
+        return
 
     wl = W.config(args.workload)
     nv = wl.tissue.nvoxels
