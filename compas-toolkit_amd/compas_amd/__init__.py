@@ -82,12 +82,24 @@ _SIGNATURES = {
     "compas_set_contraction_variant": (None, [_i]),
     "compas_contraction_variant_count": (_i, []),
     "compas_contraction_tuning_report": (_i, [C.c_char_p, _i]),
+    "compas_set_contraction_precision": (None, [_i]),
 }
 
 
 def set_contraction_variant(variant: int):
     """Force an MFMA tile variant (>= 0) or restore autotuning (-1)."""
     lib().compas_set_contraction_variant(int(variant))
+
+
+CONTRACTION_PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "bf16x1": 3, "fp16x1": 4}
+
+
+def set_contraction_precision(precision):
+    """Override the contraction precision every signal/Jacobian method selects
+    (a name from CONTRACTION_PRECISIONS), or clear the override (None)."""
+    p = -1 if precision is None else CONTRACTION_PRECISIONS[precision]
+    lib().compas_set_contraction_precision(p)
+    _check()
 
 
 def contraction_variant_count() -> int:

@@ -17,7 +17,13 @@ enum struct ContractionPrecision {
     BF16x3,  // BF16 hi+lo split, 3 MFMA products, FP32 accumulate
     FP16x3,  // FP16 hi+lo split with per-row power-of-two scaling, 3 MFMA products
     BF16x1,  // BF16 single product
+    FP16x1,  // FP16 single product with per-row power-of-two scaling
 };
+
+// Process-wide override of the precision that the methods select (for
+// side-by-side accuracy/speed studies).  Values follow ContractionPrecision's
+// order; -1 clears the override.
+void set_contraction_precision_override(int precision);
 
 // Complex contraction C = A * B^T with both operands K-contiguous:
 //   C[m][n] = sum_k A[m*lda + k] * B[n*ldb + k]      (complex FP32 operands)

@@ -512,6 +512,10 @@ void compas_set_contraction_variant(int variant) {
     catch_exceptions([&] { set_contraction_variant(variant); });
 }
 
+void compas_set_contraction_precision(int precision) {
+    catch_exceptions([&] { set_contraction_precision_override(precision); });
+}
+
 int compas_contraction_variant_count() {
     return contraction_variant_count();
 }

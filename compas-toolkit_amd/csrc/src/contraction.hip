@@ -9,6 +9,7 @@
 //   * split-K partials are summed in FP64 by a reduction kernel that also
 //     writes the caller's output layout (e.g. [coil][readout][sample]).
 
+#include <atomic>
 #include <cmath>
 
 #include "compas/device/common.h"
@@ -166,6 +167,15 @@ void complex_contraction_mfma(
     ContractionPrecision precision,
     bool* handled);
 
+namespace {
+std::atomic<int> g_precision_override {-1};
+}
+
+void set_contraction_precision_override(int precision) {
+    COMPAS_CHECK(precision >= -1 && precision <= int(ContractionPrecision::FP16x1));
+    g_precision_override.store(precision);
+}
+
 void complex_contraction(
     const CompasContext& ctx,
     const cfloat* A,
@@ -174,6 +184,7 @@ void complex_contraction(
     ContractionShape s,
     ContractionPrecision precision) {
     COMPAS_CHECK(s.M >= 0 && s.N >= 0 && s.K >= 0 && s.nsub > 0);
+    if (const int o = g_precision_override.load(); o >= 0) precision = ContractionPrecision(o);
     if (s.M == 0 || s.N == 0) return;
     if (precision != ContractionPrecision::FP32) {
         bool handled = false;

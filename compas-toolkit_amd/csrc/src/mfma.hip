@@ -11,7 +11,8 @@
 // lo = round(x - hi) in BF16 (or FP16 after a per-row power-of-two scale that
 // keeps both parts in the normal range); the product uses
 // hi*hi + hi*lo + lo*hi with FP32 accumulation in the MFMA, which recovers
-// ~16 (BF16) or ~22 (FP16) bits per product.  BF16x1 uses hi*hi only.
+// ~16 (BF16) or ~22 (FP16) bits per product.  BF16x1 and FP16x1 use hi*hi
+// only.
 //
 // Tiling.  A workgroup of WM x WN waves owns 64 WM rows x 64 WN real columns
 // (32 WN complex) and walks a K slice in steps of BKC complex elements; each
@@ -46,15 +47,22 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-enum { P_BF16X3 = 0, P_FP16X3 = 1, P_BF16X1 = 2 };
+enum { P_BF16X3 = 0, P_FP16X3 = 1, P_BF16X1 = 2, P_FP16X1 = 3, NUM_P = 4 };
 
-template<int P>
+constexpr bool is_fp16(int P) {
+    return P == P_FP16X3 || P == P_FP16X1;
+}
+constexpr bool has_lo(int P) {
+    return P == P_BF16X3 || P == P_FP16X3;
+}
+
+template<int P, bool F = is_fp16(P)>
 struct Half16 {
     using T = __bf16;
     using V = bf16x8;
 };
-template<>
-struct Half16<P_FP16X3> {
+template<int P>
+struct Half16<P, true> {
     using T = _Float16;
     using V = f16x8;
 };
@@ -64,15 +72,19 @@ struct Half16<P_FP16X3> {
 template<int P>
 __device__ inline void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
     const f32x2 x = {a, b};
-    if constexpr (P == P_FP16X3) {
+    if constexpr (is_fp16(P)) {
         const f16x2 h = __builtin_convertvector(x, f16x2);
-        const f32x2 r = x - __builtin_convertvector(h, f32x2);
         hi = __builtin_bit_cast(uint32_t, h);
-        lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, f16x2));
+        if constexpr (has_lo(P)) {
+            const f32x2 r = x - __builtin_convertvector(h, f32x2);
+            lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, f16x2));
+        } else {
+            lo = 0u;
+        }
     } else {
         const bf16x2 h = __builtin_convertvector(x, bf16x2);
         hi = __builtin_bit_cast(uint32_t, h);
-        if constexpr (P != P_BF16X1) {
+        if constexpr (has_lo(P)) {
             // bf16 -> f32 is a 16-bit shift of the bit pattern
             const f32x2 hf = {__builtin_bit_cast(float, hi << 16), __builtin_bit_cast(float, hi & 0xFFFF0000u)};
             lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - hf, bf16x2));
@@ -84,7 +96,7 @@ __device__ inline void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) 
 
 template<int P>
 __device__ inline f32x16 mfma(typename Half16<P>::V a, typename Half16<P>::V b, f32x16 c) {
-    if constexpr (P == P_FP16X3) {
+    if constexpr (is_fp16(P)) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     } else {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
@@ -108,7 +120,7 @@ struct MfmaTile {
     static constexpr int BNR = 64 * WN;  // real B rows (2 per complex column)
     static constexpr int BNC = 32 * WN;  // complex B columns
     static constexpr int LD = 2 * BKC + 8;  // LDS row stride, 16-bit elements (+16 bytes of padding)
-    static constexpr bool LO = (P != P_BF16X1);
+    static constexpr bool LO = has_lo(P);
     static constexpr int A_PER = BKC / WN;  // complex per thread per step (contiguous in K)
     static constexpr int A_TPR = WN;  // threads per A row
     static constexpr int B_PER = BKC / (2 * WM);
@@ -360,6 +372,23 @@ struct Plan {
     int tiles_m, tiles_n, splits, k_chunk, threads;
 };
 
+// Workgroups of one variant that fit on a CU at once (LDS and registers).
+template<int P, int WM, int WN, int BKC>
+int resident_per_cu() {
+    static const int cached = [] {
+        int n = 0;
+        COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, contraction_mfma<P, WM, WN, BKC>, MfmaTile<P, WM, WN, BKC>::THREADS, 0));
+        return std::max(1, n);
+    }();
+    return cached;
+}
+
+// Split-K plan.  All workgroups do the same amount of work, so the launch
+// takes ceil(workgroups / resident slots) workgroup lifetimes: a grid just
+// above a multiple of the slots leaves a nearly empty last round.  The split
+// count minimises rounds x K-chunk (the critical path), with a small charge
+// per split for the partial-sum traffic.
 template<int P, int WM, int WN, int BKC>
 Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
     using Tl = MfmaTile<P, WM, WN, BKC>;
@@ -367,13 +396,23 @@ Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
     p.threads = Tl::THREADS;
     p.tiles_m = int(dev::div_ceil(s.M, Tl::BM));
     p.tiles_n = int(dev::div_ceil(s.N, Tl::BNC));
-    const int tiles = p.tiles_m * p.tiles_n;
-    const int per_cu = std::max(1, std::min(4, (160 * 1024) / Tl::LDS_BYTES));
-    const int target = per_cu * cu_count(ctx.device());
-    int splits = std::max(1, (target + tiles - 1) / tiles);
-    splits = std::min(splits, std::max(1, s.K / (8 * BKC)));
-    p.k_chunk = std::max(BKC, int(dev::div_ceil(dev::div_ceil(s.K, splits), BKC)) * BKC);
-    p.splits = std::max(1, int(dev::div_ceil(s.K, p.k_chunk)));
+    const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
+    const int64_t slots = int64_t(resident_per_cu<P, WM, WN, BKC>()) * cu_count(ctx.device());
+    const int max_splits = std::max(1, std::min(64, s.K / (8 * BKC)));
+    double best_cost = 1e300;
+    p.splits = 1;
+    p.k_chunk = std::max(BKC, int(dev::div_ceil(s.K, BKC)) * BKC);
+    for (int want = 1; want <= max_splits; want++) {
+        const int kc = std::max(BKC, int(dev::div_ceil(dev::div_ceil(s.K, want), BKC)) * BKC);
+        const int splits = std::max(1, int(dev::div_ceil(s.K, kc)));
+        const int64_t rounds = dev::div_ceil(tiles * splits, slots);
+        const double cost = double(rounds) * kc * (1.0 + 0.01 * (splits - 1));
+        if (cost < best_cost * (1.0 - 1e-9)) {
+            best_cost = cost;
+            p.splits = splits;
+            p.k_chunk = kc;
+        }
+    }
     return p;
 }
 
@@ -406,13 +445,13 @@ using VariantFn = void (*)(const CompasContext&, const cfloat*, const cfloat*, c
 
 struct Variant {
     const char* name;
-    VariantFn fn[3];  // indexed by P
+    VariantFn fn[NUM_P];  // indexed by P
 };
 
 #define COMPAS_MFMA_VARIANT(WM, WN, BKC)                                                          \
     {#WM "x" #WN "x" #BKC,                                                                        \
      {&launch_variant<P_BF16X3, WM, WN, BKC>, &launch_variant<P_FP16X3, WM, WN, BKC>,             \
-      &launch_variant<P_BF16X1, WM, WN, BKC>}}
+      &launch_variant<P_BF16X1, WM, WN, BKC>, &launch_variant<P_FP16X1, WM, WN, BKC>}}
 
 const Variant VARIANTS[] = {
     COMPAS_MFMA_VARIANT(2, 2, 32),
@@ -498,8 +537,10 @@ std::string contraction_tuning_report() {
     std::string out;
     for (auto& kv : g_tuned) {
         char line[256];
-        std::snprintf(line, sizeof(line), "M=%d N=%d K=%d prec=%d -> %s\n", std::get<1>(kv.first),
-                      std::get<2>(kv.first), std::get<5>(kv.first), std::get<6>(kv.first), VARIANTS[kv.second].name);
+        static const char* const names[NUM_P] = {"bf16x3", "fp16x3", "bf16x1", "fp16x1"};
+        std::snprintf(line, sizeof(line), "M=%d N=%d K=%d prec=%s -> %s\n", std::get<1>(kv.first),
+                      std::get<2>(kv.first), std::get<5>(kv.first), names[std::get<6>(kv.first)],
+                      VARIANTS[kv.second].name);
         out += line;
     }
     return out;
@@ -518,13 +559,14 @@ void complex_contraction_mfma(
         case ContractionPrecision::BF16x3: P = P_BF16X3; break;
         case ContractionPrecision::FP16x3: P = P_FP16X3; break;
         case ContractionPrecision::BF16x1: P = P_BF16X1; break;
+        case ContractionPrecision::FP16x1: P = P_FP16X1; break;
         default: *handled = false; return;
     }
     *handled = true;
     Array<float, 1> scale_a, scale_b;
     const float* sa = nullptr;
     const float* sb = nullptr;
-    if (P == P_FP16X3) {
+    if (P == P_FP16X3 || P == P_FP16X1) {
         scale_a = ctx.empty<float, 1>({s.M});
         scale_b = ctx.empty<float, 1>({s.N});
         KernelScope ks(ctx, "row_pow2_scale");

@@ -173,6 +173,12 @@ void compas_set_contraction_variant(int variant);
 int compas_contraction_variant_count(void);
 int compas_contraction_tuning_report(char* buf, int buflen);
 
+/* Overrides the contraction precision that every signal / Jacobian method
+ * selects, process-wide, for side-by-side accuracy and speed studies:
+ * 0 FP32 VALU, 1 BF16 hi+lo (3 MFMA products), 2 FP16 hi+lo with per-row
+ * power-of-two scaling, 3 BF16 x1, 4 FP16 x1 (scaled); -1 clears it. */
+void compas_set_contraction_precision(int precision);
+
 /* Streaming copy of `nbytes` between two device buffers (kernel
  * "stream_copy"); returns milliseconds.  Measures achievable HBM bandwidth
  * and serves as the known-size reference for hardware traffic counters. */

@@ -300,6 +300,28 @@ def test_jacobian_hermitian(ctx, oracle, method):
     assert rel(got, want) < SIG_TOL
 
 
+PRECISION_TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "fp16x3": 1e-5, "bf16x1": 2e-2, "fp16x1": 5e-3}
+
+
+@pytest.mark.parametrize("precision", list(PRECISION_TOL))
+def test_precision_override_every_output(ctx, oracle, precision):
+    # signal, J.v and J^H.y with the contraction precision forced, against FP64
+    banner(f"precision override {precision}")
+    wl, echos, d1, d2, v, y = jac_inputs(oracle)
+    params, seq, traj = objects(wl)
+    tol = PRECISION_TOL[precision]
+    try:
+        ca.set_contraction_precision(precision)
+        s = ca.magnetization_to_signal(echos, params, traj, wl.coils).collect()
+        jv = ca.compute_jacobian(echos, (d1, d2), params, traj, wl.coils, v).collect()
+        jh = ca.compute_jacobian_hermitian(echos, (d1, d2), params, traj, wl.coils, y).collect()
+    finally:
+        ca.set_contraction_precision(None)
+    assert rel(s, oracle.signal_cartesian(wl.tissue, wl.trajectory, wl.coils, echos, "f64")) < tol
+    assert rel(jv, oracle.jacobian(wl.tissue, wl.trajectory, wl.coils, echos, d1, d2, v, "f64")) < tol
+    assert rel(jh, oracle.jacobian_hermitian(wl.tissue, wl.trajectory, wl.coils, echos, d1, d2, y, "f64")) < tol
+
+
 def test_jacobian_adjoint_on_gpu(ctx, oracle):
     wl, echos, d1, d2, v, y = jac_inputs(oracle)
     params, seq, traj = objects(wl)
