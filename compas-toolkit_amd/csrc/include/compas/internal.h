@@ -58,6 +58,22 @@ std::string contraction_tuning_report();
 // (after one warm-up) and returns the elapsed milliseconds.
 float bandwidth_probe(const CompasContext& ctx, int64_t nbytes);
 
+// Precision after the process-wide override (if one is set).
+ContractionPrecision effective_precision(ContractionPrecision requested);
+
+// Per-row power-of-two scales s with max_k |x[row][k]| s ~ 2^14 for FP16
+// operands of `rows` rows of length K with row stride ld.
+void row_pow2_scales(const CompasContext& ctx, const cfloat* X, int rows, int K, int64_t ld, float* scale);
+
+// J^H.y with the contraction and readout reduction fused on MFMA.  y
+// [nc][nr][ns]; EH [2 nv][ns] = conj(E) rows then conj(dE/dT2) rows; m, d1, d2
+// [nr][nv]; coils [nc][nv]; writes JHy [4][nv].  Returns false (doing
+// nothing) when `precision` is not an MFMA precision or nc < 3.
+bool jacobian_hermitian_mfma(
+    const CompasContext& ctx, const TissueParameters& p, const cfloat* y, const cfloat* EH, const cfloat* m,
+    const cfloat* d1, const cfloat* d2, const cfloat* coils, int nc, int nr, int ns,
+    ContractionPrecision precision, cfloat* JHy);
+
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);
 

@@ -176,6 +176,11 @@ void set_contraction_precision_override(int precision) {
     g_precision_override.store(precision);
 }
 
+ContractionPrecision effective_precision(ContractionPrecision requested) {
+    const int o = g_precision_override.load();
+    return o >= 0 ? ContractionPrecision(o) : requested;
+}
+
 void complex_contraction(
     const CompasContext& ctx,
     const cfloat* A,
@@ -184,7 +189,7 @@ void complex_contraction(
     ContractionShape s,
     ContractionPrecision precision) {
     COMPAS_CHECK(s.M >= 0 && s.N >= 0 && s.K >= 0 && s.nsub > 0);
-    if (const int o = g_precision_override.load(); o >= 0) precision = ContractionPrecision(o);
+    precision = effective_precision(precision);
     if (s.M == 0 || s.N == 0) return;
     if (precision != ContractionPrecision::FP32) {
         bool handled = false;

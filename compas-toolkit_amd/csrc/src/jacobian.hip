@@ -10,8 +10,10 @@
 // with the adjoint sources of product_kernels.cuh:112-139 as the A operand
 // ([nr][2nv]) and the coil-weighted decays as the B operand ([nc*ns][2nv]).
 //
-// J^H.y contracts y against conj(E) / conj(dE) per coil (M = nr, N = 2 nv,
-// K = ns) and reduces over readouts per voxel with m, dm/dT1, dm/dT2
+// J^H.y: on MFMA precisions the contraction of y against conj(E) / conj(dE)
+// and the readout reduction with m, dm/dT1, dm/dT2 are one fused kernel
+// (hermitian_mfma.hip).  The FP32 path contracts per coil (M = nr, N = 2 nv,
+// K = ns) and reduces over readouts per voxel in a second pass
 // (hermitian_kernels.cuh:314-360).
 
 #include "compas/device/common.h"
@@ -219,6 +221,9 @@ Array<cfloat, 2> compute_jacobian_hermitian(
             tv, ns, traj.delta_t, traj.delta_k.re, traj.delta_k.im, EH.data());
         check_launch("hermitian_decay");
     }
+    if (jacobian_hermitian_mfma(ctx, p, vector.data(), EH.data(), echos.data(), dT1.data(), dT2.data(),
+                                coils.data(), nc, nr, ns, effective_precision(prec), JHy.data()))
+        return JHy;
     auto Ev = ctx.empty<cfloat, 2>({nr, 2 * int64_t(nv)});
     for (int c = 0; c < nc; c++) {
         ContractionShape s {nr, 2 * nv, ns, ns, ns, 2 * nv, 0, false};

@@ -1,31 +1,13 @@
 // MFMA complex contraction on gfx950: C[m][n] = sum_k A[m][k] B[n][k].
 //
-// Complex-to-real mapping.  With A stored interleaved (re, im) along K, row
-// m of A is a real vector of length 2K.  For each complex column n the real
-// GEMM uses two B rows built while staging into LDS:
-//   row 2n   = (b_re, -b_im) pairs  ->  D[m][2n]   = Re(sum a b)
-//   row 2n+1 = (b_im,  b_re) pairs  ->  D[m][2n+1] = Im(sum a b)
-// so one real GEMM (M x 2N x 2K) produces interleaved complex output.
-//
-// Precision.  Every FP32 operand x is split into hi = round(x) and
-// lo = round(x - hi) in BF16 (or FP16 after a per-row power-of-two scale that
-// keeps both parts in the normal range); the product uses
-// hi*hi + hi*lo + lo*hi with FP32 accumulation in the MFMA, which recovers
-// ~16 (BF16) or ~22 (FP16) bits per product.  BF16x1 and FP16x1 use hi*hi
-// only.
-//
-// Tiling.  A workgroup of WM x WN waves owns 64 WM rows x 64 WN real columns
-// (32 WN complex) and walks a K slice in steps of BKC complex elements; each
-// wave computes 64 x 64 with 2 x 2 v_mfma_f32_32x32x16 blocks.  The next K
-// step is prefetched into registers while the MFMAs of the current one run.
-// Operands are converted to packed 16-bit pairs on the way into LDS (one
-// v_cvt_pk per pair, 128-bit stores).  Workgroups are numbered so that those
-// sharing A rows run on the same XCD (consecutive hardware ids are spread
-// over the 8 XCDs), so the XCD's L2 serves the re-reads of A across column
-// tiles.  K slices are reduced in FP64.
-//
-// Variants (WM, WN, BKC) are timed on the live shape by a small autotuner on
-// first use and the fastest is cached per shape.
+// The tile machinery (complex-to-real mapping, hi/lo operand split, LDS
+// staging, MFMA main loop) lives in compas/device/mfma_tile.h.  This file
+// holds the plain contraction kernel, its split-K plan and reduction, and a
+// small autotuner: the tile variants (WM, WN, BKC) are timed on the live shape
+// on first use and the fastest is cached per shape.  Workgroups are numbered
+// so that those sharing A rows run on the same XCD (consecutive hardware ids
+// are spread over the 8 XCDs), so the XCD's L2 serves the re-reads of A
+// across column tiles.  K slices are reduced in FP64.
 
 #include <chrono>
 #include <cmath>
@@ -35,112 +17,13 @@
 #include <tuple>
 
 #include "compas/device/common.h"
+#include "compas/device/mfma_tile.h"
 #include "compas/internal.h"
 
 namespace compas {
 namespace kernels {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-
-enum { P_BF16X3 = 0, P_FP16X3 = 1, P_BF16X1 = 2, P_FP16X1 = 3, NUM_P = 4 };
-
-constexpr bool is_fp16(int P) {
-    return P == P_FP16X3 || P == P_FP16X1;
-}
-constexpr bool has_lo(int P) {
-    return P == P_BF16X3 || P == P_FP16X3;
-}
-
-template<int P, bool F = is_fp16(P)>
-struct Half16 {
-    using T = __bf16;
-    using V = bf16x8;
-};
-template<int P>
-struct Half16<P, true> {
-    using T = _Float16;
-    using V = f16x8;
-};
-
-// (a, b) -> packed 16-bit pair hi = round(a, b) and lo = round((a, b) - hi),
-// first element in the low half (one v_cvt_pk_* per pair)
-template<int P>
-__device__ inline void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
-    const f32x2 x = {a, b};
-    if constexpr (is_fp16(P)) {
-        const f16x2 h = __builtin_convertvector(x, f16x2);
-        hi = __builtin_bit_cast(uint32_t, h);
-        if constexpr (has_lo(P)) {
-            const f32x2 r = x - __builtin_convertvector(h, f32x2);
-            lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, f16x2));
-        } else {
-            lo = 0u;
-        }
-    } else {
-        const bf16x2 h = __builtin_convertvector(x, bf16x2);
-        hi = __builtin_bit_cast(uint32_t, h);
-        if constexpr (has_lo(P)) {
-            // bf16 -> f32 is a 16-bit shift of the bit pattern
-            const f32x2 hf = {__builtin_bit_cast(float, hi << 16), __builtin_bit_cast(float, hi & 0xFFFF0000u)};
-            lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - hf, bf16x2));
-        } else {
-            lo = 0u;
-        }
-    }
-}
-
-template<int P>
-__device__ inline f32x16 mfma(typename Half16<P>::V a, typename Half16<P>::V b, f32x16 c) {
-    if constexpr (is_fp16(P)) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-    } else {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-    }
-}
-
-// Logical tile index from the hardware workgroup id: hardware ids b, b+8, ...
-// land on the same XCD, so give each XCD a contiguous range of logical ids.
-__device__ inline int xcd_logical_id(int b, int total) {
-    const int per = (total + 7) / 8;
-    const int xcd = b % 8, slot = b / 8;
-    const int full = total % 8 == 0 ? 8 : total % 8;  // XCDs that receive `per` ids
-    const int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
-    return base + slot;
-}
-
-template<int P, int WM, int WN, int BKC>
-struct MfmaTile {
-    static constexpr int THREADS = 64 * WM * WN;
-    static constexpr int BM = 64 * WM;  // A rows
-    static constexpr int BNR = 64 * WN;  // real B rows (2 per complex column)
-    static constexpr int BNC = 32 * WN;  // complex B columns
-    static constexpr int LD = 2 * BKC + 8;  // LDS row stride, 16-bit elements (+16 bytes of padding)
-    static constexpr bool LO = has_lo(P);
-    static constexpr int A_PER = BKC / WN;  // complex per thread per step (contiguous in K)
-    static constexpr int A_TPR = WN;  // threads per A row
-    static constexpr int B_PER = BKC / (2 * WM);
-    static constexpr int B_TPR = 2 * WM;
-    static constexpr int LDS_BYTES = (BM + BNR) * LD * 2 * (LO ? 2 : 1);
-    static_assert(A_PER >= 4 && A_PER % 4 == 0, "A staging granularity");
-    static_assert(B_PER >= 2 && B_PER % 2 == 0, "B staging granularity");
-};
-
-template<int N>
-__device__ inline void store_words(uint32_t* dst, const uint32_t* w) {
-    if constexpr (N % 4 == 0) {
-#pragma unroll
-        for (int i = 0; i < N / 4; i++)
-            reinterpret_cast<uint4*>(dst)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < N / 2; i++) reinterpret_cast<uint2*>(dst)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
-    }
-}
+using namespace mfma;
 
 template<int P, int WM, int WN, int BKC>
 __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
@@ -153,15 +36,7 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     float2* __restrict__ partial,
     cfloat* __restrict__ direct, int nsub, int64_t out_outer) {
     using Tl = MfmaTile<P, WM, WN, BKC>;
-    using T = typename Half16<P>::T;
-    using V = typename Half16<P>::V;
-    constexpr bool LO = Tl::LO;
-    constexpr int LD = Tl::LD;
-
-    __shared__ __attribute__((aligned(16))) T sA_hi[Tl::BM * LD];
-    __shared__ __attribute__((aligned(16))) T sB_hi[Tl::BNR * LD];
-    __shared__ __attribute__((aligned(16))) T sA_lo[LO ? Tl::BM * LD : 8];
-    __shared__ __attribute__((aligned(16))) T sB_lo[LO ? Tl::BNR * LD : 8];
+    __shared__ MfmaSmem<P, WM, WN, BKC> sm;
 
     const int total = tiles_m * tiles_n * splits;
     const int lid = xcd_logical_id(blockIdx.x, total);
@@ -178,114 +53,17 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const int wm = wave / WN, wn = wave % WN;
 
     f32x16 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+    zero(acc);
 
-    const int a_row = tid / Tl::A_TPR, a_k = (tid % Tl::A_TPR) * Tl::A_PER;
-    const int b_row = tid / Tl::B_TPR, b_k = (tid % Tl::B_TPR) * Tl::B_PER;
-    const int am = m0 + a_row, bn = n0 + b_row;
+    const int am = m0 + staged_a_row<P, WM, WN, BKC>(tid);
+    const int bn = n0 + staged_b_row<P, WM, WN, BKC>(tid);
     const float sa = (row_scale_a && am < M) ? row_scale_a[am] : 1.0f;
     const float sb = (row_scale_b && bn < N) ? row_scale_b[bn] : 1.0f;
-    const cfloat* a_src = A + int64_t(am) * lda;
-    const cfloat* b_src = B + int64_t(bn) * ldb;
     // 16-byte vector loads need 16-byte aligned rows
     const bool vec_a = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 2 == 0);
     const bool vec_b = ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % 2 == 0);
-
-    // register prefetch of one K step
-    cfloat ra[Tl::A_PER], rb[Tl::B_PER];
-    auto gload = [&](int k0) {
-        const int ka = k0 + a_k;
-        if (vec_a && am < M && ka + Tl::A_PER <= ke) {
-#pragma unroll
-            for (int i = 0; i < Tl::A_PER / 2; i++) {
-                const float4 v = *reinterpret_cast<const float4*>(a_src + ka + 2 * i);
-                ra[2 * i] = {v.x, v.y};
-                ra[2 * i + 1] = {v.z, v.w};
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < Tl::A_PER; i++) ra[i] = (am < M && ka + i < ke) ? a_src[ka + i] : cfloat {0, 0};
-        }
-        const int kb_ = k0 + b_k;
-        if (vec_b && bn < N && kb_ + Tl::B_PER <= ke) {
-#pragma unroll
-            for (int i = 0; i < Tl::B_PER / 2; i++) {
-                const float4 v = *reinterpret_cast<const float4*>(b_src + kb_ + 2 * i);
-                rb[2 * i] = {v.x, v.y};
-                rb[2 * i + 1] = {v.z, v.w};
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < Tl::B_PER; i++) rb[i] = (bn < N && kb_ + i < ke) ? b_src[kb_ + i] : cfloat {0, 0};
-        }
-    };
-
-    if (kb < ke) gload(kb);
-    for (int k0 = kb; k0 < ke; k0 += BKC) {
-        // ---- stage A: (re, im) pairs packed hi / lo
-        {
-            uint32_t h[Tl::A_PER], l[Tl::A_PER];
-#pragma unroll
-            for (int i = 0; i < Tl::A_PER; i++) split_pair<P>(ra[i].re * sa, ra[i].im * sa, h[i], l[i]);
-            store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sA_hi + a_row * LD + 2 * a_k), h);
-            if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sA_lo + a_row * LD + 2 * a_k), l);
-        }
-        // ---- stage B: conj row (re, -im) and swap row (im, re)
-        {
-            uint32_t ch[Tl::B_PER], sh[Tl::B_PER], cl[Tl::B_PER], sl[Tl::B_PER];
-#pragma unroll
-            for (int i = 0; i < Tl::B_PER; i++) {
-                uint32_t h, l;
-                split_pair<P>(rb[i].re * sb, rb[i].im * sb, h, l);
-                ch[i] = h ^ 0x80000000u;  // negate the imaginary half
-                sh[i] = __builtin_amdgcn_alignbit(h, h, 16);  // swap halves
-                cl[i] = l ^ 0x80000000u;
-                sl[i] = __builtin_amdgcn_alignbit(l, l, 16);
-            }
-            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_hi + (2 * b_row) * LD + 2 * b_k), ch);
-            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_hi + (2 * b_row + 1) * LD + 2 * b_k), sh);
-            if constexpr (LO) {
-                store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_lo + (2 * b_row) * LD + 2 * b_k), cl);
-                store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sB_lo + (2 * b_row + 1) * LD + 2 * b_k), sl);
-            }
-        }
-        __syncthreads();
-        if (k0 + BKC < ke) gload(k0 + BKC);  // in flight during the MFMAs below
-
-#pragma unroll
-        for (int kk = 0; kk < BKC / 8; kk++) {
-            const int koff = kk * 16 + (lane >> 5) * 8;
-            V ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const int r = wm * 64 + i * 32 + (lane & 31);
-                ah[i] = *reinterpret_cast<const V*>(sA_hi + r * LD + koff);
-                if constexpr (LO) al[i] = *reinterpret_cast<const V*>(sA_lo + r * LD + koff);
-            }
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int c = wn * 64 + j * 32 + (lane & 31);
-                bh[j] = *reinterpret_cast<const V*>(sB_hi + c * LD + koff);
-                if constexpr (LO) bl[j] = *reinterpret_cast<const V*>(sB_lo + c * LD + koff);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
-                    if constexpr (LO) {
-                        acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
-                        acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
-                    }
-                }
-        }
-        __syncthreads();
-    }
+    mfma_accumulate<P, WM, WN, BKC>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
+                                    bn < N ? B + int64_t(bn) * ldb : nullptr, sb, vec_b, kb, ke);
 
     // ---- epilogue: D[row][realcol], realcol = 2n + {re, im}
 #pragma unroll
@@ -546,6 +324,13 @@ std::string contraction_tuning_report() {
     return out;
 }
 
+void row_pow2_scales(const CompasContext& ctx, const cfloat* X, int rows, int K, int64_t ld, float* scale) {
+    if (rows <= 0) return;
+    KernelScope ks(ctx, "row_pow2_scale");
+    row_pow2_scale<<<rows, 256, 0, ctx.stream()>>>(X, rows, K, ld, scale);
+    check_launch("row_pow2_scale");
+}
+
 void complex_contraction_mfma(
     const CompasContext& ctx,
     const cfloat* A,
@@ -569,10 +354,8 @@ void complex_contraction_mfma(
     if (P == P_FP16X3 || P == P_FP16X1) {
         scale_a = ctx.empty<float, 1>({s.M});
         scale_b = ctx.empty<float, 1>({s.N});
-        KernelScope ks(ctx, "row_pow2_scale");
-        row_pow2_scale<<<s.M, 256, 0, ctx.stream()>>>(A, s.M, s.K, s.lda, scale_a.data());
-        row_pow2_scale<<<s.N, 256, 0, ctx.stream()>>>(B, s.N, s.K, s.ldb, scale_b.data());
-        check_launch("row_pow2_scale");
+        row_pow2_scales(ctx, A, s.M, s.K, s.lda, scale_a.data());
+        row_pow2_scales(ctx, B, s.N, s.K, s.ldb, scale_b.data());
         sa = scale_a.data();
         sb = scale_b.data();
     }

@@ -300,6 +300,36 @@ def test_jacobian_hermitian(ctx, oracle, method):
     assert rel(got, want) < SIG_TOL
 
 
+@pytest.mark.parametrize("nc,nr,ns,nv", [(1, 77, 50, 333), (2, 64, 51, 257), (3, 77, 50, 333), (4, 33, 64, 70),
+                                         (5, 130, 49, 1000), (8, 41, 32, 129)])
+def test_jacobian_hermitian_coil_groups(ctx, oracle, nc, nr, ns, nv):
+    # the fused MFMA path packs four coils per row group (padding the last group),
+    # one or two coils take the per-coil path; ragged readouts, samples and voxels
+    banner(f"J^H.y coils={nc} nr={nr} ns={ns} nv={nv}")
+    rng = W.SplitMix64(1000 * nc + nr)
+    n = int(np.ceil(np.sqrt(nv)))
+    t = W.make_tissue(rng, n, 20.0, 40.0).slice(0, nv)
+    traj = W.CartesianTraj(nr, ns, 5e-6, np.zeros(nr, np.complex64), float(np.float32(2 * np.pi / 20.0)))
+
+    def cn(*shape, s=1.0):
+        k = int(np.prod(shape))
+        return ((rng.normal(k) + 1j * rng.normal(k)) * s).reshape(shape).astype(np.complex64)
+
+    coils, echos, d1, d2, y = cn(nc, nv, s=0.5), cn(nr, nv, s=0.3), cn(nr, nv, s=0.01), cn(nr, nv, s=0.1), cn(nc, nr, ns)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    ctraj = ca.CartesianTrajectory.from_trajectory(traj)
+    want = oracle.jacobian_hermitian(t, traj, coils, echos, d1, d2, y, "f64")
+    got = ca.compute_jacobian_hermitian(echos, (d1, d2), params, ctraj, coils, y, method="gemm_fast").collect()
+    assert rel(got, want) < SIG_TOL
+    try:
+        ca.set_contraction_precision("fp16x3")
+        got = ca.compute_jacobian_hermitian(echos, (d1, d2), params, ctraj, coils, y, method="gemm_fast").collect()
+    finally:
+        ca.set_contraction_precision(None)
+    assert rel(got, want) < 1e-5
+
+
 PRECISION_TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "fp16x3": 1e-5, "bf16x1": 2e-2, "fp16x1": 5e-3}
 
 
