@@ -145,6 +145,97 @@ def cpu_baseline(wl, gpu_echos, gpu_signal, seconds_budget=20.0):
     return base, accuracy
 
 
+PRODUCT_METHODS = {"matmul_fast": "gemm_fast", "matmul_low": "gemm_low", "direct": "direct"}
+MFMA_PASSES = {"matmul_fast": 3, "matmul_low": 1, "direct": 0}
+
+
+def products_bench(args):
+    """signal + J.v + J^H.y at the 4-coil Cartesian shapes (1120 readouts x 224
+    samples x 50176 voxels): one step is the three products on resident inputs
+    (FISP echos and their finite-difference T1/T2 derivatives, a seeded [4][nv]
+    direction and a seeded [4][1120][224] residual).  value = k-space samples/s
+    summed over the three outputs; the roofline covers the contraction kernels
+    (useful complex-MAC flops; the hi+lo split issues 3x that on MFMA)."""
+    sig_method = args.method
+    if sig_method not in PRODUCT_METHODS:
+        sys.exit(f"--products takes --method in {sorted(PRODUCT_METHODS)}")
+    jac_method = PRODUCT_METHODS[sig_method]
+    wl = W.config(2)
+    nv = wl.tissue.nvoxels
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+    ctx = ca.init_context(0)
+    params = ca.TissueParameters.from_tissue(wl.tissue)
+    seq = ca.FispSequence.from_sequence(wl.sequence)
+    traj = ca.CartesianTrajectory.from_trajectory(wl.trajectory)
+    coils = ca.CompasArray.from_numpy(wl.coils)
+    echos = ca.simulate_magnetization(params, seq)
+    d = ca.simulate_magnetization_derivatives(echos, params, seq)
+    rng = W.SplitMix64(0xC0A50004)
+    v = ca.CompasArray.from_numpy((rng.normal(4 * nv) + 1j * rng.normal(4 * nv)).reshape(4, nv).astype(np.complex64))
+    y = ca.CompasArray.from_numpy(((rng.normal(nc * nr * ns) + 1j * rng.normal(nc * nr * ns)) / np.sqrt(2))
+                                  .reshape(nc, nr, ns).astype(np.complex64))
+    ctx.synchronize()
+
+    def step():
+        s = ca.magnetization_to_signal(echos, params, traj, coils, method=sig_method)
+        jv = ca.compute_jacobian(echos, d, params, traj, coils, v, method=jac_method)
+        jh = ca.compute_jacobian_hermitian(echos, d, params, traj, coils, y, method=jac_method)
+        return s, jv, jh
+
+    for _ in range(args.warmup):
+        out = step()
+    ctx.synchronize()
+    del out
+    ctx.read_kernel_timings()
+    ctx.set_kernel_timing(True)
+    lat = []
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = step()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+    t_total = time.perf_counter() - t_start
+    ctx.set_kernel_timing(False)
+    timings = ctx.read_kernel_timings()
+    kernels = {k: {"launches": c, "avg_ms": tot / max(c, 1), "total_ms": tot} for k, (c, tot) in timings.items()}
+    contraction = [k for k in kernels if k in ("contraction_mfma", "hermitian_mfma", "contraction_fp32")]
+    t_con = sum(kernels[k]["total_ms"] for k in contraction) / args.steps * 1e-3
+    useful = 8.0 * nc * nr * ns * nv * (1 + 2 + 2)  # signal, J.v (two terms), J^H.y (two terms)
+    passes = MFMA_PASSES[sig_method]
+    if passes:
+        roof = {"kernel": "+".join(sorted(contraction)), "bound": "mfma", "achieved": useful / t_con / 1e12,
+                "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": useful / t_con / 1e12 / PEAK_BF16_TFLOPS,
+                "issued_frac": passes * useful / t_con / 1e12 / PEAK_BF16_TFLOPS, "traffic": None}
+    else:
+        roof = {"kernel": "+".join(sorted(contraction)), "bound": "valu", "achieved": useful / t_con / 1e12,
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": useful / t_con / 1e12 / PEAK_FP32_TFLOPS,
+                "traffic": None}
+    line = {
+        "metric": "signal + J.v + J^H.y k-space samples/s (4 coils x 1120 x 224, 50176 voxels)",
+        "value": 3 * nc * nr * ns * args.steps / t_total,
+        "unit": "k-space samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if not passes else f"f32 operands as bf16 x{passes} on MFMA, f32 accumulate",
+        "data": "synthetic (seeded tissue, FISP echos, finite-difference derivatives, random direction/residual)",
+        "config": {"workload": f"Cartesian signal + J.v + J^H.y, {nc} coils x {nr} readouts x {ns} samples x "
+                               f"{nv} voxels", "signal_method": sig_method, "jacobian_method": jac_method},
+        "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99)},
+        "kernels": kernels,
+        "contraction_tuning": ca.contraction_tuning_report().strip().splitlines(),
+        "roofline": roof,
+    }
+    print(json.dumps(line))
+
+
 def precision_study(args):
     """Contraction precisions side by side: FP32 VALU, BF16 hi+lo, FP16 hi+lo
     (per-row power-of-two scaling), BF16 x1 and FP16 x1.  For the 4-coil
@@ -264,6 +355,8 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2])
     ap.add_argument("--method", default="matmul_fast", choices=list(ca.SIGNAL_METHODS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--products", action="store_true",
+                    help="time signal + J.v + J^H.y at the 4-coil Cartesian shapes (--method picks the precision)")
     ap.add_argument("--precision-study", action="store_true",
                     help="time signal, J.v and J^H.y per contraction method with their error vs FP64")
     args = ap.parse_args()
@@ -274,6 +367,9 @@ def main():
         sys.exit(2)
     if args.precision_study:
         precision_study(args)
+        return
+    if args.products:
+        products_bench(args)
         return
 
     wl = W.config(args.workload)
