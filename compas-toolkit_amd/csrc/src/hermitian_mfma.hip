@@ -39,8 +39,15 @@ __device__ inline float2 conj_factor(cfloat z, int part) {
 template<int P, int WM, int WN, int BKC, int NCP>
 union HermitianSmem {
     MfmaSmem<P, WM, WN, BKC> tile;
-    float red[WM][WN][2][32][NCP][6];  // per-lane readout sums: (j, lane, coil) -> G conj(m), conj(d1), conj(d2)
+    // per-lane readout sums (lane = 2 voxel + part, coil): G conj(m), G conj(dm/dT1),
+    // G conj(dm/dT2), G' conj(m)
+    float red[WM][WN][32][NCP][8];
 };
+
+// Readout tiles per group of the launch order: a group sweeps every voxel tile
+// for GROUP_R readout tiles, so the group's y rows stay in L2 while the
+// voxel tiles stream past (and each voxel tile is re-read once per group).
+constexpr int GROUP_R = 8;
 
 template<int P, int WM, int WN, int BKC, int NCP>
 __global__ void __launch_bounds__(64 * WM * WN, 4) hermitian_mfma(
@@ -60,9 +67,12 @@ __global__ void __launch_bounds__(64 * WM * WN, 4) hermitian_mfma(
     const int total = tiles_r * tiles_n;
     const int lid = xcd_logical_id(blockIdx.x, total);
     if (lid >= total) return;
-    // consecutive ids share the voxel tile, so its decay rows stay in the XCD's L2
-    const int tr = lid % tiles_r, tn = lid / tiles_r;
-    const int m0 = tr * Tl::BM, n0 = tn * Tl::BNC;
+    const int group = lid / (GROUP_R * tiles_n), rem = lid % (GROUP_R * tiles_n);
+    const int tr0 = group * GROUP_R, gr = min(GROUP_R, tiles_r - tr0);
+    const int tr = tr0 + rem % gr, tn = rem / gr;
+    // rows: (readout, coil) with the coil innermost; complex columns of wave wn:
+    // 16 voxels with conj(E) (block j = 0) and the same 16 with conj(dE) (j = 1)
+    const int m0 = tr * Tl::BM, vox0 = tn * (Tl::BNC / 2);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -75,8 +85,8 @@ __global__ void __launch_bounds__(64 * WM * WN, 4) hermitian_mfma(
         const int ar = am / NCP, ac = am % NCP;
         const bool a_ok = ar < nr && ac < ncg;
         const int64_t yrow = int64_t(c0 + ac) * nr + ar;
-        const int bn = n0 + staged_b_row<P, WM, WN, BKC>(tid);
-        const int bv = bn >> 1, bvar = bn & 1;
+        const int nl = staged_b_row<P, WM, WN, BKC>(tid);  // complex column within the tile
+        const int bv = vox0 + (nl / 32) * 16 + nl % 16, bvar = (nl / 16) % 2;
         const bool b_ok = bv < nv;
         const int64_t brow = int64_t(bvar) * nv + bv;
         const float sa = (scale_y && a_ok) ? scale_y[yrow] : 1.0f;
@@ -88,100 +98,91 @@ __global__ void __launch_bounds__(64 * WM * WN, 4) hermitian_mfma(
     }
     // mfma_accumulate ends with a barrier: the tile buffers are free for the sums
 
-    // ---- epilogue: reduce G, G' against m, dm over the lane's readouts.  Lane
-    // roles within a voxel: (l % 4) = 2 variant + part; variant 0 holds G,
-    // variant 1 holds G' (its d1/d2 sums are not used).
+    // ---- epilogue: the lane holds G (block j = 0) and G' (j = 1) of one voxel
+    // for its readouts; reduce them against m, dm (one load serves both blocks
+    // and all NCP coils of the readout)
+    const int part = lane & 1;
+    const int v = vox0 + wn * 16 + (lane & 31) / 2;
+    const bool v_ok = v < nv;
+    // readout of (i, q, tt) = r_base + (32 i + 8 q + NCP tt) / NCP
+    const int r_base = (m0 + wm * 64 + (lane >> 5) * 4) / NCP;
+    const int64_t base = int64_t(r_base) * nv + v;
+    float s[NCP][8];
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int rc = wn * 64 + j * 32 + (lane & 31);
-        const int n = n0 + (rc >> 1);
-        const int v = n >> 1, var = n & 1, part = rc & 1;
-        const bool v_ok = v < nv;
-        // readout of (i, q, tt) = r_base + (32 i + 8 q + NCP tt) / NCP
-        const int r_base = (m0 + wm * 64 + (lane >> 5) * 4) / NCP;
-        const int64_t base = int64_t(r_base) * nv + v;
-        float s[NCP][6];
+    for (int c = 0; c < NCP; c++)
 #pragma unroll
-        for (int c = 0; c < NCP; c++)
+        for (int k = 0; k < 8; k++) s[c][k] = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 6; k++) s[c][k] = 0.0f;
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int q = 0; q < 4; q++)
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-                for (int tt = 0; tt < 4 / NCP; tt++) {
-                    const int dr = (i * 32 + q * 8 + tt * NCP) / NCP;
-                    const int r = r_base + dr;
-                    cfloat mv {0, 0}, a1 {0, 0}, a2 {0, 0};
-                    if (v_ok && r < nr) {
-                        const int64_t idx = base + int64_t(dr) * nv;
-                        mv = m[idx];
-                        a1 = d1[idx];
-                        a2 = d2[idx];
-                    }
-                    const float2 fm = conj_factor(mv, part), f1 = conj_factor(a1, part), f2 = conj_factor(a2, part);
-#pragma unroll
-                    for (int c = 0; c < NCP; c++) {
-                        float x = acc[i][j][q * 4 + tt * NCP + c];
-                        if constexpr (is_fp16(P)) {
-                            if (v_ok && r < nr && c < ncg)
-                                x /= scale_y[int64_t(c0 + c) * nr + r] * scale_b[int64_t(var) * nv + v];
-                        }
-                        s[c][0] = fmaf(x, fm.x, s[c][0]);
-                        s[c][1] = fmaf(x, fm.y, s[c][1]);
-                        s[c][2] = fmaf(x, f1.x, s[c][2]);
-                        s[c][3] = fmaf(x, f1.y, s[c][3]);
-                        s[c][4] = fmaf(x, f2.x, s[c][4]);
-                        s[c][5] = fmaf(x, f2.y, s[c][5]);
-                    }
-                    // keep the scheduler from hoisting every readout's loads at once
-                    __builtin_amdgcn_sched_barrier(0);
+            for (int tt = 0; tt < 4 / NCP; tt++) {
+                const int dr = (i * 32 + q * 8 + tt * NCP) / NCP;
+                const int r = r_base + dr;
+                cfloat mv {0, 0}, a1 {0, 0}, a2 {0, 0};
+                if (v_ok && r < nr) {
+                    const int64_t idx = base + int64_t(dr) * nv;
+                    mv = m[idx];
+                    a1 = d1[idx];
+                    a2 = d2[idx];
                 }
-        // readouts held by the other half-wave (same voxel, same coil)
+                const float2 fm = conj_factor(mv, part), f1 = conj_factor(a1, part), f2 = conj_factor(a2, part);
+#pragma unroll
+                for (int c = 0; c < NCP; c++) {
+                    const int e = q * 4 + tt * NCP + c;
+                    float g = acc[i][0][e], gp = acc[i][1][e];
+                    if constexpr (is_fp16(P)) {
+                        if (v_ok && r < nr && c < ncg) {
+                            const float sy = scale_y[int64_t(c0 + c) * nr + r];
+                            g /= sy * scale_b[v];
+                            gp /= sy * scale_b[int64_t(nv) + v];
+                        }
+                    }
+                    s[c][0] = fmaf(g, fm.x, s[c][0]);
+                    s[c][1] = fmaf(g, fm.y, s[c][1]);
+                    s[c][2] = fmaf(g, f1.x, s[c][2]);
+                    s[c][3] = fmaf(g, f1.y, s[c][3]);
+                    s[c][4] = fmaf(g, f2.x, s[c][4]);
+                    s[c][5] = fmaf(g, f2.y, s[c][5]);
+                    s[c][6] = fmaf(gp, fm.x, s[c][6]);
+                    s[c][7] = fmaf(gp, fm.y, s[c][7]);
+                }
+            }
+    // readouts held by the other half-wave (same voxel, same coil)
+#pragma unroll
+    for (int c = 0; c < NCP; c++)
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[c][k] += __shfl_xor(s[c][k], 32);
+    if (lane < 32) {
 #pragma unroll
         for (int c = 0; c < NCP; c++)
 #pragma unroll
-            for (int k = 0; k < 6; k++) s[c][k] += __shfl_xor(s[c][k], 32);
-        if (lane < 32) {
-#pragma unroll
-            for (int c = 0; c < NCP; c++)
-#pragma unroll
-                for (int k = 0; k < 6; k++) sm.red[wm][wn][j][lane][c][k] = s[c][k];
-        }
+            for (int k = 0; k < 8; k++) sm.red[wm][wn][lane][c][k] = s[c][k];
     }
     __syncthreads();
 
     // ---- one thread per (coil, voxel) of the tile: sum the waves along M and
-    // the four lanes of the voxel (parts; G' conj(m) goes into d2)
+    // the two parts of the voxel; G' conj(m) joins d2
     constexpr int VOX = Tl::BNC / 2;
     for (int o = tid; o < NCP * VOX; o += Tl::THREADS) {
         const int c = o / VOX, vl = o % VOX;
-        const int v = n0 / 2 + vl;
-        if (c >= ncg || v >= nv) continue;
-        const int owm_wn = vl / 16, oj = (vl / 8) % 2, g = vl % 8;
-        float u0 = 0, u1 = 0, a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        const int vo = vox0 + vl;
+        if (c >= ncg || vo >= nv) continue;
+        const int ow = vl / 16, g = vl % 16;
+        float t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int w = 0; w < WM; w++)
 #pragma unroll
-            for (int role = 0; role < 4; role++) {
-                const float* e = sm.red[w][owm_wn][oj][4 * g + role][c];
-                if (role < 2) {
-                    u0 += e[0];
-                    u1 += e[1];
-                    a0 += e[2];
-                    a1 += e[3];
-                    b0 += e[4];
-                    b1 += e[5];
-                } else {
-                    b0 += e[0];
-                    b1 += e[1];
-                }
+            for (int pt = 0; pt < 2; pt++) {
+                const float* e = sm.red[w][ow][2 * g + pt][c];
+#pragma unroll
+                for (int k = 0; k < 8; k++) t[k] += e[k];
             }
-        float2* out = partial + (int64_t(tr) * NCP + c) * 3 * int64_t(nv) + v;
-        out[0] = make_float2(u0, u1);
-        out[nv] = make_float2(a0, a1);
-        out[2 * int64_t(nv)] = make_float2(b0, b1);
+        float2* out = partial + (int64_t(tr) * NCP + c) * 3 * int64_t(nv) + vo;
+        out[0] = make_float2(t[0], t[1]);
+        out[nv] = make_float2(t[2], t[3]);
+        out[2 * int64_t(nv)] = make_float2(t[4] + t[6], t[5] + t[7]);
     }
 }
 
