@@ -83,6 +83,8 @@ _SIGNATURES = {
     "compas_contraction_variant_count": (_i, []),
     "compas_contraction_tuning_report": (_i, [C.c_char_p, _i]),
     "compas_set_contraction_precision": (None, [_i]),
+    "compas_compute_jacobian_spiral": (_vp, [_vp] * 10),
+    "compas_compute_jacobian_hermitian_spiral": (_vp, [_vp] * 10),
 }
 
 
@@ -437,6 +439,39 @@ def compute_jacobian_hermitian(echos, dechos, parameters, trajectory, coils, y, 
                                                                     trajectory.ptr, c.ptr, yy.ptr, m))
     return _wrap(lib().compas_compute_jacobian_hermitian(ctx.ptr, c.shape[0], e.ptr, d1.ptr, d2.ptr, parameters.ptr,
                                                          trajectory.ptr, c.ptr, yy.ptr, int(bool(low_precision))))
+
+
+SPIRAL_FIELDS = ("T1", "T2", "B1", "B0")
+
+
+def _spiral_derivs(dechos):
+    if isinstance(dechos, dict):
+        dechos = [dechos[f] for f in SPIRAL_FIELDS]
+    return [as_array(d, np.complex64) for d in dechos]
+
+
+def compute_jacobian_spiral(echos, dechos, parameters, trajectory, coils, v) -> CompasArray:
+    """J.v on a spiral trajectory, six parameters (T1, T2, B1, B0, rho_x, rho_y).
+    dechos: derivatives of the echoes for T1, T2, B1, B0 (dict or sequence);
+    v: [6][nv] complex.  Returns [nc][nr][ns]."""
+    ctx = get_context()
+    e = as_array(echos, np.complex64)
+    d = _spiral_derivs(dechos)
+    c = as_array(coils, np.complex64)
+    vv = as_array(v, np.complex64)
+    return _wrap(lib().compas_compute_jacobian_spiral(ctx.ptr, e.ptr, *[x.ptr for x in d], parameters.ptr,
+                                                      trajectory.ptr, c.ptr, vv.ptr))
+
+
+def compute_jacobian_hermitian_spiral(echos, dechos, parameters, trajectory, coils, y) -> CompasArray:
+    """J^H.y on a spiral trajectory, six parameters; y: [nc][nr][ns].  Returns [6][nv]."""
+    ctx = get_context()
+    e = as_array(echos, np.complex64)
+    d = _spiral_derivs(dechos)
+    c = as_array(coils, np.complex64)
+    yy = as_array(y, np.complex64)
+    return _wrap(lib().compas_compute_jacobian_hermitian_spiral(ctx.ptr, e.ptr, *[x.ptr for x in d], parameters.ptr,
+                                                                trajectory.ptr, c.ptr, yy.ptr))
 
 
 def phase_encoding(echos, parameters, trajectory) -> CompasArray:

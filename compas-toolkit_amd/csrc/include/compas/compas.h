@@ -299,6 +299,37 @@ Array<cfloat, 2> compute_jacobian_hermitian(
     Array<cfloat, 3> vector,
     JacobianComputeMethod kind = JacobianComputeMethod::Direct);
 
+// Spiral trajectory, six parameters (no counterpart in the reference, whose
+// Jacobian is Cartesian with four): rows of `vector` and of the J^H.y result
+// are (T1, T2, B1, B0, rho_x, rho_y); the T rows are scaled by T1 and T2 as in
+// the Cartesian products.  vector is [6][nv] for J.v and [nc][nr][ns] for
+// J^H.y.  The spiral sums run on FP32 VALU for every method.
+Array<cfloat, 3> compute_jacobian(
+    const CompasContext& ctx,
+    Array<cfloat, 2> echos,
+    Array<cfloat, 2> delta_echos_T1,
+    Array<cfloat, 2> delta_echos_T2,
+    Array<cfloat, 2> delta_echos_B1,
+    Array<cfloat, 2> delta_echos_B0,
+    TissueParameters parameters,
+    SpiralTrajectory trajectory,
+    Array<cfloat, 2> coil_sensitivities,
+    Array<cfloat, 2> vector,
+    JacobianComputeMethod kind = JacobianComputeMethod::Direct);
+
+Array<cfloat, 2> compute_jacobian_hermitian(
+    const CompasContext& ctx,
+    Array<cfloat, 2> echos,
+    Array<cfloat, 2> delta_echos_T1,
+    Array<cfloat, 2> delta_echos_T2,
+    Array<cfloat, 2> delta_echos_B1,
+    Array<cfloat, 2> delta_echos_B0,
+    TissueParameters parameters,
+    SpiralTrajectory trajectory,
+    Array<cfloat, 2> coil_sensitivities,
+    Array<cfloat, 3> vector,
+    JacobianComputeMethod kind = JacobianComputeMethod::Direct);
+
 // ---------------------------------------------------------------- utils --
 
 Array<cfloat, 3> compute_residual(

@@ -74,6 +74,13 @@ bool jacobian_hermitian_mfma(
     const cfloat* d1, const cfloat* d2, const cfloat* coils, int nc, int nr, int ns,
     ContractionPrecision precision, cfloat* JHy);
 
+// Spiral sum out[c][r][s] = sum_v C[c][v] (A[r][v] + (s - ns/2) Ab[r][v]) E_rv^s
+// with the per-readout sample exponent of the spiral trajectory (Ab may be
+// nullptr).  A and Ab already carry the readout factor (rewind, prephaser).
+void spiral_sum(
+    const CompasContext& ctx, const TissueParameters& p, const cfloat* A, const cfloat* Ab, const cfloat* coils,
+    int nc, const SpiralTrajectory& t, cfloat* out);
+
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);
 

@@ -484,6 +484,40 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_method(
     });
 }
 
+CompasArrayHandle* compas_compute_jacobian_spiral(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasArrayHandle* dB1, const CompasArrayHandle* dB0,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory, const CompasArrayHandle* coils,
+    const CompasArrayHandle* vector) {
+    return catch_exceptions([&] {
+        return new_array(compute_jacobian(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
+            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
+            array(dB1, "delta_echos_B1").as<cfloat, 2>("delta_echos_B1"),
+            array(dB0, "delta_echos_B0").as<cfloat, 2>("delta_echos_B0"),
+            unwrap<TissueParameters>(parameters, "parameters"), unwrap<SpiralTrajectory>(trajectory, "trajectory"),
+            array(coils, "coils").as<cfloat, 2>("coils"), array(vector, "vector").as<cfloat, 2>("vector")));
+    });
+}
+
+CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasArrayHandle* dB1, const CompasArrayHandle* dB0,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory, const CompasArrayHandle* coils,
+    const CompasArrayHandle* vector) {
+    return catch_exceptions([&] {
+        return new_array(compute_jacobian_hermitian(
+            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
+            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
+            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
+            array(dB1, "delta_echos_B1").as<cfloat, 2>("delta_echos_B1"),
+            array(dB0, "delta_echos_B0").as<cfloat, 2>("delta_echos_B0"),
+            unwrap<TissueParameters>(parameters, "parameters"), unwrap<SpiralTrajectory>(trajectory, "trajectory"),
+            array(coils, "coils").as<cfloat, 2>("coils"), array(vector, "vector").as<cfloat, 3>("vector")));
+    });
+}
+
 void compas_set_kernel_timing(const CompasContextHandle* ctx, int enabled) {
     catch_exceptions([&] { context(ctx).runtime()->set_timing(enabled != 0); });
 }

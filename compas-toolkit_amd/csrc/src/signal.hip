@@ -61,10 +61,13 @@ __global__ void signal_spiral_factors(
 constexpr int SP_SAMPLES = 8;  // samples per thread (recurrence length)
 constexpr int SP_COILS = 4;  // coils per block
 
+// out[c][r][s] = sum_v C[c][v] (A[r][v] + (s - ns/2) Ab[r][v]) E_rv^s  (Ab only if TWO)
 // grid: (nr, ceil(ns / 64), ceil(nc / SP_COILS)); block 256 = 4 waves.
 // lane % 32 strides over voxels; (wave, lane / 32) selects 8 consecutive samples.
+template<bool TWO>
 __global__ void __launch_bounds__(256) signal_spiral_direct(
-    dev::TissueView tissue, const cfloat* __restrict__ A, const cfloat* __restrict__ coils, int nc,
+    dev::TissueView tissue, const cfloat* __restrict__ A, const cfloat* __restrict__ Ab,
+    const cfloat* __restrict__ coils, int nc,
     const cfloat* __restrict__ delta_k, int nr, int ns, float dt, cfloat* __restrict__ signal) {
     const int r = blockIdx.x;
     const int lane = threadIdx.x & 63;
@@ -92,17 +95,25 @@ __global__ void __launch_bounds__(256) signal_spiral_direct(
                 + double(dt) * dev::TWO_PI * double(tissue.B0(v));
             const float decay = -dt * R2;
             const cfloat step = dev::polar_f(expf(decay), dev::reduce_phase(theta));
-            cfloat cur = dev::polar_f(expf(float(s0) * decay), dev::reduce_phase(double(s0) * theta));
-            cur = dev::cmul(cur, A[size_t(r) * nv + v]);
+            const cfloat anchor = dev::polar_f(expf(float(s0) * decay), dev::reduce_phase(double(s0) * theta));
+            cfloat cur = dev::cmul(anchor, A[size_t(r) * nv + v]);
+            cfloat cur_b {0, 0};
+            if constexpr (TWO) cur_b = dev::cmul(anchor, Ab[size_t(r) * nv + v]);
             cfloat cv[SP_COILS];
 #pragma unroll
             for (int c = 0; c < SP_COILS; c++) cv[c] = (c0 + c < nc) ? coils[size_t(c0 + c) * nv + v] : cfloat {0, 0};
 #pragma unroll
             for (int i = 0; i < SP_SAMPLES; i++) {
+                cfloat amp = cur;
+                if constexpr (TWO) {
+                    const float sp = float(s0 + i) - 0.5f * float(ns);
+                    amp = {fmaf(sp, cur_b.re, cur.re), fmaf(sp, cur_b.im, cur.im)};
+                    cur_b = dev::cmul(cur_b, step);
+                }
 #pragma unroll
                 for (int c = 0; c < SP_COILS; c++) {
-                    acc_re[c][i] = fmaf(cur.re, cv[c].re, fmaf(-cur.im, cv[c].im, acc_re[c][i]));
-                    acc_im[c][i] = fmaf(cur.re, cv[c].im, fmaf(cur.im, cv[c].re, acc_im[c][i]));
+                    acc_re[c][i] = fmaf(amp.re, cv[c].re, fmaf(-amp.im, cv[c].im, acc_re[c][i]));
+                    acc_im[c][i] = fmaf(amp.re, cv[c].im, fmaf(amp.im, cv[c].re, acc_im[c][i]));
                 }
                 cur = dev::cmul(cur, step);
             }
@@ -179,6 +190,23 @@ static Array<cfloat, 3> signal_cartesian(
     return signal;
 }
 
+void spiral_sum(
+    const CompasContext& ctx, const TissueParameters& p, const cfloat* A, const cfloat* Ab, const cfloat* coils,
+    int nc, const SpiralTrajectory& t, cfloat* out) {
+    const int nr = t.nreadouts, ns = t.samples_per_readout;
+    if (nc == 0 || nr == 0 || ns == 0) return;
+    KernelScope ks(ctx, "signal_spiral_direct");
+    dim3 grid(nr, dev::div_ceil(ns, 64), dev::div_ceil(nc, kernels::SP_COILS));
+    if (Ab) {
+        kernels::signal_spiral_direct<true><<<grid, 256, 0, ctx.stream()>>>(
+            dev::tissue_view(p), A, Ab, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
+    } else {
+        kernels::signal_spiral_direct<false><<<grid, 256, 0, ctx.stream()>>>(
+            dev::tissue_view(p), A, nullptr, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
+    }
+    check_launch("signal_spiral_direct");
+}
+
 static Array<cfloat, 3> signal_spiral(
     const CompasContext& ctx, const Array<cfloat, 2>& echos, const TissueParameters& p,
     const SpiralTrajectory& t, const Array<cfloat, 2>& coils) {
@@ -198,13 +226,7 @@ static Array<cfloat, 3> signal_spiral(
             dev::tissue_view(p), echos.data(), t.k_start.data(), nr, ns, t.delta_t, A.data());
         check_launch("signal_spiral_factors");
     }
-    {
-        KernelScope ks(ctx, "signal_spiral_direct");
-        dim3 grid(nr, dev::div_ceil(ns, 64), dev::div_ceil(nc, kernels::SP_COILS));
-        kernels::signal_spiral_direct<<<grid, 256, 0, ctx.stream()>>>(
-            dev::tissue_view(p), A.data(), coils.data(), nc, t.delta_k.data(), nr, ns, t.delta_t, signal.data());
-        check_launch("signal_spiral_direct");
-    }
+    spiral_sum(ctx, p, A.data(), nullptr, coils.data(), nc, t, signal.data());
     return signal;
 }
 

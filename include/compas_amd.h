@@ -157,6 +157,24 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_method(
     const CompasArrayHandle* delta_echos_T2, const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory,
     const CompasArrayHandle* coils, const CompasArrayHandle* vector, int method);
 
+/* Jacobian products on a spiral trajectory with six parameters (T1, T2, B1,
+ * B0, rho_x, rho_y): the reference's Jacobian is Cartesian with four, so these
+ * have no counterpart there; they extend its conventions (T rows scaled by T,
+ * rho rows multiply m and i m) to the derivatives of its spiral signal.
+ * echos and the four derivative arrays are [nreadouts][nvoxels]; J.v takes
+ * vector [6][nvoxels] and returns [ncoils][nreadouts][nsamples]; J^H.y takes
+ * [ncoils][nreadouts][nsamples] and returns [6][nvoxels]. */
+CompasArrayHandle* compas_compute_jacobian_spiral(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasArrayHandle* delta_echos_B1,
+    const CompasArrayHandle* delta_echos_B0, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* vector);
+CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasArrayHandle* delta_echos_B1,
+    const CompasArrayHandle* delta_echos_B0, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* vector);
+
 /* Per-kernel timing with HIP events on the context's stream.  While enabled,
  * every library launch is bracketed by events; compas_read_kernel_timings
  * synchronizes, writes "name count total_ms\n" lines into buf (NUL-terminated,

@@ -604,6 +604,135 @@ void SFX(jacobian_hermitian)(
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Jacobian products on the spiral trajectory, 6 parameters                 */
+/* (T1, T2, B1, B0, rho_x, rho_y).  The reference has no spiral Jacobian:    */
+/* these are the exact derivatives of its spiral signal (the S of            */
+/* signal_spiral above, spiral_view.cuh:13-73) with the parameter scaling of */
+/* its Cartesian J.v (product_kernels.cuh:56-60: the T1 and T2 rows are      */
+/* scaled by T1 and T2, the rho rows multiply m and i m).  With              */
+/*   Phi[r][s][v] = exp((s - ns/2) dt (-R2 + i 2 pi B0)                      */
+/*                      + i (Re k0_r x + s (Re dk_r x + Im dk_r y)))         */
+/* S = sum_v C rho m Phi, dPhi/dT2 = (s - ns/2) dt R2^2 Phi and              */
+/* dPhi/dB0 = (s - ns/2) i 2 pi dt Phi.  Pinned by finite differences of     */
+/* signal_spiral and by the adjoint identity <y, J v> = <J^H y, v>.          */
+/* ------------------------------------------------------------------------ */
+
+/* Phi[r][0][v] (readout factor of signal_spiral) and the per-sample exponent */
+static inline void SFX(spiral_phi)(int ns, REAL dt, CPX k0, CPX dkr, REAL T2, REAL B0, REAL x, REAL y, CPX* phi0, CPX* z) {
+    REAL R2 = (REAL)1 / T2;
+    CPX rw = C_EXP(C_SCALE(C_MK(R2, -(REAL)(2.0 * M_PI) * B0), (REAL)(0.5 * ns) * dt));
+    CPX pre = C_EXP(C_MK(0, k0.re * x));
+    REAL Theta = dkr.re * x + dkr.im * y;
+    *phi0 = C_MUL(rw, pre);
+    Theta += dt * (REAL)(2.0 * M_PI) * B0;
+    *z = C_MK(-dt * R2, Theta);
+}
+
+/*
+ * J.v, spiral, 6 parameters.  vec: [6][nv] complex (T1, T2, B1, B0, rho_x,
+ * rho_y); d*: [nr][nv] derivatives of the magnetisation; Jv: [nc][nr][ns].
+ *   Jv = sum_v C (a + (s - ns/2) b) Phi,
+ *   a = rho (v0 T1 dm_T1 + v1 T2 dm_T2 + v2 dm_B1 + v3 dm_B0) + (v4 + i v5) m
+ *   b = rho m dt (v1 T2 R2^2 + v3 i 2 pi)
+ */
+void SFX(jacobian_product_spiral)(
+    int nv, const float* T1, const float* T2, const float* B0, const float* rho, const float* x, const float* y,
+    int nr, int ns, float dt, const float* k_start, const float* dk, int nc, const float* coils,
+    const REAL* echos, const REAL* dT1, const REAL* dT2, const REAL* dB1, const REAL* dB0, const REAL* vec,
+    REAL* Jv, int nthreads) {
+    long row;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+    for (row = 0; row < (long)nc * nr; row++) {
+        int c = (int)(row / nr), r = (int)(row % nr), s;
+        CPX k0 = C_LDC(k_start, r), dkr = C_LDC(dk, r);
+        long v;
+        REAL* accre = (REAL*)calloc((size_t)ns, sizeof(REAL));
+        REAL* accim = (REAL*)calloc((size_t)ns, sizeof(REAL));
+        for (v = 0; v < nv; v++) {
+            long i = (long)r * nv + v;
+            REAL R2 = (REAL)1 / (REAL)T2[v];
+            CPX phi0, z, a, b, cv = C_LDC(coils, (long)c * nv + v), rh = C_LDC(rho, v), me = C_LDR(echos, i);
+            SFX(spiral_phi)(ns, (REAL)dt, k0, dkr, (REAL)T2[v], (REAL)B0[v], (REAL)x[v], (REAL)y[v], &phi0, &z);
+            a = C_SCALE(C_MUL(C_LDR(vec, v), C_LDR(dT1, i)), (REAL)T1[v]);
+            a = C_ADD(a, C_SCALE(C_MUL(C_LDR(vec, (long)nv + v), C_LDR(dT2, i)), (REAL)T2[v]));
+            a = C_ADD(a, C_MUL(C_LDR(vec, 2L * nv + v), C_LDR(dB1, i)));
+            a = C_ADD(a, C_MUL(C_LDR(vec, 3L * nv + v), C_LDR(dB0, i)));
+            a = C_MUL(a, rh);
+            a = C_ADD(a, C_MUL(C_ADD(C_LDR(vec, 4L * nv + v), C_MUL(C_LDR(vec, 5L * nv + v), C_MK(0, 1))), me));
+            b = C_ADD(C_SCALE(C_LDR(vec, (long)nv + v), (REAL)T2[v] * R2 * R2),
+                      C_MUL(C_LDR(vec, 3L * nv + v), C_MK(0, (REAL)(2.0 * M_PI))));
+            b = C_SCALE(C_MUL(C_MUL(b, rh), me), (REAL)dt);
+            for (s = 0; s < ns; s++) {
+                CPX phi = C_MUL(phi0, C_EXP(C_SCALE(z, (REAL)s)));
+                CPX amp = C_ADD(a, C_SCALE(b, (REAL)s - (REAL)0.5 * (REAL)ns));
+                CPX t = C_MUL(C_MUL(amp, phi), cv);
+                accre[s] += t.re;
+                accim[s] += t.im;
+            }
+        }
+        for (s = 0; s < ns; s++) C_ST(Jv, ((long)c * nr + r) * ns + s, C_MK(accre[s], accim[s]));
+        free(accre);
+        free(accim);
+    }
+}
+
+/*
+ * J^H.y, spiral, 6 parameters: the adjoint of jacobian_product_spiral.
+ * yv: [nc][nr][ns]; JHy: [6][nv].  Per voxel, coil and readout
+ *   U = sum_s y conj(Phi),  U' = sum_s (s - ns/2) y conj(Phi)
+ * and JHy = sum_c (conj(T1 rho C) sum_r conj(dm_T1) U,
+ *                  conj(T2 rho C) sum_r [conj(dm_T2) U + conj(m) dt R2^2 U'],
+ *                  conj(rho C) sum_r conj(dm_B1) U,
+ *                  conj(rho C) sum_r [conj(dm_B0) U - i 2 pi dt conj(m) U'],
+ *                  conj(C) sum_r conj(m) U,  -i conj(C) sum_r conj(m) U).
+ */
+void SFX(jacobian_hermitian_spiral)(
+    int nv, const float* T1, const float* T2, const float* B0, const float* rho, const float* x, const float* y,
+    int nr, int ns, float dt, const float* k_start, const float* dk, int nc, const float* coils,
+    const REAL* echos, const REAL* dT1, const REAL* dT2, const REAL* dB1, const REAL* dB0, const REAL* yv,
+    REAL* JHy, int nthreads) {
+    long v;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 8)
+    for (v = 0; v < nv; v++) {
+        CPX out[6];
+        int c, r, s, p;
+        REAL R2 = (REAL)1 / (REAL)T2[v];
+        CPX rh = C_LDC(rho, v);
+        for (p = 0; p < 6; p++) out[p] = C_MK(0, 0);
+        for (c = 0; c < nc; c++) {
+            CPX g1 = C_MK(0, 0), g2 = C_MK(0, 0), gb1 = C_MK(0, 0), gb0 = C_MK(0, 0), u = C_MK(0, 0);
+            CPX cc = C_LDC(coils, (long)c * nv + v);
+            for (r = 0; r < nr; r++) {
+                long i = (long)r * nv + v;
+                CPX phi0, z, U = C_MK(0, 0), Up = C_MK(0, 0);
+                CPX mc = C_CONJ(C_LDR(echos, i));
+                SFX(spiral_phi)(ns, (REAL)dt, C_LDC(k_start, r), C_LDC(dk, r), (REAL)T2[v], (REAL)B0[v], (REAL)x[v],
+                                (REAL)y[v], &phi0, &z);
+                for (s = 0; s < ns; s++) {
+                    CPX phi = C_MUL(phi0, C_EXP(C_SCALE(z, (REAL)s)));
+                    CPX t = C_MUL(C_LDR(yv, ((long)c * nr + r) * ns + s), C_CONJ(phi));
+                    U = C_ADD(U, t);
+                    Up = C_ADD(Up, C_SCALE(t, (REAL)s - (REAL)0.5 * (REAL)ns));
+                }
+                g1 = C_ADD(g1, C_MUL(C_CONJ(C_LDR(dT1, i)), U));
+                g2 = C_ADD(g2, C_ADD(C_MUL(C_CONJ(C_LDR(dT2, i)), U), C_SCALE(C_MUL(mc, Up), (REAL)dt * R2 * R2)));
+                gb1 = C_ADD(gb1, C_MUL(C_CONJ(C_LDR(dB1, i)), U));
+                gb0 = C_ADD(gb0, C_ADD(C_MUL(C_CONJ(C_LDR(dB0, i)), U),
+                                       C_MUL(C_MUL(mc, Up), C_MK(0, -(REAL)(2.0 * M_PI) * (REAL)dt))));
+                u = C_ADD(u, C_MUL(mc, U));
+            }
+            out[0] = C_ADD(out[0], C_MUL(C_CONJ(C_MUL(C_SCALE(rh, (REAL)T1[v]), cc)), g1));
+            out[1] = C_ADD(out[1], C_MUL(C_CONJ(C_MUL(C_SCALE(rh, (REAL)T2[v]), cc)), g2));
+            out[2] = C_ADD(out[2], C_MUL(C_CONJ(C_MUL(rh, cc)), gb1));
+            out[3] = C_ADD(out[3], C_MUL(C_CONJ(C_MUL(rh, cc)), gb0));
+            out[4] = C_ADD(out[4], C_MUL(C_CONJ(cc), u));
+            out[5] = C_ADD(out[5], C_MUL(C_MUL(C_CONJ(cc), u), C_MK(0, -1)));
+        }
+        for (p = 0; p < 6; p++) C_ST(JHy, (long)p * nv + v, out[p]);
+    }
+}
+
 #undef C_MK
 #undef C_ADD
 #undef C_SUB

@@ -188,6 +188,35 @@ def jacobian_hermitian(tissue, traj, coils, echos, dT1, dT2, yv, precision="f64"
     return out
 
 
+def _spiral6(name, shape, tissue, traj, coils, arrays, precision, threads):
+    _, ct = _real_t(precision)
+    nv = tissue.nvoxels
+    nc = coils.shape[0]
+    out = np.zeros(shape, dtype=ct)
+    ins = [_creal(a, precision) for a in arrays]
+    cl = _cf32(coils)
+    ks, dk = _cf32(traj.k_start), _cf32(traj.delta_k)
+    ta = _jac_common(tissue, traj)
+    _fn(name, precision)(
+        C.c_int(nv), *[_p(a) for a in ta], C.c_int(traj.nreadouts), C.c_int(traj.samples_per_readout),
+        C.c_float(traj.delta_t), _p(ks), _p(dk), C.c_int(nc), _p(cl), *[_p(a) for a in ins], _p(out),
+        C.c_int(_threads(threads)))
+    return out
+
+
+def jacobian_spiral(tissue, traj, coils, echos, dT1, dT2, dB1, dB0, vec, precision="f64", threads=None):
+    """J.v on a spiral trajectory, 6 parameters (T1, T2, B1, B0, rho_x, rho_y); vec [6][nv]."""
+    nc = coils.shape[0]
+    return _spiral6("jacobian_product_spiral", (nc, traj.nreadouts, traj.samples_per_readout), tissue, traj,
+                    coils, (echos, dT1, dT2, dB1, dB0, vec), precision, threads)
+
+
+def jacobian_hermitian_spiral(tissue, traj, coils, echos, dT1, dT2, dB1, dB0, yv, precision="f64", threads=None):
+    """J^H.y on a spiral trajectory, 6 parameters; returns [6][nv]."""
+    return _spiral6("jacobian_hermitian_spiral", (6, tissue.nvoxels), tissue, traj, coils,
+                    (echos, dT1, dT2, dB1, dB0, yv), precision, threads)
+
+
 def rel_l2(x, ref):
     x = np.asarray(x, dtype=np.complex128)
     ref = np.asarray(ref, dtype=np.complex128)

@@ -249,6 +249,55 @@ def test_spiral_signal(ctx, oracle):
     assert rel(got, want) < SIG_TOL
 
 
+def spiral6_inputs(nc, nr, ns, nv, seed):
+    rng = W.SplitMix64(seed)
+    n = int(np.ceil(np.sqrt(nv)))
+    t = W.make_tissue(rng, n, 22.4, 50.0).slice(0, nv)
+    traj = W.make_spiral(nr, ns, 22.4, n, 5e-6)
+
+    def cn(*shape, s=1.0):
+        k = int(np.prod(shape))
+        return ((rng.normal(k) + 1j * rng.normal(k)) * s).reshape(shape).astype(np.complex64)
+
+    coils = cn(nc, nv, s=0.5)
+    echos = cn(nr, nv, s=0.1)
+    ds = [cn(nr, nv, s=0.01), cn(nr, nv, s=0.1), cn(nr, nv, s=0.1), cn(nr, nv, s=1e-3)]  # T1, T2, B1, B0
+    v = rng.normal(6 * nv).reshape(6, nv).astype(np.complex64)  # real perturbation, stored complex
+    y = cn(nc, nr, ns, s=0.7)
+    return t, traj, coils, echos, ds, v, y
+
+
+@pytest.mark.parametrize("nc,nr,ns,nv", [(4, 70, 64, 784), (1, 33, 50, 333), (5, 41, 37, 1000), (8, 16, 130, 257)])
+def test_spiral_jacobian_six_parameters(ctx, oracle, nc, nr, ns, nv):
+    # J.v and J^H.y on a golden-angle spiral with (T1, T2, B1, B0, rho_x, rho_y)
+    banner(f"spiral J 6 params coils={nc} nr={nr} ns={ns} nv={nv}")
+    t, traj, coils, echos, ds, v, y = spiral6_inputs(nc, nr, ns, nv, 7 * nc + nr)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    straj = ca.SpiralTrajectory.from_trajectory(traj)
+    jv = ca.compute_jacobian_spiral(echos, ds, params, straj, coils, v).collect()
+    assert rel(jv, oracle.jacobian_spiral(t, traj, coils, echos, *ds, v, "f64")) < SIG_TOL
+    jh = ca.compute_jacobian_hermitian_spiral(echos, ds, params, straj, coils, y).collect()
+    assert rel(jh, oracle.jacobian_hermitian_spiral(t, traj, coils, echos, *ds, y, "f64")) < SIG_TOL
+    # the two GPU products are adjoint
+    lhs = np.vdot(y.ravel().astype(np.complex128), jv.ravel())
+    rhs = np.vdot(jh.ravel().astype(np.complex128), v.ravel().astype(np.complex128))
+    assert abs(lhs - rhs) / abs(lhs) < SIG_TOL
+
+
+def test_spiral_jacobian_shape_errors(ctx):
+    t, traj, coils, echos, ds, v, y = spiral6_inputs(2, 10, 16, 64, 5)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    straj = ca.SpiralTrajectory.from_trajectory(traj)
+    with pytest.raises(ca.CompasError):
+        ca.compute_jacobian_spiral(echos, ds, params, straj, coils, v[:4])  # four rows
+    with pytest.raises(ca.CompasError):
+        ca.compute_jacobian_hermitian_spiral(echos, ds, params, straj, coils, y[:, :5])
+    with pytest.raises(ca.CompasError):
+        ca.compute_jacobian_spiral(echos[:5], ds, params, straj, coils, v)
+
+
 def test_phase_encoding(ctx, oracle):
     wl = W.config(1, scale=2)
     params, seq, traj = objects(wl)

@@ -363,3 +363,109 @@ def test_jacobian_T_rows_equal_finite_difference(oracle, field):
         sig.append(oracle.signal_cartesian(tp, traj, coils, oracle.fisp(tp, wl.sequence, "f64"), "f64"))
     fd = (sig[0] - sig[1]) / (2 * H)
     assert rel(fd, Jv) < 1e-5
+
+
+# ---------------------------------------------- spiral jacobian, 6 params ---
+
+SPIRAL_FIELDS = ("T1", "T2", "B1", "B0")
+B0_STEP = 2.0 ** -16  # Hz; exact in float32 for |B0| < 64 Hz
+
+
+def _spiral_setup():
+    wl, coils = _jac_setup()
+    n = int(round(np.sqrt(wl.tissue.nvoxels)))
+    traj = W.make_spiral(wl.trajectory.nreadouts, 32, wl.fov, n, 5e-6)
+    return wl, coils, traj
+
+
+def _step(t, name, direction, sgn):
+    """Tissue with field `name` moved by sgn * direction along the parameter's
+    scaling (T1, T2: relative by H; B1: absolute H; B0: absolute B0_STEP), and
+    the step per voxel in the parameter's own units."""
+    base = getattr(t, name).astype(np.float64)
+    if name in ("T1", "T2"):
+        return _perturbed(t, name, sgn * direction), H * base
+    h = H if name == "B1" else B0_STEP
+    out = (base + sgn * h * direction).astype(np.float32)
+    assert np.array_equal(out.astype(np.float64), base + sgn * h * direction)  # exact steps
+    return t.replace(**{name: out}), np.full(t.nvoxels, h)
+
+
+def _spiral_derivs(oracle, wl):
+    t, seq = wl.tissue, wl.sequence
+    one = np.ones(t.nvoxels)
+    out = []
+    for f in SPIRAL_FIELDS:
+        tp, h = _step(t, f, one, 1)
+        tm, _ = _step(t, f, one, -1)
+        out.append((oracle.fisp(tp, seq, "f64") - oracle.fisp(tm, seq, "f64")) / (2 * h)[None, :])
+    return out
+
+
+def test_spiral_jacobian_rho_columns_equal_signal(oracle):
+    wl, coils, traj = _spiral_setup()
+    t = wl.tissue
+    nv = t.nvoxels
+    echos = oracle.fisp(t, wl.sequence, "f64")
+    zero = np.zeros_like(echos)
+    S1 = oracle.signal_spiral(t.replace(rho=np.ones(nv, np.complex64)), traj, coils, echos, "f64")
+    v = np.zeros((6, nv), complex)
+    v[4] = 1
+    got = oracle.jacobian_spiral(t, traj, coils, echos, zero, zero, zero, zero, v, "f64")
+    assert rel(got, S1) < 1e-11
+    v[4], v[5] = 0, 1
+    got = oracle.jacobian_spiral(t, traj, coils, echos, zero, zero, zero, zero, v, "f64")
+    assert rel(got, 1j * S1) < 1e-11
+
+
+def test_spiral_jacobian_adjoint_identity(oracle):
+    wl, coils, traj = _spiral_setup()
+    t = wl.tissue
+    nv = t.nvoxels
+    rng = W.SplitMix64(4321)
+    echos = oracle.fisp(t, wl.sequence, "f64")
+    ds = _spiral_derivs(oracle, wl)
+    v = (rng.normal(6 * nv) + 1j * rng.normal(6 * nv)).reshape(6, nv)
+    nc, nr, ns = coils.shape[0], traj.nreadouts, traj.samples_per_readout
+    yv = (rng.normal(nc * nr * ns) + 1j * rng.normal(nc * nr * ns)).reshape(nc, nr, ns)
+    Jv = oracle.jacobian_spiral(t, traj, coils, echos, *ds, v, "f64")
+    JHy = oracle.jacobian_hermitian_spiral(t, traj, coils, echos, *ds, yv, "f64")
+    lhs = np.vdot(yv.ravel(), Jv.ravel())
+    rhs = np.vdot(JHy.ravel(), v.ravel())
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3])
+def test_spiral_jacobian_rows_equal_finite_difference(oracle, field):
+    # each J.v row is the derivative of the whole pipeline S_spiral(FISP(p), p) along
+    # random signs (T1, T2 scaled by T as in the reference's Cartesian J.v)
+    wl, coils, traj = _spiral_setup()
+    t = wl.tissue
+    nv = t.nvoxels
+    echos = oracle.fisp(t, wl.sequence, "f64")
+    ds = _spiral_derivs(oracle, wl)
+    sign = np.where(W.SplitMix64(99 + field).uniform(nv) < 0.5, -1.0, 1.0)
+    v = np.zeros((6, nv), complex)
+    v[field] = sign
+    Jv = oracle.jacobian_spiral(t, traj, coils, echos, *ds, v, "f64")
+    name = SPIRAL_FIELDS[field]
+    sig = []
+    for sgn in (1, -1):
+        tp, _ = _step(t, name, sign, sgn)
+        sig.append(oracle.signal_spiral(tp, traj, coils, oracle.fisp(tp, wl.sequence, "f64"), "f64"))
+    h = H if name != "B0" else B0_STEP
+    fd = (sig[0] - sig[1]) / (2 * h)
+    assert rel(fd, Jv) < 1e-5
+
+
+def test_spiral_jacobian_f32_port_close_to_f64(oracle):
+    wl, coils, traj = _spiral_setup()
+    t = wl.tissue
+    nv = t.nvoxels
+    rng = W.SplitMix64(11)
+    echos = oracle.fisp(t, wl.sequence, "f64")
+    ds = _spiral_derivs(oracle, wl)
+    v = (rng.normal(6 * nv) + 1j * rng.normal(6 * nv)).reshape(6, nv)
+    a = oracle.jacobian_spiral(t, traj, coils, echos, *ds, v, "f64")
+    b = oracle.jacobian_spiral(t, traj, coils, echos, *ds, v, "f32")
+    assert rel(b, a) < 1e-4
