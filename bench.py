@@ -236,6 +236,197 @@ def products_bench(args):
     print(json.dumps(line))
 
 
+def spiral_bench(args):
+    """pSSFP + spiral workload: 50176 voxels x 1120 TRs x 35 isochromats x 25 RF
+    sub-steps, golden-angle spiral 4 coils x 1120 x 512, six-parameter J.v and
+    J^H.y.  One step = simulate_magnetization (pSSFP) + magnetization_to_signal
+    (spiral) + J.v + J^H.y; the finite-difference derivatives of the echoes
+    (T1, T2, B1, B0) are computed once before timing and held fixed.  value =
+    voxel.TR/s of the step; the CPU baseline runs the FP32 port of the same four
+    operations on bounded subsets (voxels for the simulation and J^H.y, readouts
+    for the signal and J.v) and extrapolates linearly."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    wl = W.config(3)
+    nv = wl.tissue.nvoxels
+    nTR = int(wl.sequence.RF.shape[0])
+    nz, nRF = int(wl.sequence.z.shape[0]), int(wl.sequence.gamma_dt_RF.shape[0])
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+    ctx = ca.init_context(0)
+    params = ca.TissueParameters.from_tissue(wl.tissue)
+    seq = ca.pSSFPSequence.from_sequence(wl.sequence)
+    traj = ca.SpiralTrajectory.from_trajectory(wl.trajectory)
+    coils = ca.CompasArray.from_numpy(wl.coils)
+    v = ca.CompasArray.from_numpy(wl.extra["v"])
+    y = ca.CompasArray.from_numpy(wl.extra["y"])
+    echos0 = ca.simulate_magnetization(params, seq)
+    ds = [ca.simulate_magnetization_derivative(f, echos0, params, seq) for f in (ca.T1, ca.T2, ca.B1, ca.B0)]
+    ctx.synchronize()
+
+    def step():
+        e = ca.simulate_magnetization(params, seq)
+        s = ca.magnetization_to_signal(e, params, traj, coils)
+        jv = ca.compute_jacobian_spiral(e, ds, params, traj, coils, v)
+        jh = ca.compute_jacobian_hermitian_spiral(e, ds, params, traj, coils, y)
+        return e, s, jv, jh
+
+    for _ in range(args.warmup):
+        out = step()
+    ctx.synchronize()
+    del out
+    ctx.read_kernel_timings()
+    ctx.set_kernel_timing(True)
+    lat = []
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = step()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+    t_total = time.perf_counter() - t_start
+    ctx.set_kernel_timing(False)
+    timings = ctx.read_kernel_timings()
+    kernels = {k: {"launches": c, "avg_ms": tot / max(c, 1), "total_ms": tot} for k, (c, tot) in timings.items()}
+
+    # algorithmic work per launch (FP32 VALU): pSSFP (57 nRF + 28) flop per isochromat.TR; spiral
+    # sums 8 nc flop per (readout, sample, voxel) for the coil MACs + 6 for the sample recurrence
+    # (J.v: two amplitudes, 12); J^H.y: 8 nc (y conj(Phi)) + 4 nc (the (s - ns/2) moment) + 6
+    rsv = float(nr) * ns * nv
+    work = {"pssfp_iso": (57.0 * nRF + 28) * nz * nTR * nv,
+            "signal_spiral_direct": None,  # two launches per step with different amplitude counts
+            "spiral_hermitian": (12.0 * nc + 6) * rsv}
+    rooflines = {}
+    for k in ("pssfp_iso", "spiral_hermitian"):
+        if k in kernels:
+            ach = work[k] / (kernels[k]["avg_ms"] * 1e-3) / 1e12
+            rooflines[k] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                            "frac": ach / PEAK_FP32_TFLOPS}
+    if "signal_spiral_direct" in kernels:
+        flop = (8.0 * nc + 6) * rsv + (8.0 * nc + 12 + 2) * rsv  # signal + J.v launches of one step
+        ach = flop / (kernels["signal_spiral_direct"]["total_ms"] / args.steps * 1e-3) / 1e12
+        rooflines["signal_spiral_direct"] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS,
+                                             "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS}
+    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    roof = dict(rooflines.get(dominant, {}), kernel=dominant, traffic=None) if dominant in rooflines else None
+
+    e, s, jv, jh = (a.collect() for a in out)
+    dsn = [d.collect() for d in ds]
+    line = {
+        "metric": "voxel*TR/s sim + k-space samples/s (pSSFP + spiral signal + 6-parameter J.v and J^H.y)",
+        "value": nv * nTR * args.steps / t_total,
+        "unit": "voxel*TR/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 tissue maps, sinc-Gaussian RF, golden-angle spiral, Gaussian coils)",
+        "config": {"workload": f"pSSFP {nv} voxels x {nTR} TRs x {nz} isochromats x {nRF} RF steps + spiral signal "
+                               f"{nc} coils x {nr} readouts x {ns} samples + J.v [6][{nv}] + J^H.y"},
+        "isochromat_tr_per_s": nv * nTR * nz * args.steps / t_total,
+        "kspace_samples_per_s": 3 * nc * nr * ns * args.steps / t_total,
+        "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99)},
+        "kernels": kernels,
+        "rooflines": rooflines,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        base, acc = spiral_cpu_baseline(wl, e, s, jv, jh, dsn)
+        line["cpu_baseline"] = base
+        line["accuracy"] = acc
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+def spiral_cpu_baseline(wl, e, s, jv, jh, ds, seconds_budget=16.0):
+    """FP32 port (oracle/) of the pSSFP + spiral step on bounded subsets, and the
+    error of the GPU outputs against the FP64 oracle on the same subsets."""
+    import pyoracle
+
+    pyoracle.build()
+    threads = int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    nv = wl.tissue.nvoxels
+    nTR = int(wl.sequence.RF.shape[0])
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+    budget = seconds_budget / 4
+
+    def calibrated(run, unit_max, first):
+        t0 = time.perf_counter()
+        run(first)
+        t_cal = max(time.perf_counter() - t0, 1e-6)
+        return int(min(unit_max, max(first, first * budget / t_cal)))
+
+    def tissue_sub(idx):
+        return wl.tissue.replace(**{f: getattr(wl.tissue, f)[idx] for f in ("T1", "T2", "B1", "B0", "rho", "x", "y",
+                                                                            "z")})
+
+    def traj_sub(ridx):
+        t = wl.trajectory
+        return W.SpiralTraj(len(ridx), ns, t.delta_t, t.k_start[ridx], t.delta_k[ridx])
+
+    # simulation: contiguous voxel slice
+    nsim = calibrated(lambda n: pyoracle.pssfp(wl.tissue.slice(0, n), wl.sequence, "f32", threads), nv, 16)
+    t0 = time.perf_counter()
+    pyoracle.pssfp(wl.tissue.slice(0, nsim), wl.sequence, "f32", threads)
+    t_sim = time.perf_counter() - t0
+    e64 = pyoracle.pssfp(wl.tissue.slice(0, nsim), wl.sequence, "f64", threads)
+    # signal and J.v: readout subsets (rows are independent)
+    def rows(n):
+        return np.linspace(0, nr - 1, n).astype(int)
+
+    nsig = calibrated(lambda n: pyoracle.signal_spiral(wl.tissue, traj_sub(rows(n)), wl.coils, e[rows(n)], "f32",
+                                                       threads), nr, 2)
+    ri = rows(nsig)
+    t0 = time.perf_counter()
+    pyoracle.signal_spiral(wl.tissue, traj_sub(ri), wl.coils, e[ri], "f32", threads)
+    t_sig = time.perf_counter() - t0
+    s64 = pyoracle.signal_spiral(wl.tissue, traj_sub(ri), wl.coils, e[ri].astype(np.complex128), "f64", threads)
+    v = wl.extra["v"]
+    njv = max(2, nsig // 2)
+    rj = rows(njv)
+    t0 = time.perf_counter()
+    pyoracle.jacobian_spiral(wl.tissue, traj_sub(rj), wl.coils, e[rj], *[d[rj] for d in ds], v, "f32", threads)
+    t_jv = time.perf_counter() - t0
+    jv64 = pyoracle.jacobian_spiral(wl.tissue, traj_sub(rj), wl.coils, e[rj], *[d[rj] for d in ds], v, "f64",
+                                    threads)
+    # J^H.y: voxel subset (outputs per voxel are independent)
+    nvh = calibrated(lambda n: pyoracle.jacobian_hermitian_spiral(
+        tissue_sub(np.arange(n)), wl.trajectory, wl.coils[:, :n], e[:, :n], *[d[:, :n] for d in ds], wl.extra["y"],
+        "f32", threads), nv, 8)
+    vi = np.linspace(0, nv - 1, nvh).astype(int)
+    ts = tissue_sub(vi)
+    t0 = time.perf_counter()
+    pyoracle.jacobian_hermitian_spiral(ts, wl.trajectory, wl.coils[:, vi], e[:, vi], *[d[:, vi] for d in ds],
+                                       wl.extra["y"], "f32", threads)
+    t_jh = time.perf_counter() - t0
+    jh64 = pyoracle.jacobian_hermitian_spiral(ts, wl.trajectory, wl.coils[:, vi], e[:, vi], *[d[:, vi] for d in ds],
+                                              wl.extra["y"], "f64", threads)
+    t_step = (nv / nsim) * t_sim + (nr / nsig) * t_sig + (nr / njv) * t_jv + (nv / nvh) * t_jh
+    base = {
+        "value": nv * nTR / t_step,
+        "unit": "voxel*TR/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP32 port: pSSFP on voxels [0, {nsim}) ({t_sim:.2f} s), spiral signal on {nsig} of {nr} readouts "
+                   f"({t_sig:.2f} s), J.v on {njv} readouts ({t_jv:.2f} s), J^H.y on {nvh} voxels ({t_jh:.2f} s); "
+                   f"full-step time extrapolated linearly ({t_step:.1f} s)"),
+    }
+    acc = {
+        "magnetization_rel_l2_vs_fp64": pyoracle.rel_l2(e[:, :nsim], e64),
+        "signal_rel_l2_vs_fp64": pyoracle.rel_l2(s[:, ri, :], s64),
+        "jv_rel_l2_vs_fp64": pyoracle.rel_l2(jv[:, rj, :], jv64),
+        "jhy_rel_l2_vs_fp64": pyoracle.rel_l2(jh[:, vi], jh64),
+        "note": "signal, J.v and J^H.y are checked against the FP64 oracle on the GPU echoes and derivatives",
+    }
+    return base, acc
+
+
 def precision_study(args):
     """Contraction precisions side by side: FP32 VALU, BF16 hi+lo, FP16 hi+lo
     (per-row power-of-two scaling), BF16 x1 and FP16 x1.  For the 4-coil
@@ -352,7 +543,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3])
     ap.add_argument("--method", default="matmul_fast", choices=list(ca.SIGNAL_METHODS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--products", action="store_true",
@@ -370,6 +561,9 @@ def main():
         return
     if args.products:
         products_bench(args)
+        return
+    if args.workload == 3:
+        spiral_bench(args)
         return
 
     wl = W.config(args.workload)
