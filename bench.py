@@ -427,6 +427,198 @@ def spiral_cpu_baseline(wl, e, s, jv, jh, ds, seconds_budget=16.0):
     return base, acc
 
 
+SERVE_NS = 128  # samples per readout of the serving requests (Cartesian and spiral)
+
+
+def serve_request_spec(i):
+    """Request i of the serving mix (seed = 0xC0A50005 + i): type FISP sim+signal
+    40%, pSSFP sim+signal 30%, J^H.y 30%; voxels log-uniform 1e3..2e5; TRs
+    uniform 250..2000; coils uniform 1..8; Cartesian or spiral 50/50."""
+    rng = W.SplitMix64(0xC0A50005 + i)
+    u = rng.uniform(6)
+    kind = "fisp" if u[0] < 0.4 else ("pssfp" if u[0] < 0.7 else "jhy")
+    nv = int(round(np.exp(np.log(1e3) + u[1] * (np.log(2e5) - np.log(1e3)))))
+    nTR = 250 + min(1750, int(u[2] * 1751))
+    nc = 1 + min(7, int(u[3] * 8))
+    spiral = bool(u[4] < 0.5)
+    return {"id": i, "kind": kind, "nv": nv, "nTR": nTR, "nc": nc, "spiral": spiral, "offset": float(u[5])}
+
+
+class ServeCatalog:
+    """Resident objects of the serving run: tissue maps (slices of one
+    200704-voxel map), sequences and trajectories per length, coil maps and
+    residuals per request -- all uploaded before the clock starts."""
+
+    def __init__(self):
+        n = 448
+        self.fov = 22.4
+        self.master = W.make_tissue(W.SplitMix64(0xC0A50005), n, self.fov, 50.0)
+        self.seqs, self.trajs = {}, {}
+
+    def sequence(self, kind, nTR):
+        key = (kind == "pssfp", nTR)
+        if key not in self.seqs:
+            if kind == "pssfp":
+                self.seqs[key] = ca.pSSFPSequence.from_sequence(W.make_pssfp(nTR, 70.0))
+            else:
+                self.seqs[key] = ca.FispSequence.from_sequence(W.make_fisp(nTR, 70.0, 25))
+        return self.seqs[key]
+
+    def trajectory(self, spiral, nTR, n):
+        key = (spiral, nTR, n)
+        if key not in self.trajs:
+            if spiral:
+                t = W.make_spiral(nTR, SERVE_NS, self.fov, n, 5e-6)
+                self.trajs[key] = ca.SpiralTrajectory.from_trajectory(t)
+            else:
+                t = W.make_cartesian(n, nTR // n + 1, SERVE_NS, self.fov, 5e-6)
+                t = W.CartesianTraj(nTR, SERVE_NS, t.delta_t, t.k_start[:nTR], t.delta_k)
+                self.trajs[key] = ca.CartesianTrajectory.from_trajectory(t)
+        return self.trajs[key]
+
+    def request(self, spec):
+        nv, nc = spec["nv"], spec["nc"]
+        total = self.master.nvoxels
+        off = int(spec["offset"] * (total - nv))
+        t = self.master.slice(off, off + nv)
+        n = int(np.ceil(np.sqrt(nv)))
+        rng = W.SplitMix64(0xC0A50005 + spec["id"] + 0x9E3779B9)
+        coils = W.make_coils(n, self.fov, nc)[:, :nv]
+        phase = np.exp(1j * rng.uniform(nc) * 2 * np.pi)[:, None]
+        r = {"spec": spec, "params": ca.TissueParameters.from_tissue(t),
+             "seq": self.sequence(spec["kind"], spec["nTR"]),
+             "traj": self.trajectory(spec["spiral"], spec["nTR"], n),
+             "coils": ca.CompasArray.from_numpy((coils * phase).astype(np.complex64))}
+        if spec["kind"] == "jhy":
+            k = nc * spec["nTR"] * SERVE_NS
+            y = ((rng.normal(k) + 1j * rng.normal(k)) / np.sqrt(2)).reshape(nc, spec["nTR"], SERVE_NS)
+            r["y"] = ca.CompasArray.from_numpy(y.astype(np.complex64))
+        return r
+
+
+def serve_submit(server, r):
+    if r["spec"]["kind"] == "jhy":
+        return server.submit_jacobian_hermitian(r["params"], r["seq"], r["traj"], r["coils"], r["y"], "gemm_fast")
+    return server.submit_simulate_signal(r["params"], r["seq"], r["traj"], r["coils"], "matmul_fast")
+
+
+def serve_bench(args):
+    """Serving loop: requests with Poisson arrivals at `--load` x the measured
+    capacity, continuous batching on `--slots` slots, p50/p99 latency (arrival
+    -> result ready in HBM) and aggregate throughput; a seeded sample of the
+    requests is re-run alone and compared with its result in the mix."""
+    import threading
+
+    ctx = ca.init_context(0)
+    n_req = args.requests
+    specs = [serve_request_spec(i) for i in range(n_req)]
+    t0 = time.perf_counter()
+    cat = ServeCatalog()
+    reqs = [cat.request(sp) for sp in specs]
+    ctx.synchronize()
+    t_setup = time.perf_counter() - t0
+    check_ids = set(np.random.default_rng(5).choice(n_req, size=min(args.check, n_req), replace=False).tolist())
+    budget = int(args.budget_gib * (1 << 30))
+
+    # capacity: a closed-loop burst of a seeded sample of the requests (also warms up the
+    # autotuner's shapes and the slot pools)
+    cal_ids = np.random.default_rng(6).choice(n_req, size=min(args.calibrate, n_req), replace=False).tolist()
+    server = ca.Server(slots=args.slots, memory_budget=budget)
+    tickets = [serve_submit(server, reqs[i]) for i in cal_ids]
+    times = []
+    for t in tickets:
+        times.append(server.wait(t))
+        server.release(t)
+    cal_span = max(d for _, _, d in times) - min(s for _, s, _ in times)
+    capacity = len(cal_ids) / cal_span
+    lam = args.load * capacity
+
+    # open loop: Poisson arrivals, a collector thread waits and releases in order
+    gaps = np.random.default_rng(7).exponential(1.0 / lam, size=n_req)
+    arrivals = np.cumsum(gaps)
+    results = {}
+    stamps = [None] * n_req
+    tickets = [None] * n_req
+    submitted = threading.Semaphore(0)
+    errors = []
+
+    def collector():
+        for i in range(n_req):
+            submitted.acquire()
+            try:
+                stamps[i] = server.wait(tickets[i])
+                if i in check_ids:
+                    results[i] = server.result(tickets[i]).collect()
+                server.release(tickets[i])
+            except Exception as e:  # reported in the JSON line
+                errors.append(f"request {i}: {e}")
+
+    th = threading.Thread(target=collector)
+    th.start()
+    base = server.now() + 0.05
+    for i in range(n_req):
+        delay = base + arrivals[i] - server.now()
+        if delay > 0:
+            time.sleep(delay)
+        tickets[i] = serve_submit(server, reqs[i])
+        submitted.release()
+    th.join()
+    server.close()
+
+    ok = [s for s in stamps if s is not None]
+    lat = np.array([d - a for a, _, d in ok])
+    queue = np.array([s - a for a, s, _ in ok])
+    span = max(d for _, _, d in ok) - min(a for a, _, _ in ok)
+    vox_tr = sum(sp["nv"] * sp["nTR"] for sp in specs)
+    kspace = sum(sp["nc"] * sp["nTR"] * SERVE_NS for sp in specs)
+
+    # the same requests alone: one slot, one request in flight at a time
+    alone = ca.Server(slots=1, memory_budget=budget)
+    worst = 0.0
+    for i in sorted(check_ids):
+        t = serve_submit(alone, reqs[i])
+        alone.wait(t)
+        out = alone.result(t).collect()
+        alone.release(t)
+        if i in results:
+            worst = max(worst, rel_l2(results[i], out))
+    alone.close()
+
+    counts = {k: sum(1 for sp in specs if sp["kind"] == k) for k in ("fisp", "pssfp", "jhy")}
+    line = {
+        "metric": "serving: aggregate voxel*TR/s and k-space samples/s with p50/p99 latency (arrival -> result in HBM)",
+        "value": vox_tr / span,
+        "unit": "voxel*TR/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32 (signal and J^H.y contractions BF16 hi+lo on MFMA)",
+        "data": "synthetic (seeded request mix; resident tissue maps, sequences, trajectories, coils)",
+        "config": {"workload": f"{n_req} requests, Poisson arrivals at {args.load:.0%} of measured capacity, "
+                               f"{args.slots} slots, budget {args.budget_gib:g} GiB",
+                   "mix": counts, "samples_per_readout": SERVE_NS},
+        "requests_per_s": len(ok) / span,
+        "kspace_samples_per_s": kspace / span,
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99)),
+                       "mean": 1e3 * float(lat.mean())},
+        "queue_ms": {"p50": 1e3 * float(np.percentile(queue, 50)), "p99": 1e3 * float(np.percentile(queue, 99))},
+        "arrival_rate_per_s": lam,
+        "capacity_per_s": capacity,
+        "calibration": f"{len(cal_ids)} requests submitted at once: {cal_span:.2f} s",
+        "alone_vs_mix_max_rel_l2": worst,
+        "alone_vs_mix_checked": len(check_ids),
+        "errors": errors[:5],
+        "setup_s": t_setup,
+    }
+    print(json.dumps(line))
+
+
+def rel_l2(x, ref):
+    x = np.asarray(x, dtype=np.complex128)
+    ref = np.asarray(ref, dtype=np.complex128)
+    den = np.linalg.norm(ref.ravel())
+    return float(np.linalg.norm((x - ref).ravel()) / (den if den > 0 else 1.0))
+
+
 def precision_study(args):
     """Contraction precisions side by side: FP32 VALU, BF16 hi+lo, FP16 hi+lo
     (per-row power-of-two scaling), BF16 x1 and FP16 x1.  For the 4-coil
@@ -546,6 +738,13 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3])
     ap.add_argument("--method", default="matmul_fast", choices=list(ca.SIGNAL_METHODS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve", action="store_true", help="serving loop with Poisson arrivals")
+    ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--calibrate", type=int, default=60)
+    ap.add_argument("--check", type=int, default=100)
+    ap.add_argument("--budget-gib", type=float, default=96.0)
     ap.add_argument("--products", action="store_true",
                     help="time signal + J.v + J^H.y at the 4-coil Cartesian shapes (--method picks the precision)")
     ap.add_argument("--precision-study", action="store_true",
@@ -561,6 +760,9 @@ def main():
         return
     if args.products:
         products_bench(args)
+        return
+    if args.serve:
+        serve_bench(args)
         return
     if args.workload == 3:
         spiral_bench(args)

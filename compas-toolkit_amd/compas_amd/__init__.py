@@ -85,6 +85,14 @@ _SIGNATURES = {
     "compas_set_contraction_precision": (None, [_i]),
     "compas_compute_jacobian_spiral": (_vp, [_vp] * 10),
     "compas_compute_jacobian_hermitian_spiral": (_vp, [_vp] * 10),
+    "compas_server_create": (_vp, [_vp, _i, C.c_int64]),
+    "compas_server_destroy": (None, [_vp]),
+    "compas_server_submit_simulate_signal": (C.c_int64, [_vp, _vp, _vp, _vp, _vp, _i]),
+    "compas_server_submit_jacobian_hermitian": (C.c_int64, [_vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "compas_server_wait": (_i, [_vp, C.c_int64, C.c_double, C.POINTER(C.c_double)]),
+    "compas_server_result": (_vp, [_vp, C.c_int64, _i]),
+    "compas_server_release": (None, [_vp, C.c_int64]),
+    "compas_server_now": (C.c_double, [_vp]),
 }
 
 
@@ -472,6 +480,63 @@ def compute_jacobian_hermitian_spiral(echos, dechos, parameters, trajectory, coi
     yy = as_array(y, np.complex64)
     return _wrap(lib().compas_compute_jacobian_hermitian_spiral(ctx.ptr, e.ptr, *[x.ptr for x in d], parameters.ptr,
                                                                 trajectory.ptr, c.ptr, yy.ptr))
+
+
+class Server:
+    """Request server of the library (continuous batching on one GPU): `slots`
+    requests run concurrently on private streams, admitted in arrival order
+    within `memory_budget` bytes of estimated device memory."""
+
+    def __init__(self, slots: int = 4, memory_budget: int = 64 << 30):
+        ctx = get_context()
+        self.ptr = _check(lib().compas_server_create(ctx.ptr, int(slots), int(memory_budget)))
+
+    def submit_simulate_signal(self, parameters, sequence, trajectory, coils, method="matmul_fast") -> int:
+        c = as_array(coils, np.complex64)
+        m = SIGNAL_METHODS[method] if isinstance(method, str) else int(method)
+        t = lib().compas_server_submit_simulate_signal(self.ptr, parameters.ptr, sequence.ptr, trajectory.ptr,
+                                                      c.ptr, m)
+        _check()
+        return int(t)
+
+    def submit_jacobian_hermitian(self, parameters, sequence, trajectory, coils, y, method="gemm_fast") -> int:
+        c = as_array(coils, np.complex64)
+        yy = as_array(y, np.complex64)
+        m = JACOBIAN_METHODS[method] if isinstance(method, str) else int(method)
+        t = lib().compas_server_submit_jacobian_hermitian(self.ptr, parameters.ptr, sequence.ptr, trajectory.ptr,
+                                                         c.ptr, yy.ptr, m)
+        _check()
+        return int(t)
+
+    def wait(self, ticket: int, timeout: float = 600.0):
+        """(arrival, start, done) seconds on the server clock, or None on timeout."""
+        times = (C.c_double * 3)()
+        rc = lib().compas_server_wait(self.ptr, int(ticket), float(timeout), times)
+        _check()
+        return (times[0], times[1], times[2]) if rc == 1 else None
+
+    def result(self, ticket: int, which: int = 1) -> CompasArray:
+        """which = 0: echos; 1: signal (simulate_signal) or J^H.y (jacobian_hermitian)."""
+        return _wrap(lib().compas_server_result(self.ptr, int(ticket), int(which)))
+
+    def release(self, ticket: int):
+        lib().compas_server_release(self.ptr, int(ticket))
+        _check()
+
+    def now(self) -> float:
+        return float(lib().compas_server_now(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            lib().compas_server_destroy(self.ptr)
+            self.ptr = None
+            _check()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def phase_encoding(echos, parameters, trajectory) -> CompasArray:

@@ -14,6 +14,7 @@
 
 #include "compas/compas.h"
 #include "compas/internal.h"
+#include "compas/serving.h"
 #include "compas_amd.h"
 
 // Provided by libjulia when loaded from CompasToolkit.jl; absent otherwise.
@@ -516,6 +517,121 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
             unwrap<TissueParameters>(parameters, "parameters"), unwrap<SpiralTrajectory>(trajectory, "trajectory"),
             array(coils, "coils").as<cfloat, 2>("coils"), array(vector, "vector").as<cfloat, 3>("vector")));
     });
+}
+
+// ---- request server
+
+namespace {
+
+struct ServerHandle {
+    std::unique_ptr<Server> server;
+};
+
+Server& server_of(CompasServerHandle* s) {
+    if (s == nullptr) throw CompasException("null server");
+    return *reinterpret_cast<ServerHandle*>(s)->server;
+}
+
+void request_objects(ServeRequest& r, const void* parameters, const void* sequence, const void* trajectory,
+                     const void* coils) {
+    r.parameters = unwrap<TissueParameters>(parameters, "parameters");
+    if (sequence == nullptr) throw CompasException("null handle passed as `sequence`");
+    const auto* sh = static_cast<const Handle*>(sequence);
+    if (const auto* f = dynamic_cast<const FISPSequence*>(sh)) {
+        r.fisp = std::make_shared<FISPSequence>(*f);
+    } else if (const auto* q = dynamic_cast<const pSSFPSequence*>(sh)) {
+        r.pssfp = std::make_shared<pSSFPSequence>(*q);
+    } else {
+        throw CompasException("invalid object for `sequence` (expecting a FISP or pSSFP sequence)");
+    }
+    if (trajectory == nullptr) throw CompasException("null handle passed as `trajectory`");
+    const auto* th = static_cast<const Handle*>(trajectory);
+    if (const auto* c = dynamic_cast<const CartesianTrajectory*>(th)) {
+        r.cartesian = std::make_shared<CartesianTrajectory>(*c);
+    } else if (const auto* sp = dynamic_cast<const SpiralTrajectory*>(th)) {
+        r.spiral = std::make_shared<SpiralTrajectory>(*sp);
+    } else {
+        throw CompasException("invalid object for `trajectory` (expecting a Cartesian or spiral trajectory)");
+    }
+    r.coils = array(coils, "coils").as<cfloat, 2>("coils");
+}
+
+}  // namespace
+
+CompasServerHandle* compas_server_create(const CompasContextHandle* ctx, int slots, int64_t memory_budget) {
+    return catch_exceptions([&]() -> CompasServerHandle* {
+        COMPAS_CHECK(memory_budget > 0);
+        auto* h = new ServerHandle {std::make_unique<Server>(context(ctx), slots, size_t(memory_budget))};
+        return reinterpret_cast<CompasServerHandle*>(h);
+    });
+}
+
+void compas_server_destroy(CompasServerHandle* server) {
+    catch_exceptions([&] { delete reinterpret_cast<ServerHandle*>(server); });
+}
+
+int64_t compas_server_submit_simulate_signal(
+    CompasServerHandle* server, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, int method) {
+    const int64_t t = catch_exceptions([&]() -> int64_t {
+        ServeRequest r;
+        r.kind = RequestKind::SimulateSignal;
+        request_objects(r, parameters, sequence, trajectory, coils);
+        r.signal_method = signal_method(method);
+        return server_of(server).submit(std::move(r));
+    });
+    return g_last_error.empty() ? t : -1;
+}
+
+int64_t compas_server_submit_jacobian_hermitian(
+    CompasServerHandle* server, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* residual,
+    int method) {
+    const int64_t t = catch_exceptions([&]() -> int64_t {
+        ServeRequest r;
+        r.kind = RequestKind::JacobianHermitian;
+        request_objects(r, parameters, sequence, trajectory, coils);
+        r.residual = array(residual, "residual").as<cfloat, 3>("residual");
+        r.jacobian_method = jacobian_method(method);
+        return server_of(server).submit(std::move(r));
+    });
+    return g_last_error.empty() ? t : -1;
+}
+
+int compas_server_wait(CompasServerHandle* server, int64_t ticket, double timeout_seconds, double* times) {
+    const int rc = catch_exceptions([&]() -> int {
+        Server& s = server_of(server);
+        if (!s.wait(ticket, timeout_seconds)) return 0;
+        const ServeResult r = s.result(ticket);
+        if (times) {
+            times[0] = r.t_arrival;
+            times[1] = r.t_start;
+            times[2] = r.t_done;
+        }
+        if (!r.error.empty()) throw CompasException("request failed: " + r.error);
+        return 1;
+    });
+    return g_last_error.empty() ? rc : -1;
+}
+
+CompasArrayHandle* compas_server_result(CompasServerHandle* server, int64_t ticket, int which) {
+    return catch_exceptions([&]() -> CompasArrayHandle* {
+        const ServeResult r = server_of(server).result(ticket);
+        if (which == 0) return new_array(r.echos);
+        if (which == 1) {
+            if (r.signal.buffer()) return new_array(r.signal);
+            return new_array(r.jhy);
+        }
+        throw CompasException("`which` must be 0 (echos) or 1 (signal or J^H.y)");
+    });
+}
+
+void compas_server_release(CompasServerHandle* server, int64_t ticket) {
+    catch_exceptions([&] { server_of(server).release(ticket); });
+}
+
+double compas_server_now(CompasServerHandle* server) {
+    return catch_exceptions([&] { return server_of(server).now(); });
 }
 
 void compas_set_kernel_timing(const CompasContextHandle* ctx, int enabled) {

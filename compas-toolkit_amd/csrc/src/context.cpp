@@ -1,5 +1,6 @@
 #include "compas/core/context.h"
 
+#include <set>
 #include <sstream>
 
 namespace compas {
@@ -19,14 +20,41 @@ void check_launch(const char* name) {
     }
 }
 
+// Every live runtime, so that an allocation failing in one of them can
+// reclaim the blocks cached by the others (several runtimes per device exist
+// when a request server runs one per slot).
+static std::mutex g_live_mutex;
+static std::set<Runtime*>& live_runtimes() {
+    static auto* s = new std::set<Runtime*>();  // never destroyed
+    return *s;
+}
+
 Runtime::Runtime(int device) : device_(device) {
     COMPAS_HIP_CHECK(hipSetDevice(device));
     // default priority on purpose: streams of different priorities are not mixed
     COMPAS_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    std::lock_guard<std::mutex> lock(g_live_mutex);
+    live_runtimes().insert(this);
 }
 
 Runtime::~Runtime() {
-    // Runtimes live until process exit; the driver reclaims memory then.
+    // The per-device runtimes of the registry live until process exit.  A
+    // private runtime (a server slot) ends when its last buffer is released:
+    // all its blocks are cached by then, so give them back with its stream.
+    {
+        std::lock_guard<std::mutex> lock(g_live_mutex);
+        live_runtimes().erase(this);
+    }
+    (void)hipSetDevice(device_);
+    (void)hipStreamSynchronize(stream_);
+    for (auto& kv : free_blocks_) (void)hipFree(kv.second);
+    free_blocks_.clear();
+    for (auto& p : pending_) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (auto e : event_pool_) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(stream_);
 }
 
 void Runtime::activate() const {
@@ -61,10 +89,22 @@ void* Runtime::allocate(size_t nbytes) {
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
-        // out of memory: give the cache back and retry once
+        // out of memory: give the cache back and retry, then the caches of the
+        // other runtimes on this device
         (void)hipGetLastError();
         trim();
         e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            {
+                // holding the registry lock keeps the others alive while they trim
+                std::lock_guard<std::mutex> lock(g_live_mutex);
+                for (Runtime* r : live_runtimes())
+                    if (r != this && r->device_ == device_) r->trim();
+            }
+            activate();
+            e = hipMalloc(&p, n);
+        }
     }
     if (e != hipSuccess) {
         std::stringstream ss;

@@ -27,6 +27,7 @@ extern "C" {
 typedef struct CompasContextHandle CompasContextHandle; /* compas::CompasContext */
 typedef struct CompasObjectHandle CompasObjectHandle;   /* parameters, sequences, trajectories */
 typedef struct CompasArrayHandle CompasArrayHandle;     /* device array */
+typedef struct CompasServerHandle CompasServerHandle;   /* request server (extension) */
 
 typedef struct {
     float re;
@@ -174,6 +175,37 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
     const CompasArrayHandle* delta_echos_T2, const CompasArrayHandle* delta_echos_B1,
     const CompasArrayHandle* delta_echos_B0, const CompasObjectHandle* parameters,
     const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* vector);
+
+/* Request server (no counterpart in the reference, whose host is a Julia
+ * session calling one operation at a time): continuous batching on one GPU.
+ * `slots` requests run concurrently, each on a private stream and memory
+ * pool; pending requests are admitted in arrival order as soon as a slot is
+ * free and the estimated device memory of the running requests fits
+ * `memory_budget` bytes.  Requests reference resident objects (parameters,
+ * sequence, trajectory, coils, residual handles).  Tickets are >= 0; -1 means
+ * the submission failed (compas_last_error).
+ *   simulate_signal: simulate_magnetization + magnetization_to_signal
+ *     (FISP or pSSFP, Cartesian or spiral), method = SimulateSignalMethod;
+ *   jacobian_hermitian: simulation, finite-difference derivatives and J^H.y
+ *     (Cartesian: 4 parameters; spiral: 6), method = JacobianComputeMethod.
+ * compas_server_wait returns 1 when done (times = arrival, start, done in
+ * seconds on the server clock), 0 on timeout, -1 if the request failed.
+ * compas_server_result: which = 0 echos, 1 signal or J^H.y (new handle);
+ * results stay in HBM until compas_server_release.  Destroying the server
+ * waits for the running and pending requests. */
+CompasServerHandle* compas_server_create(const CompasContextHandle* ctx, int slots, int64_t memory_budget);
+void compas_server_destroy(CompasServerHandle* server);
+int64_t compas_server_submit_simulate_signal(
+    CompasServerHandle* server, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, int method);
+int64_t compas_server_submit_jacobian_hermitian(
+    CompasServerHandle* server, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* residual,
+    int method);
+int compas_server_wait(CompasServerHandle* server, int64_t ticket, double timeout_seconds, double* times);
+CompasArrayHandle* compas_server_result(CompasServerHandle* server, int64_t ticket, int which);
+void compas_server_release(CompasServerHandle* server, int64_t ticket);
+double compas_server_now(CompasServerHandle* server);
 
 /* Per-kernel timing with HIP events on the context's stream.  While enabled,
  * every library launch is bracketed by events; compas_read_kernel_timings
