@@ -6,11 +6,16 @@ readouts x 224 samples -- one "step" is simulate_magnetization followed by
 magnetization_to_signal with all inputs resident in HBM, called through the
 C-ABI of libcompas_amd.so.
 
-Prints one JSON line (rank 0).  `value` is voxel.TR/s of the whole step;
-k-space samples/s, per-step latency percentiles, per-kernel times (HIP events
-on the library's stream), the roofline of the dominant kernel and the CPU
-baseline (the FP32 port of the reference algorithm on the host cores, on a
-bounded sample) are reported beside it.
+Prints one JSON line (rank 0).  The K timed steps are independent requests
+run through the library's request server on `--pipeline` slots (separate HIP
+streams), so the VALU-bound EPG simulation of one step overlaps the MFMA
+contraction of another; `value` is voxel.TR/s over those K steps.  The same K
+steps are also timed one after another on one stream (`sequential`), which
+gives the per-step latency percentiles and kernel durations without overlap.
+k-space samples/s, per-kernel times (HIP events on the stream each kernel is
+launched on), the roofline of the dominant kernel and the CPU baseline (the
+FP32 port of the reference algorithm on the host cores, on a bounded sample)
+are reported beside it.
 """
 
 from __future__ import annotations
@@ -738,6 +743,8 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3])
     ap.add_argument("--method", default="matmul_fast", choices=list(ca.SIGNAL_METHODS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pipeline", type=int, default=3,
+                    help="request-server slots the timed steps are spread over (1 = one stream, no overlap)")
     ap.add_argument("--serve", action="store_true", help="serving loop with Poisson arrivals")
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--slots", type=int, default=4)
@@ -791,7 +798,23 @@ def main():
     ctx.synchronize()
     del out
 
-    # timed region: HIP events around every launch on the library's stream
+    def kernel_table(timings):
+        return {k: {"launches": c, "avg_ms": tot / max(c, 1), "total_ms": tot} for k, (c, tot) in timings.items()}
+
+    def roofline_of(kernels):
+        dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
+        if dominant is None:
+            return None
+        work = algorithmic_work(dominant, wl, args.method)
+        if work is None:
+            return None
+        amount, unit, bound, peak = work
+        achieved = amount / (kernels[dominant]["avg_ms"] / 1e3)
+        return {"kernel": dominant, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+                "frac": achieved / peak, "traffic": measured_traffic(dominant, args.workload)}
+
+    # ---- sequential steps on the library's stream: per-step latency of one step on an
+    # otherwise idle GPU, and kernel durations without overlap (HIP events per launch)
     ctx.read_kernel_timings()  # reset
     ctx.set_kernel_timing(True)
     lat = []
@@ -802,31 +825,47 @@ def main():
         out = step()
         ctx.synchronize()
         lat.append(time.perf_counter() - t0)
-    t_total = time.perf_counter() - t_start
+    t_seq = time.perf_counter() - t_start
     ctx.set_kernel_timing(False)
-    timings = ctx.read_kernel_timings()
+    kernels_seq = kernel_table(ctx.read_kernel_timings())
+    sequential = {"ms_per_step": 1e3 * t_seq / args.steps, "value": nv * nTR * args.steps / t_seq,
+                  "kernels": kernels_seq, "roofline": roofline_of(kernels_seq)}
+
+    # ---- the same K steps as independent requests on the request server: consecutive
+    # steps overlap on the device (the VALU-bound EPG of one step beside the MFMA
+    # contraction of another).  This is the timed region of `value`.
+    pipeline = None
+    t_total, kernels, roofline = t_seq, kernels_seq, sequential["roofline"]
+    if args.pipeline > 1:
+        srv = ca.Server(slots=args.pipeline, memory_budget=64 << 30)
+        for _ in range(max(args.warmup, 2 * args.pipeline)):
+            t = srv.submit_simulate_signal(params, seq, traj, coils, method=args.method)
+            srv.wait(t)
+            srv.release(t)
+        srv.read_kernel_timings()  # reset
+        srv.set_kernel_timing(True)
+        t_start = time.perf_counter()
+        tickets = [srv.submit_simulate_signal(params, seq, traj, coils, method=args.method)
+                   for _ in range(args.steps)]
+        stamps = [srv.wait(t) for t in tickets]
+        t_total = time.perf_counter() - t_start
+        srv.set_kernel_timing(False)
+        kernels = kernel_table(srv.read_kernel_timings())
+        for t in tickets:
+            srv.release(t)
+        srv.close()
+        roofline = roofline_of(kernels)
+        service = [d - st for _, st, d in stamps]
+        pipeline = {"slots": args.pipeline, "ms_per_step": 1e3 * t_total / args.steps,
+                    "service_ms": {"p50": 1e3 * percentile(service, 50), "p99": 1e3 * percentile(service, 99)}}
 
     ms_per_step = 1e3 * t_total / args.steps
     value = nv * nTR * args.steps / t_total
-    kernels = {k: {"launches": c, "avg_ms": tot / max(c, 1), "total_ms": tot} for k, (c, tot) in timings.items()}
-    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
-
-    roofline = None
-    if dominant is not None:
-        work = algorithmic_work(dominant, wl, args.method)
-        if work is not None:
-            amount, unit, bound, peak = work
-            avg_s = kernels[dominant]["avg_ms"] / 1e3
-            achieved = amount / avg_s
-            roofline = {"kernel": dominant, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                        "frac": achieved / peak, "traffic": None}
 
     # plain streaming copy of known size: achievable HBM bandwidth, and the
     # reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters to bytes
     probe_bytes = 1 << 30
     probe_ms = ctx.bandwidth_probe(probe_bytes)
-    if roofline is not None:
-        roofline["traffic"] = measured_traffic(roofline["kernel"], args.workload)
 
     echos, signal = out
     line = {
@@ -848,7 +887,10 @@ def main():
             "signal_method": args.method,
         },
         "kspace_samples_per_s": nc * nr * ns * args.steps / t_total,
-        "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99)},
+        "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99),
+                       "note": "one step alone on an idle GPU (sequential phase)"},
+        "pipeline": pipeline,
+        "sequential": sequential,
         "kernels": kernels,
         "hbm_copy_GBs": 2 * probe_bytes / (probe_ms * 1e-3) / 1e9,
         "contraction_tuning": ca.contraction_tuning_report().strip().splitlines(),

@@ -93,6 +93,8 @@ _SIGNATURES = {
     "compas_server_result": (_vp, [_vp, C.c_int64, _i]),
     "compas_server_release": (None, [_vp, C.c_int64]),
     "compas_server_now": (C.c_double, [_vp]),
+    "compas_server_set_kernel_timing": (None, [_vp, _i]),
+    "compas_server_read_kernel_timings": (_i, [_vp, C.c_char_p, _i]),
 }
 
 
@@ -150,6 +152,19 @@ def _check(result=None):
     return result
 
 
+def _parse_timings(read) -> dict:
+    buf = C.create_string_buffer(1 << 20)  # one call: reading resets the counters
+    n = read(buf)
+    _check()
+    if n > len(buf):
+        raise CompasError("kernel timing report truncated")
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, count, total = line.rsplit(" ", 2)
+        out[name] = (int(count), float(total))
+    return out
+
+
 def version() -> str:
     return lib().compas_version().decode()
 
@@ -179,16 +194,7 @@ class Context:
 
     def read_kernel_timings(self) -> dict:
         """Per-kernel (launches, total ms) since the last read; resets the counters."""
-        buf = C.create_string_buffer(1 << 20)  # one call: reading resets the counters
-        n = lib().compas_read_kernel_timings(self.ptr, buf, len(buf))
-        _check()
-        if n > len(buf):
-            raise CompasError("kernel timing report truncated")
-        out = {}
-        for line in buf.value.decode().splitlines():
-            name, count, total = line.rsplit(" ", 2)
-            out[name] = (int(count), float(total))
-        return out
+        return _parse_timings(lambda buf: lib().compas_read_kernel_timings(self.ptr, buf, len(buf)))
 
     def trim_memory(self):
         lib().compas_trim_memory(self.ptr)
@@ -525,6 +531,14 @@ class Server:
 
     def now(self) -> float:
         return float(lib().compas_server_now(self.ptr))
+
+    def set_kernel_timing(self, enabled: bool):
+        lib().compas_server_set_kernel_timing(self.ptr, int(enabled))
+        _check()
+
+    def read_kernel_timings(self) -> dict:
+        """Per-kernel (launches, total ms) over all slots since the last read."""
+        return _parse_timings(lambda buf: lib().compas_server_read_kernel_timings(self.ptr, buf, len(buf)))
 
     def close(self):
         if self.ptr:

@@ -66,6 +66,10 @@ class Server {
     ServeResult result(int64_t ticket);
     void release(int64_t ticket);
 
+    // Per-kernel HIP-event timing on every slot stream (merged when read).
+    void set_kernel_timing(bool on);
+    std::map<std::string, KernelStats> collect_timings();
+
     static size_t estimate_bytes(const ServeRequest& r);
     double now() const;
     int slots() const { return int(slots_.size()); }

@@ -178,6 +178,24 @@ JacobianComputeMethod jacobian_method(int m) {
     return JacobianComputeMethod(m);
 }
 
+// "name count total_ms" lines into buf (NUL-terminated, truncated); returns bytes needed
+int format_timings(const std::map<std::string, KernelStats>& stats, char* buf, int buflen) {
+    std::stringstream ss;
+    for (auto& kv : stats) {
+        char line[256];
+        std::snprintf(line, sizeof(line), "%s %lld %.6f\n", kv.first.c_str(), (long long)kv.second.count,
+                      kv.second.total_ms);
+        ss << line;
+    }
+    const std::string s = ss.str();
+    if (buf && buflen > 0) {
+        const size_t n = std::min(size_t(buflen - 1), s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return int(s.size() + 1);
+}
+
 }  // namespace
 
 extern "C" {
@@ -630,6 +648,14 @@ void compas_server_release(CompasServerHandle* server, int64_t ticket) {
     catch_exceptions([&] { server_of(server).release(ticket); });
 }
 
+void compas_server_set_kernel_timing(CompasServerHandle* server, int enabled) {
+    catch_exceptions([&] { server_of(server).set_kernel_timing(enabled != 0); });
+}
+
+int compas_server_read_kernel_timings(CompasServerHandle* server, char* buf, int buflen) {
+    return catch_exceptions([&] { return format_timings(server_of(server).collect_timings(), buf, buflen); });
+}
+
 double compas_server_now(CompasServerHandle* server) {
     return catch_exceptions([&] { return server_of(server).now(); });
 }
@@ -639,23 +665,7 @@ void compas_set_kernel_timing(const CompasContextHandle* ctx, int enabled) {
 }
 
 int compas_read_kernel_timings(const CompasContextHandle* ctx, char* buf, int buflen) {
-    return catch_exceptions([&] {
-        auto stats = context(ctx).runtime()->collect_timings();
-        std::stringstream ss;
-        for (auto& kv : stats) {
-            char line[256];
-            std::snprintf(line, sizeof(line), "%s %lld %.6f\n", kv.first.c_str(), (long long)kv.second.count,
-                          kv.second.total_ms);
-            ss << line;
-        }
-        std::string s = ss.str();
-        if (buf && buflen > 0) {
-            size_t n = std::min(size_t(buflen - 1), s.size());
-            std::memcpy(buf, s.data(), n);
-            buf[n] = '\0';
-        }
-        return int(s.size() + 1);
-    });
+    return catch_exceptions([&] { return format_timings(context(ctx).runtime()->collect_timings(), buf, buflen); });
 }
 
 void compas_set_contraction_variant(int variant) {

@@ -186,6 +186,22 @@ void Server::dispatch_loop() {
     }
 }
 
+void Server::set_kernel_timing(bool on) {
+    for (auto& c : slots_) c.runtime()->set_timing(on);
+}
+
+std::map<std::string, KernelStats> Server::collect_timings() {
+    std::map<std::string, KernelStats> out;
+    for (auto& c : slots_) {
+        for (auto& kv : c.runtime()->collect_timings()) {
+            auto& o = out[kv.first];
+            o.total_ms += kv.second.total_ms;
+            o.count += kv.second.count;
+        }
+    }
+    return out;
+}
+
 bool Server::wait(int64_t ticket, double timeout_seconds) {
     std::unique_lock<std::mutex> lock(mutex_);
     auto it = entries_.find(ticket);

@@ -206,6 +206,10 @@ int compas_server_wait(CompasServerHandle* server, int64_t ticket, double timeou
 CompasArrayHandle* compas_server_result(CompasServerHandle* server, int64_t ticket, int which);
 void compas_server_release(CompasServerHandle* server, int64_t ticket);
 double compas_server_now(CompasServerHandle* server);
+/* Per-kernel HIP-event timing on every slot stream of the server; the read
+ * has the format of compas_read_kernel_timings (merged over the slots). */
+void compas_server_set_kernel_timing(CompasServerHandle* server, int enabled);
+int compas_server_read_kernel_timings(CompasServerHandle* server, char* buf, int buflen);
 
 /* Per-kernel timing with HIP events on the context's stream.  While enabled,
  * every library launch is bracketed by events; compas_read_kernel_timings
