@@ -3,10 +3,14 @@
 //
 // Complex-to-real mapping.  With A stored interleaved (re, im) along K, row
 // m of A is a real vector of length 2K.  For each complex column n the real
-// GEMM uses two B rows built while staging into LDS:
+// GEMM uses two B rows
 //   row 2n   = (b_re, -b_im) pairs  ->  D[m][2n]   = Re(sum a b)
 //   row 2n+1 = (b_im,  b_re) pairs  ->  D[m][2n+1] = Im(sum a b)
-// so one real GEMM (M x 2N x 2K) produces interleaved complex output.
+// so one real GEMM (M x 2N x 2K) produces interleaved complex output.  LDS
+// holds each complex column once, as (b_re, b_im) pairs; the lanes feeding
+// real column 2n and 2n+1 read the same LDS row (a broadcast) and form their
+// row in registers with one v_perm_b32 (half swap) and one v_xor_b32 (sign
+// of the imaginary half) per packed word.
 //
 // Precision.  Every FP32 operand x is split into hi = round(x) and
 // lo = round(x - hi) in BF16 (or FP16 after a per-row power-of-two scale that
@@ -110,7 +114,6 @@ template<int P, int WM, int WN, int BKC>
 struct MfmaTile {
     static constexpr int THREADS = 64 * WM * WN;
     static constexpr int BM = 64 * WM;  // A rows
-    static constexpr int BNR = 64 * WN;  // real B rows (2 per complex column)
     static constexpr int BNC = 32 * WN;  // complex B columns
     static constexpr int LD = 2 * BKC + 8;  // LDS row stride, 16-bit elements (+16 bytes of padding)
     static constexpr bool LO = has_lo(P);
@@ -118,7 +121,7 @@ struct MfmaTile {
     static constexpr int A_TPR = WN;  // threads per A row
     static constexpr int B_PER = BKC / (2 * WM);
     static constexpr int B_TPR = 2 * WM;
-    static constexpr int LDS_BYTES = (BM + BNR) * LD * 2 * (LO ? 2 : 1);
+    static constexpr int LDS_BYTES = (BM + BNC) * LD * 2 * (LO ? 2 : 1);
     static_assert(A_PER >= 4 && A_PER % 4 == 0, "A staging granularity");
     static_assert(B_PER >= 2 && B_PER % 2 == 0, "B staging granularity");
 };
@@ -129,9 +132,9 @@ struct MfmaSmem {
     using Tl = MfmaTile<P, WM, WN, BKC>;
     using T = typename Half16<P>::T;
     __attribute__((aligned(16))) T a_hi[Tl::BM * Tl::LD];
-    __attribute__((aligned(16))) T b_hi[Tl::BNR * Tl::LD];
+    __attribute__((aligned(16))) T b_hi[Tl::BNC * Tl::LD];  // one row per complex column
     __attribute__((aligned(16))) T a_lo[Tl::LO ? Tl::BM * Tl::LD : 8];
-    __attribute__((aligned(16))) T b_lo[Tl::LO ? Tl::BNR * Tl::LD : 8];
+    __attribute__((aligned(16))) T b_lo[Tl::LO ? Tl::BNC * Tl::LD : 8];
 };
 
 template<int N>
@@ -144,6 +147,17 @@ __device__ inline void store_words(uint32_t* dst, const uint32_t* w) {
 #pragma unroll
         for (int i = 0; i < N / 2; i++) reinterpret_cast<uint2*>(dst)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
     }
+}
+
+// The B operand row of a lane from the packed (re, im) words of its complex
+// column: half swap by byte permutation, then the sign flip.
+template<typename V>
+__device__ inline V b_row_form_t(uint4 w, uint32_t sel, uint32_t flip) {
+    w.x = __builtin_amdgcn_perm(w.x, w.x, sel) ^ flip;
+    w.y = __builtin_amdgcn_perm(w.y, w.y, sel) ^ flip;
+    w.z = __builtin_amdgcn_perm(w.z, w.z, sel) ^ flip;
+    w.w = __builtin_amdgcn_perm(w.w, w.w, sel) ^ flip;
+    return __builtin_bit_cast(V, w);
 }
 
 // Row of the A tile that thread `tid` stages, and row of the B tile (complex
@@ -179,6 +193,9 @@ __device__ inline void mfma_accumulate(
     const int wm = wave / WN, wn = wave % WN;
     const int a_row = tid / Tl::A_TPR, a_k = (tid % Tl::A_TPR) * Tl::A_PER;
     const int b_row = tid / Tl::B_TPR, b_k = (tid % Tl::B_TPR) * Tl::B_PER;
+    // even lanes feed Re columns (b_re, -b_im), odd lanes Im columns (b_im, b_re)
+    const uint32_t sel = (lane & 1) ? 0x01000302u : 0x03020100u;
+    const uint32_t flip = (lane & 1) ? 0u : 0x80000000u;
 
     // register prefetch of one K step
     cfloat ra[Tl::A_PER], rb[Tl::B_PER];
@@ -219,24 +236,13 @@ __device__ inline void mfma_accumulate(
             store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_hi + a_row * LD + 2 * a_k), h);
             if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_lo + a_row * LD + 2 * a_k), l);
         }
-        // ---- stage B: conj row (re, -im) and swap row (im, re)
+        // ---- stage B: (re, im) pairs, one row per complex column
         {
-            uint32_t ch[Tl::B_PER], sh[Tl::B_PER], cl[Tl::B_PER], sl[Tl::B_PER];
+            uint32_t h[Tl::B_PER], l[Tl::B_PER];
 #pragma unroll
-            for (int i = 0; i < Tl::B_PER; i++) {
-                uint32_t h, l;
-                split_pair<P>(rb[i].re * sb, rb[i].im * sb, h, l);
-                ch[i] = h ^ 0x80000000u;  // negate the imaginary half
-                sh[i] = __builtin_amdgcn_alignbit(h, h, 16);  // swap halves
-                cl[i] = l ^ 0x80000000u;
-                sl[i] = __builtin_amdgcn_alignbit(l, l, 16);
-            }
-            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_hi + (2 * b_row) * LD + 2 * b_k), ch);
-            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_hi + (2 * b_row + 1) * LD + 2 * b_k), sh);
-            if constexpr (LO) {
-                store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_lo + (2 * b_row) * LD + 2 * b_k), cl);
-                store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_lo + (2 * b_row + 1) * LD + 2 * b_k), sl);
-            }
+            for (int i = 0; i < Tl::B_PER; i++) split_pair<P>(rb[i].re * sb, rb[i].im * sb, h[i], l[i]);
+            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_hi + b_row * LD + 2 * b_k), h);
+            if constexpr (LO) store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_lo + b_row * LD + 2 * b_k), l);
         }
         __syncthreads();
         if (k0 + BKC < ke) gload(k0 + BKC);  // in flight during the MFMAs below
@@ -253,9 +259,9 @@ __device__ inline void mfma_accumulate(
             }
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                const int c = wn * 64 + j * 32 + (lane & 31);
-                bh[j] = *reinterpret_cast<const V*>(sm.b_hi + c * LD + koff);
-                if constexpr (LO) bl[j] = *reinterpret_cast<const V*>(sm.b_lo + c * LD + koff);
+                const int c = wn * 32 + j * 16 + (lane & 31) / 2;  // complex column of this lane
+                bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_hi + c * LD + koff), sel, flip);
+                if constexpr (LO) bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_lo + c * LD + koff), sel, flip);
             }
 #pragma unroll
             for (int i = 0; i < 2; i++)

@@ -65,11 +65,110 @@ __global__ void spiral_jv_amplitudes(
 
 constexpr int JHS_THREADS = 256;  // voxels per workgroup (one per lane)
 constexpr int JHS_ANCHOR = 32;  // samples between FP64-anchored evaluations of Phi
-constexpr int JHS_COILS = 4;  // coils per pass over the samples
+constexpr int JHS_BLOCK = 8;  // samples per unrolled block (y of the block loaded up front)
+
+struct SpiralVoxel {
+    float R2, decay, T1, T2;
+    double B0, x, y, half;
+    cfloat rho;
+};
+
+// Adds the coils [cg, cg + CG) of one voxel to out[6]: per readout r in
+// [r0, r1), U = sum_s y conj(Phi) and U' = sum_s (s - ns/2) y conj(Phi), then
+//   out += conj(rho T1 C) sum_r conj(dm_T1) U, conj(rho T2 C) sum_r
+//   (conj(dm_T2) U + dt R2^2 conj(m) U'), conj(rho C) sum_r conj(dm_B1) U,
+//   conj(rho C) sum_r (conj(dm_B0) U - i 2 pi dt conj(m) U'), conj(C) sum_r conj(m) U.
+// y is wave-uniform: the loads of a sample block become scalar loads issued
+// before the block's arithmetic.
+template<int CG>
+__device__ inline void spiral_hermitian_coils(
+    const SpiralVoxel& sv, int vv, int nv, const cfloat* __restrict__ m, const cfloat* __restrict__ dT1,
+    const cfloat* __restrict__ dT2, const cfloat* __restrict__ dB1, const cfloat* __restrict__ dB0,
+    const cfloat* __restrict__ coils, int cg, const cfloat* __restrict__ y, const cfloat* __restrict__ k_start,
+    const cfloat* __restrict__ delta_k, int nr, int ns, float dt, int r0, int r1, cfloat (&out)[6]) {
+    cfloat g1[CG], g2[CG], gb1[CG], gb0[CG], gu[CG];
+#pragma unroll
+    for (int c = 0; c < CG; c++) g1[c] = g2[c] = gb1[c] = gb0[c] = gu[c] = {0, 0};
+    const size_t coil_stride = size_t(nr) * ns;
+
+    for (int r = r0; r < r1; r++) {
+        const cfloat dk = delta_k[r], k0 = k_start[r];
+        const double theta = double(dk.re) * sv.x + double(dk.im) * sv.y + double(dt) * dev::TWO_PI * sv.B0;
+        const double phi0 = -sv.half * dev::TWO_PI * sv.B0 + double(k0.re) * sv.x;
+        const cfloat step = dev::polar_f(expf(sv.decay), dev::reduce_phase(theta));
+        cfloat U[CG], Up[CG];
+#pragma unroll
+        for (int c = 0; c < CG; c++) U[c] = Up[c] = {0, 0};
+        const cfloat* yr = y + (size_t(cg) * nr + r) * ns;
+        for (int sa = 0; sa < ns; sa += JHS_ANCHOR) {
+            // Phi at the anchor sample: magnitude exp((ns/2 - s) dt R2), phase phi0 + s theta
+            cfloat ph = dev::polar_f(expf((0.5f * float(ns) - float(sa)) * dt * sv.R2),
+                                     dev::reduce_phase(phi0 + double(sa) * theta));
+            const int se = min(ns, sa + JHS_ANCHOR);
+            int s = sa;
+            for (; s + JHS_BLOCK <= se; s += JHS_BLOCK) {
+                cfloat yb[CG][JHS_BLOCK];
+#pragma unroll
+                for (int c = 0; c < CG; c++)
+#pragma unroll
+                    for (int j = 0; j < JHS_BLOCK; j++) yb[c][j] = yr[size_t(c) * coil_stride + s + j];
+#pragma unroll
+                for (int j = 0; j < JHS_BLOCK; j++) {
+                    const float sp = float(s + j) - 0.5f * float(ns);
+#pragma unroll
+                    for (int c = 0; c < CG; c++) {
+                        const cfloat yv = yb[c][j];
+                        const cfloat q {fmaf(yv.re, ph.re, yv.im * ph.im), fmaf(yv.im, ph.re, -yv.re * ph.im)};
+                        U[c].re += q.re;
+                        U[c].im += q.im;
+                        Up[c].re = fmaf(sp, q.re, Up[c].re);
+                        Up[c].im = fmaf(sp, q.im, Up[c].im);
+                    }
+                    ph = dev::cmul(ph, step);
+                }
+            }
+            for (; s < se; s++) {
+                const float sp = float(s) - 0.5f * float(ns);
+#pragma unroll
+                for (int c = 0; c < CG; c++) {
+                    const cfloat yv = yr[size_t(c) * coil_stride + s];
+                    const cfloat q {fmaf(yv.re, ph.re, yv.im * ph.im), fmaf(yv.im, ph.re, -yv.re * ph.im)};
+                    U[c].re += q.re;
+                    U[c].im += q.im;
+                    Up[c].re = fmaf(sp, q.re, Up[c].re);
+                    Up[c].im = fmaf(sp, q.im, Up[c].im);
+                }
+                ph = dev::cmul(ph, step);
+            }
+        }
+        const size_t i = size_t(r) * nv + vv;
+        const cfloat me = m[i], a1 = dT1[i], a2 = dT2[i], ab1 = dB1[i], ab0 = dB0[i];
+#pragma unroll
+        for (int c = 0; c < CG; c++) {
+            g1[c] = dev::cfma_conj(U[c], a1, g1[c]);
+            g2[c] = dev::cfma_conj(U[c], a2, g2[c]);
+            g2[c] = dev::cfma_conj(Up[c] * (dt * sv.R2 * sv.R2), me, g2[c]);
+            gb1[c] = dev::cfma_conj(U[c], ab1, gb1[c]);
+            gb0[c] = dev::cfma_conj(U[c], ab0, gb0[c]);
+            gb0[c] = dev::cfma_conj(dev::cmul(Up[c], cfloat {0.0f, -dev::TWO_PI_F * dt}), me, gb0[c]);
+            gu[c] = dev::cfma_conj(U[c], me, gu[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CG; c++) {
+        const cfloat C = coils[size_t(cg + c) * nv + vv];
+        const cfloat rc = dev::cmul(sv.rho, C);
+        out[0] = out[0] + dev::cmul(conj(rc * sv.T1), g1[c]);
+        out[1] = out[1] + dev::cmul(conj(rc * sv.T2), g2[c]);
+        out[2] = out[2] + dev::cmul(conj(rc), gb1[c]);
+        out[3] = out[3] + dev::cmul(conj(rc), gb0[c]);
+        out[4] = out[4] + dev::cmul(conj(C), gu[c]);
+    }
+}
 
 // Partial J^H.y over readouts [r0, r1) for one voxel per lane:
-// partial[chunk][p][v], p over the six parameters.  y[c][r][s] is uniform
-// across the lanes (scalar loads).
+// partial[chunk][p][v], p over the six parameters.  Coils run in groups of
+// four, then one group of the remaining one to three.
 __global__ void __launch_bounds__(JHS_THREADS) spiral_hermitian(
     dev::TissueView t, const cfloat* __restrict__ m, const cfloat* __restrict__ dT1,
     const cfloat* __restrict__ dT2, const cfloat* __restrict__ dB1, const cfloat* __restrict__ dB0,
@@ -82,79 +181,39 @@ __global__ void __launch_bounds__(JHS_THREADS) spiral_hermitian(
     const bool ok = v < nv;
     const int vv = ok ? v : 0;  // out-of-range lanes compute on voxel 0 and store nothing
 
-    const float R2 = 1.0f / t.T2(vv);
-    const double B0 = t.B0(vv), x = t.x(vv), yy = t.y(vv);
-    const double half = 0.5 * double(ns) * double(dt);
-    const float decay = -dt * R2;
-    const cfloat rho = t.rho(vv);
-    const float T1 = t.T1(vv), T2 = t.T2(vv);
+    SpiralVoxel sv;
+    sv.T1 = t.T1(vv);
+    sv.T2 = t.T2(vv);
+    sv.R2 = 1.0f / sv.T2;
+    sv.decay = -dt * sv.R2;
+    sv.B0 = t.B0(vv);
+    sv.x = t.x(vv);
+    sv.y = t.y(vv);
+    sv.half = 0.5 * double(ns) * double(dt);
+    sv.rho = t.rho(vv);
 
     cfloat out[6];
 #pragma unroll
     for (int p = 0; p < 6; p++) out[p] = {0, 0};
-
-    for (int cg = 0; cg < nc; cg += JHS_COILS) {
-        // per coil: sum_r conj(dm_T1) U, conj(dm_T2) U + dt R2^2 conj(m) U',
-        // conj(dm_B1) U, conj(dm_B0) U - i 2 pi dt conj(m) U', conj(m) U
-        cfloat g1[JHS_COILS], g2[JHS_COILS], gb1[JHS_COILS], gb0[JHS_COILS], gu[JHS_COILS];
-#pragma unroll
-        for (int c = 0; c < JHS_COILS; c++) g1[c] = g2[c] = gb1[c] = gb0[c] = gu[c] = {0, 0};
-
-        for (int r = r0; r < r1; r++) {
-            const cfloat dk = delta_k[r], k0 = k_start[r];
-            const double theta = double(dk.re) * x + double(dk.im) * yy + double(dt) * dev::TWO_PI * B0;
-            const double phi0 = -half * dev::TWO_PI * B0 + double(k0.re) * x;
-            const cfloat step = dev::polar_f(expf(decay), dev::reduce_phase(theta));
-            cfloat U[JHS_COILS], Up[JHS_COILS];
-#pragma unroll
-            for (int c = 0; c < JHS_COILS; c++) U[c] = Up[c] = {0, 0};
-            const cfloat* yr = y + (size_t(cg) * nr + r) * ns;
-            for (int sa = 0; sa < ns; sa += JHS_ANCHOR) {
-                // Phi at the anchor sample: magnitude exp((ns/2 - s) dt R2), phase phi0 + s theta
-                cfloat ph = dev::polar_f(expf((0.5f * float(ns) - float(sa)) * dt * R2),
-                                         dev::reduce_phase(phi0 + double(sa) * theta));
-                const int se = min(ns, sa + JHS_ANCHOR);
-                for (int s = sa; s < se; s++) {
-                    const float sp = float(s) - 0.5f * float(ns);
-#pragma unroll
-                    for (int c = 0; c < JHS_COILS; c++) {
-                        if (cg + c < nc) {
-                            const cfloat yv = yr[size_t(c) * nr * ns + s];
-                            // y conj(Phi)
-                            const cfloat q {fmaf(yv.re, ph.re, yv.im * ph.im), fmaf(yv.im, ph.re, -yv.re * ph.im)};
-                            U[c].re += q.re;
-                            U[c].im += q.im;
-                            Up[c].re = fmaf(sp, q.re, Up[c].re);
-                            Up[c].im = fmaf(sp, q.im, Up[c].im);
-                        }
-                    }
-                    ph = dev::cmul(ph, step);
-                }
-            }
-            const size_t i = size_t(r) * nv + vv;
-            const cfloat me = m[i], a1 = dT1[i], a2 = dT2[i], ab1 = dB1[i], ab0 = dB0[i];
-#pragma unroll
-            for (int c = 0; c < JHS_COILS; c++) {
-                g1[c] = dev::cfma_conj(U[c], a1, g1[c]);
-                g2[c] = dev::cfma_conj(U[c], a2, g2[c]);
-                g2[c] = dev::cfma_conj(Up[c] * (dt * R2 * R2), me, g2[c]);
-                gb1[c] = dev::cfma_conj(U[c], ab1, gb1[c]);
-                gb0[c] = dev::cfma_conj(U[c], ab0, gb0[c]);
-                gb0[c] = dev::cfma_conj(dev::cmul(Up[c], cfloat {0.0f, -dev::TWO_PI_F * dt}), me, gb0[c]);
-                gu[c] = dev::cfma_conj(U[c], me, gu[c]);
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < JHS_COILS; c++) {
-            if (cg + c >= nc) continue;
-            const cfloat C = coils[size_t(cg + c) * nv + vv];
-            const cfloat rc = dev::cmul(rho, C);
-            out[0] = out[0] + dev::cmul(conj(rc * T1), g1[c]);
-            out[1] = out[1] + dev::cmul(conj(rc * T2), g2[c]);
-            out[2] = out[2] + dev::cmul(conj(rc), gb1[c]);
-            out[3] = out[3] + dev::cmul(conj(rc), gb0[c]);
-            out[4] = out[4] + dev::cmul(conj(C), gu[c]);
-        }
+    int cg = 0;
+    for (; cg + 4 <= nc; cg += 4)
+        spiral_hermitian_coils<4>(sv, vv, nv, m, dT1, dT2, dB1, dB0, coils, cg, y, k_start, delta_k, nr, ns, dt,
+                                  r0, r1, out);
+    switch (nc - cg) {
+        case 3:
+            spiral_hermitian_coils<3>(sv, vv, nv, m, dT1, dT2, dB1, dB0, coils, cg, y, k_start, delta_k, nr, ns, dt,
+                                      r0, r1, out);
+            break;
+        case 2:
+            spiral_hermitian_coils<2>(sv, vv, nv, m, dT1, dT2, dB1, dB0, coils, cg, y, k_start, delta_k, nr, ns, dt,
+                                      r0, r1, out);
+            break;
+        case 1:
+            spiral_hermitian_coils<1>(sv, vv, nv, m, dT1, dT2, dB1, dB0, coils, cg, y, k_start, delta_k, nr, ns, dt,
+                                      r0, r1, out);
+            break;
+        default:
+            break;
     }
     out[5] = dev::cmul(out[4], cfloat {0.0f, -1.0f});
     if (ok) {

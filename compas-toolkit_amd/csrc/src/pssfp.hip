@@ -15,11 +15,13 @@
 //     applies nRF Rodrigues rotations with little T2 damping.
 //   * Rodrigues' rotation is evaluated without sqrt/rcp/sin/cos for the small
 //     per-step angles: with u = |a|^2, s1(u) = sin(t)/t and s2(u) =
-//     (1 - cos t)/t^2 are Taylor polynomials in u (|t| <= 0.5: 7 terms, error
-//     < 1e-16), cos t = 1 - u s2(u), and
+//     (1 - cos t)/t^2 are Taylor polynomials in u (|t| <= 1/4: 6 terms, error
+//     < 1e-17), cos t = 1 - u s2(u), and
 //       m' = cos t m + s1 (a x m) + s2 (a . m) a,
 //     which equals the reference's normalised form.  Larger angles fall back
 //     to FP64 sincos.
+//   * The RF sub-pulse table is staged in LDS as doubles (broadcast reads in
+//     the sub-step loop instead of a global load per sub-step).
 
 #include "compas/device/common.h"
 #include "compas/internal.h"
@@ -45,20 +47,23 @@ struct Vec3d {
 __device__ inline void rotate_small(Vec3d& m, double ax, double ay, double az, double axy2) {
     const double u = fma(az, az, axy2);
     double s1, s2;
-    if (u <= 0.25) {
-        // Taylor coefficients of sin(t)/t and (1-cos t)/t^2 in u = t^2
-        s1 = fma(u, -1.0 / 6227020800.0, 1.0 / 39916800.0);
-        s1 = fma(u, s1, -1.0 / 362880.0);
-        s1 = fma(u, s1, 1.0 / 5040.0);
-        s1 = fma(u, s1, -1.0 / 120.0);
-        s1 = fma(u, s1, 1.0 / 6.0);
-        s1 = fma(u, -s1, 1.0);
-        s2 = fma(u, -1.0 / 87178291200.0, 1.0 / 479001600.0);
-        s2 = fma(u, s2, -1.0 / 3628800.0);
-        s2 = fma(u, s2, 1.0 / 40320.0);
-        s2 = fma(u, s2, -1.0 / 720.0);
-        s2 = fma(u, s2, 1.0 / 24.0);
-        s2 = fma(u, -s2, 0.5);
+    if (u <= 0.0625) {
+        // Taylor series of sin(t)/t and (1-cos t)/t^2 in u = t^2, |t| <= 1/4: six
+        // terms each (first omitted terms below 1e-17).  Summed over the powers
+        // of u with the running sum as the FMA addend, so every step is one
+        // v_fmac with the coefficient from a scalar register and both series
+        // share the powers.
+        const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
+        s1 = fma(u, -1.0 / 6.0, 1.0);
+        s1 = fma(u2, 1.0 / 120.0, s1);
+        s1 = fma(u3, -1.0 / 5040.0, s1);
+        s1 = fma(u4, 1.0 / 362880.0, s1);
+        s1 = fma(u5, -1.0 / 39916800.0, s1);
+        s2 = fma(u, -1.0 / 24.0, 0.5);
+        s2 = fma(u2, 1.0 / 720.0, s2);
+        s2 = fma(u3, -1.0 / 40320.0, s2);
+        s2 = fma(u4, 1.0 / 3628800.0, s2);
+        s2 = fma(u5, -1.0 / 479001600.0, s2);
     } else {
         const double t = sqrt(u);
         double s, c;
@@ -89,9 +94,12 @@ __device__ inline void rotate_z(Vec3d& m, double c, double s) {
     m.y = y;
 }
 
+constexpr int RF_CHUNK = 256;  // RF sub-steps staged in LDS at a time
+
 template<int ZPL>
 __global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
     __shared__ double2 partial[2][8][64];
+    __shared__ double2 grf[RF_CHUNK];  // gamma_dt_RF as doubles (LDS broadcast reads)
     const int lane = threadIdx.x & 63;
     const int group = threadIdx.x >> 6;
     const int groups = blockDim.x >> 6;
@@ -123,17 +131,37 @@ __global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
         relax3(m[i], E1_inv, E2_inv, R1_inv);
     }
 
+    auto stage_rf = [&](int k0) {
+        for (int k = threadIdx.x; k < RF_CHUNK && k0 + k < a.nRF; k += blockDim.x) {
+            const cfloat g = a.gamma_dt_RF[k0 + k];
+            grf[k] = make_double2(double(g.re), double(g.im));
+        }
+    };
+    const bool rf_resident = a.nRF <= RF_CHUNK;
+    if (rf_resident) stage_rf(0);
+    __syncthreads();
+
+    // every thread of the block runs the same excitations, so the barriers of a
+    // chunked RF table are uniform
     auto excite = [&](double tre, double tim) {
-        for (int k = 0; k < a.nRF; k++) {
-            const cfloat g = a.gamma_dt_RF[k];  // wave-uniform
-            const double zre = tre * double(g.re) - tim * double(g.im);
-            const double zim = tre * double(g.im) + tim * double(g.re);
-            const double ax = zre * B1, ay = -zim * B1;
-            const double axy2 = fma(ax, ax, ay * ay);
+        for (int k0 = 0; k0 < a.nRF; k0 += RF_CHUNK) {
+            if (!rf_resident) {
+                __syncthreads();
+                stage_rf(k0);
+                __syncthreads();
+            }
+            const int kn = min(RF_CHUNK, a.nRF - k0);
+            for (int k = 0; k < kn; k++) {
+                const double2 g = grf[k];
+                const double zre = tre * g.x - tim * g.y;
+                const double zim = tre * g.y + tim * g.x;
+                const double ax = zre * B1, ay = -zim * B1;
+                const double axy2 = fma(ax, ax, ay * ay);
 #pragma unroll
-            for (int i = 0; i < ZPL; i++) {
-                rotate_small(m[i], ax, ay, az[i], axy2);
-                relax3(m[i], E1_ex, E2_ex, R1_ex);
+                for (int i = 0; i < ZPL; i++) {
+                    rotate_small(m[i], ax, ay, az[i], axy2);
+                    relax3(m[i], E1_ex, E2_ex, R1_ex);
+                }
             }
         }
     };

@@ -125,7 +125,7 @@ def test_pssfp_workload(ctx, oracle):
     assert rel(got, want) < MAG_TOL
 
 
-@pytest.mark.parametrize("nz,nRF", [(1, 1), (3, 5), (35, 25), (60, 7), (130, 3)])
+@pytest.mark.parametrize("nz,nRF", [(1, 1), (3, 5), (35, 25), (60, 7), (130, 3), (3, 300), (2, 600)])
 def test_pssfp_shapes(ctx, oracle, nz, nRF):
     banner(f"pssfp nz={nz} nRF={nRF}")
     n = 12
