@@ -85,6 +85,8 @@ _SIGNATURES = {
     "compas_set_contraction_precision": (None, [_i]),
     "compas_compute_jacobian_spiral": (_vp, [_vp] * 10),
     "compas_compute_jacobian_hermitian_spiral": (_vp, [_vp] * 10),
+    "compas_simulate_magnetization_tangent_fisp": (_vp, [_vp, _i, _vp, _vp]),
+    "compas_simulate_magnetization_tangent_pssfp": (_vp, [_vp, _i, _vp, _vp]),
     "compas_server_create": (_vp, [_vp, _i, C.c_int64]),
     "compas_server_destroy": (None, [_vp]),
     "compas_server_submit_simulate_signal": (C.c_int64, [_vp, _vp, _vp, _vp, _vp, _i]),
@@ -402,6 +404,20 @@ def simulate_magnetization_derivative(field, echos, parameters, sequence, delta=
     fn = (lib().compas_simulate_magnetization_derivative_fisp if isinstance(sequence, FispSequence)
           else lib().compas_simulate_magnetization_derivative_pssfp)
     return _wrap(fn(ctx.ptr, int(field), e.ptr, parameters.ptr, float(delta), sequence.ptr))
+
+
+def simulate_magnetization_tangent(field, parameters, sequence) -> CompasArray:
+    """Forward-mode derivative of the echoes w.r.t. T1, T2, B1 or B0: the exact
+    derivative of the discrete recursion (no finite-difference step), computed
+    beside the FISP EPG or pSSFP isochromat recursion in one pass."""
+    ctx = get_context()
+    if isinstance(sequence, FispSequence):
+        fn = lib().compas_simulate_magnetization_tangent_fisp
+    elif isinstance(sequence, pSSFPSequence):
+        fn = lib().compas_simulate_magnetization_tangent_pssfp
+    else:
+        raise TypeError("expected a FispSequence or pSSFPSequence")
+    return _wrap(fn(ctx.ptr, int(field), parameters.ptr, sequence.ptr))
 
 
 def simulate_magnetization_derivatives(echos, parameters, sequence, delta=1e-4):

@@ -245,6 +245,18 @@ Array<cfloat, 2> simulate_magnetization_derivative(
     pSSFPSequence sequence,
     float delta = DEFAULT_FINITE_DIFFERENCE_DELTA);
 
+// Forward-mode derivative of the FISP echoes with respect to one tissue
+// parameter (field = T1, T2, B1 or B0 of TissueParameterField): the exact
+// derivative of the discrete EPG recursion, computed beside it in one pass.
+// No counterpart in the reference, which differentiates by finite
+// differences (simulate_magnetization_derivative, derivative.h:21-47).
+Array<cfloat, 2> simulate_magnetization_tangent(
+    const CompasContext& ctx, int field, TissueParameters parameters, FISPSequence sequence);
+
+// The same for the pSSFP isochromat simulation (FP64 state, FP32 result).
+Array<cfloat, 2> simulate_magnetization_tangent(
+    const CompasContext& ctx, int field, TissueParameters parameters, pSSFPSequence sequence);
+
 // ---------------------------------------------------------------- signal --
 
 // Same enumerators as the reference (signal.h:14-44).  On MI355X:

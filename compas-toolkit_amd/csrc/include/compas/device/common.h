@@ -76,6 +76,36 @@ __device__ inline void sincos_fast(float x, float* s, float* c) {
     *c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// sin and cos of |x| <= pi + 1e-3 (a phase already reduced by reduce_phase):
+// the polynomial part of sincos_fast without its large-argument branch.
+__device__ inline void sincos_reduced(float x, float* s, float* c) {
+    const float kf = rintf(x * 0.63661977236758134f);
+    float r = fmaf(-kf, 1.5707963705062866f, x);
+    r = fmaf(-kf, -4.3711388286737929e-08f, r);
+    const float r2 = r * r;
+    float ps = fmaf(r2, 2.7557319e-6f, -1.9841270e-4f);
+    ps = fmaf(r2, ps, 8.3333333e-3f);
+    ps = fmaf(r2, ps, -1.6666667e-1f);
+    const float sr = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, -2.7557319e-7f, 2.4801587e-5f);
+    pc = fmaf(r2, pc, -1.3888889e-3f);
+    pc = fmaf(r2, pc, 4.1666668e-2f);
+    pc = fmaf(r2, pc, -0.5f);
+    const float cr = fmaf(r2, pc, 1.0f);
+    const int q = int(kf);
+    const float ss = (q & 1) ? cr : sr;
+    const float cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// mag * exp(i phi) with the phase formed in FP64 and reduced before the float sincos
+__device__ inline cfloat polar_reduced(float mag, double phi) {
+    float s, c;
+    sincos_reduced(reduce_phase(phi), &s, &c);
+    return {mag * c, mag * s};
+}
+
 __device__ inline cfloat polar_f(float mag, float phase) {
     float s, c;
     sincosf(phase, &s, &c);

@@ -1,0 +1,85 @@
+#pragma once
+// Pieces shared by the pSSFP isochromat kernels (primal and tangent).
+//
+// Semantics: reference src/operators/isochromat.cuh:9-128.  Rodrigues'
+// rotation is evaluated without sqrt/rcp/sin/cos for the small per-step
+// angles: with u = |a|^2, s1(u) = sin(t)/t and s2(u) = (1 - cos t)/t^2 are
+// Taylor polynomials in u (|t| <= 1/4: 6 terms, error < 1e-17),
+// cos t = 1 - u s2(u), and
+//   m' = cos t m + s1 (a x m) + s2 (a . m) a,
+// which equals the reference's normalised form.  Larger angles use FP64 sincos.
+
+#include "compas/device/common.h"
+
+namespace compas {
+namespace kernels {
+
+struct PssfpArgs {
+    dev::TissueView tissue;
+    const cfloat* RF;  // [nTR]
+    const cfloat* gamma_dt_RF;  // [nRF]
+    const float* z;  // [nz]
+    int nTR, nRF, nz;
+    float dt_ex, dt_inv, dt_pr;
+    float g_ex, g_pr;
+    cfloat* echos;  // [nTR][nvoxels]
+};
+
+struct Vec3d {
+    double x, y, z;
+};
+
+__device__ inline void rotate_small(Vec3d& m, double ax, double ay, double az, double axy2) {
+    const double u = fma(az, az, axy2);
+    double s1, s2;
+    if (u <= 0.0625) {
+        // Taylor series of sin(t)/t and (1-cos t)/t^2 in u = t^2, |t| <= 1/4: six
+        // terms each (first omitted terms below 1e-17).  Summed over the powers
+        // of u with the running sum as the FMA addend, so every step is one
+        // v_fmac with the coefficient from a scalar register and both series
+        // share the powers.
+        const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
+        s1 = fma(u, -1.0 / 6.0, 1.0);
+        s1 = fma(u2, 1.0 / 120.0, s1);
+        s1 = fma(u3, -1.0 / 5040.0, s1);
+        s1 = fma(u4, 1.0 / 362880.0, s1);
+        s1 = fma(u5, -1.0 / 39916800.0, s1);
+        s2 = fma(u, -1.0 / 24.0, 0.5);
+        s2 = fma(u2, 1.0 / 720.0, s2);
+        s2 = fma(u3, -1.0 / 40320.0, s2);
+        s2 = fma(u4, 1.0 / 3628800.0, s2);
+        s2 = fma(u5, -1.0 / 479001600.0, s2);
+    } else {
+        const double t = sqrt(u);
+        double s, c;
+        sincos(t, &s, &c);
+        s1 = s / t;
+        s2 = (1.0 - c) / u;
+    }
+    const double c = fma(-u, s2, 1.0);
+    const double cx = ay * m.z - az * m.y;
+    const double cy = az * m.x - ax * m.z;
+    const double cz = ax * m.y - ay * m.x;
+    const double d = s2 * (ax * m.x + ay * m.y + az * m.z);
+    m.x = fma(c, m.x, fma(s1, cx, d * ax));
+    m.y = fma(c, m.y, fma(s1, cy, d * ay));
+    m.z = fma(c, m.z, fma(s1, cz, d * az));
+}
+
+__device__ inline void relax3(Vec3d& m, double E1, double E2, double R1) {
+    m.x *= E2;
+    m.y *= E2;
+    m.z = fma(m.z, E1, R1);
+}
+
+__device__ inline void rotate_z(Vec3d& m, double c, double s) {
+    const double x = c * m.x - s * m.y;
+    const double y = c * m.y + s * m.x;
+    m.x = x;
+    m.y = y;
+}
+
+constexpr int RF_CHUNK = 256;  // RF sub-steps staged in LDS at a time
+
+}  // namespace kernels
+}  // namespace compas

@@ -81,6 +81,9 @@ void spiral_sum(
     const CompasContext& ctx, const TissueParameters& p, const cfloat* A, const cfloat* Ab, const cfloat* coils,
     int nc, const SpiralTrajectory& t, cfloat* out);
 
+// Per (slice, TR) RF table of the FISP kernels: (deg2rad |RF|, cos phi, sin phi, 0).
+Array<float4, 1> fisp_rf_table(const CompasContext& ctx, const FISPSequence& seq);
+
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);
 

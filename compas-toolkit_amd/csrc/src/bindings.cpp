@@ -537,6 +537,26 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
     });
 }
 
+CompasArrayHandle* compas_simulate_magnetization_tangent_fisp(
+    const CompasContextHandle* ctx, int field, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* sequence) {
+    return catch_exceptions([&] {
+        return new_array(simulate_magnetization_tangent(context(ctx), field,
+                                                        unwrap<TissueParameters>(parameters, "parameters"),
+                                                        unwrap<FISPSequence>(sequence, "sequence")));
+    });
+}
+
+CompasArrayHandle* compas_simulate_magnetization_tangent_pssfp(
+    const CompasContextHandle* ctx, int field, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* sequence) {
+    return catch_exceptions([&] {
+        return new_array(simulate_magnetization_tangent(context(ctx), field,
+                                                        unwrap<TissueParameters>(parameters, "parameters"),
+                                                        unwrap<pSSFPSequence>(sequence, "sequence")));
+    });
+}
+
 // ---- request server
 
 namespace {

@@ -24,23 +24,11 @@
 #include <cmath>
 
 #include "compas/device/common.h"
+#include "compas/device/fisp_common.h"
+#include "compas/internal.h"
 
 namespace compas {
 namespace kernels {
-
-struct FispArgs {
-    dev::TissueView tissue;
-    const float4* table;  // [nslices][nTR]: (deg2rad*|RF|, cos phi, sin phi, 0)
-    int nTR;
-    int nslices;
-    float TR, TE, TW, TI;
-    int N;  // max_state (runtime; <= SPL * LPV)
-    int usf;
-    int repetitions;
-    bool inversion;
-    bool wait_spoiling;
-    cfloat* echos;  // [nTR * usf][nvoxels]
-};
 
 __global__ void fisp_rf_table(const cfloat* RF, const cfloat* profiles, int nTR, int nslices, float4* table) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,18 +43,6 @@ __global__ void fisp_rf_table(const cfloat* RF, const cfloat* profiles, int nTR,
         s = rf.im / a;
     }
     table[i] = make_float4(dev::DEG2RAD_F * a, c, s, 0.0f);
-}
-
-__device__ inline float relax(float dt, float T) {
-    return expf(-dt * (1.0f / T));
-}
-
-// exp(i 2 pi dt B0), or exactly 1 when B0 == 0 (fisp_kernels.cuh:110-117)
-__device__ inline cfloat off_resonance(float dt, float B0) {
-    if (B0 == 0.0f) return {1.0f, 0.0f};
-    float s, c;
-    sincosf(dev::TWO_PI_F * dt * B0, &s, &c);
-    return {c, s};
 }
 
 template<int SPL, int LPV>
@@ -312,6 +288,18 @@ static FispLauncher fisp_launcher_lpv4(int spl) {
     }
 }
 
+Array<float4, 1> fisp_rf_table(const CompasContext& ctx, const FISPSequence& seq) {
+    const int nTR = int(seq.RF_train.size(0)), nslices = int(seq.sliceprofiles.size(0));
+    auto table = ctx.empty<float4, 1>({int64_t(nslices) * nTR});
+    if (table.numel() == 0) return table;
+    KernelScope ks(ctx, "fisp_rf_table");
+    const int total = nslices * nTR;
+    kernels::fisp_rf_table<<<dev::div_ceil(total, 256), 256, 0, ctx.stream()>>>(
+        seq.RF_train.data(), seq.sliceprofiles.data(), nTR, nslices, table.data());
+    check_launch("fisp_rf_table");
+    return table;
+}
+
 static int simd_count(int device) {
     static int cached[64] = {0};
     if (device >= 0 && device < 64 && cached[device]) return cached[device];
@@ -374,14 +362,7 @@ Array<cfloat, 2> simulate_magnetization(
         return echos;
     }
 
-    auto table = ctx.empty<float4, 1>({int64_t(nslices) * nTR});
-    {
-        KernelScope ks(ctx, "fisp_rf_table");
-        int total = nslices * nTR;
-        kernels::fisp_rf_table<<<dev::div_ceil(total, 256), 256, 0, ctx.stream()>>>(
-            seq.RF_train.data(), seq.sliceprofiles.data(), nTR, nslices, table.data());
-        check_launch("fisp_rf_table");
-    }
+    auto table = fisp_rf_table(ctx, seq);
 
     kernels::FispArgs a;
     a.tissue = dev::tissue_view(parameters);

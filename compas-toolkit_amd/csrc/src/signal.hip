@@ -13,8 +13,11 @@
 // reduced to [-pi, pi) before the float sincos.
 //
 // Spiral:  S[c][r][s] = sum_v A[r][v] C[c][v] E_rv^s with per-readout
-// exponents, evaluated by a direct kernel: FP64-anchored powers E^s0 every 8
+// exponents, evaluated by a direct kernel: FP64-anchored powers E^s0 every 16
 // samples, complex recurrence in between.
+
+#include <cstdlib>
+#include <type_traits>
 
 #include "compas/device/common.h"
 #include "compas/internal.h"
@@ -58,13 +61,39 @@ __global__ void signal_spiral_factors(
     A[size_t(r) * nv + v] = dev::cmul(echos[size_t(r) * nv + v], f);
 }
 
-constexpr int SP_SAMPLES = 8;  // samples per thread (recurrence length)
 constexpr int SP_COILS = 4;  // coils per block
 
-// out[c][r][s] = sum_v C[c][v] (A[r][v] + (s - ns/2) Ab[r][v]) E_rv^s  (Ab only if TWO)
-// grid: (nr, ceil(ns / 64), ceil(nc / SP_COILS)); block 256 = 4 waves.
-// lane % 32 strides over voxels; (wave, lane / 32) selects 8 consecutive samples.
+// Per-voxel inputs of the spiral sum, loaded one voxel ahead of their use.
 template<bool TWO>
+struct SpiralInputs {
+    float T2, x, y, B0;
+    cfloat A, Ab;
+    cfloat C[SP_COILS];
+};
+
+template<bool TWO>
+__device__ inline SpiralInputs<TWO> spiral_inputs(
+    const dev::TissueView& t, const cfloat* __restrict__ A, const cfloat* __restrict__ Ab,
+    const cfloat* __restrict__ coils, int c0, int ncl, size_t row, int v) {
+    SpiralInputs<TWO> in;
+    const int nv = t.nvoxels;
+    in.T2 = t.T2(v);
+    in.x = t.x(v);
+    in.y = t.y(v);
+    in.B0 = t.B0(v);
+    in.A = A[row + v];
+    in.Ab = TWO ? Ab[row + v] : cfloat {0, 0};
+#pragma unroll
+    for (int c = 0; c < SP_COILS; c++) in.C[c] = (c < ncl) ? coils[size_t(c0 + c) * nv + v] : cfloat {0, 0};
+    return in;
+}
+
+// out[c][r][s] = sum_v C[c][v] (A[r][v] + (s - ns/2) Ab[r][v]) E_rv^s  (Ab only if TWO)
+// grid: (nr, ceil(ns / (8 SPS)), ceil(nc / SP_COILS)); block 256 = 4 waves.
+// lane % 32 strides over voxels; (wave, lane / 32) selects SPS consecutive
+// samples.  E^s0 is evaluated from an FP64 phase at the first sample of the
+// thread, then by the complex recurrence.
+template<bool TWO, int SPS>
 __global__ void __launch_bounds__(256) signal_spiral_direct(
     dev::TissueView tissue, const cfloat* __restrict__ A, const cfloat* __restrict__ Ab,
     const cfloat* __restrict__ coils, int nc,
@@ -74,46 +103,50 @@ __global__ void __launch_bounds__(256) signal_spiral_direct(
     const int wave = threadIdx.x >> 6;
     const int vl = lane & 31;
     const int sg = wave * 2 + (lane >> 5);
-    const int s0 = blockIdx.y * 64 + sg * SP_SAMPLES;
+    const int s0 = blockIdx.y * 8 * SPS + sg * SPS;
     const int c0 = blockIdx.z * SP_COILS;
+    const int ncl = min(SP_COILS, nc - c0);
     const int nv = tissue.nvoxels;
+    const size_t row = size_t(r) * nv;
     const cfloat dk = delta_k[r];
+    const double two_pi_dt = double(dt) * dev::TWO_PI;
+    const float half = 0.5f * float(ns);
 
-    float acc_re[SP_COILS][SP_SAMPLES], acc_im[SP_COILS][SP_SAMPLES];
+    float acc_re[SP_COILS][SPS], acc_im[SP_COILS][SPS];
 #pragma unroll
     for (int c = 0; c < SP_COILS; c++)
 #pragma unroll
-        for (int i = 0; i < SP_SAMPLES; i++) {
+        for (int i = 0; i < SPS; i++) {
             acc_re[c][i] = 0.0f;
             acc_im[c][i] = 0.0f;
         }
 
-    if (s0 < ns) {
+    if (s0 < ns && vl < nv) {
+        SpiralInputs<TWO> next = spiral_inputs<TWO>(tissue, A, Ab, coils, c0, ncl, row, vl);
         for (int v = vl; v < nv; v += 32) {
-            const float R2 = 1.0f / tissue.T2(v);
-            const double theta = double(dk.re) * double(tissue.x(v)) + double(dk.im) * double(tissue.y(v))
-                + double(dt) * dev::TWO_PI * double(tissue.B0(v));
+            const SpiralInputs<TWO> in = next;
+            if (v + 32 < nv) next = spiral_inputs<TWO>(tissue, A, Ab, coils, c0, ncl, row, v + 32);
+            const float R2 = 1.0f / in.T2;
+            const double theta = double(dk.re) * double(in.x) + double(dk.im) * double(in.y)
+                + two_pi_dt * double(in.B0);
             const float decay = -dt * R2;
-            const cfloat step = dev::polar_f(expf(decay), dev::reduce_phase(theta));
-            const cfloat anchor = dev::polar_f(expf(float(s0) * decay), dev::reduce_phase(double(s0) * theta));
-            cfloat cur = dev::cmul(anchor, A[size_t(r) * nv + v]);
+            const cfloat step = dev::polar_reduced(expf(decay), theta);
+            const cfloat anchor = dev::polar_reduced(expf(float(s0) * decay), double(s0) * theta);
+            cfloat cur = dev::cmul(anchor, in.A);
             cfloat cur_b {0, 0};
-            if constexpr (TWO) cur_b = dev::cmul(anchor, Ab[size_t(r) * nv + v]);
-            cfloat cv[SP_COILS];
+            if constexpr (TWO) cur_b = dev::cmul(anchor, in.Ab);
 #pragma unroll
-            for (int c = 0; c < SP_COILS; c++) cv[c] = (c0 + c < nc) ? coils[size_t(c0 + c) * nv + v] : cfloat {0, 0};
-#pragma unroll
-            for (int i = 0; i < SP_SAMPLES; i++) {
+            for (int i = 0; i < SPS; i++) {
                 cfloat amp = cur;
                 if constexpr (TWO) {
-                    const float sp = float(s0 + i) - 0.5f * float(ns);
+                    const float sp = float(s0 + i) - half;
                     amp = {fmaf(sp, cur_b.re, cur.re), fmaf(sp, cur_b.im, cur.im)};
                     cur_b = dev::cmul(cur_b, step);
                 }
 #pragma unroll
                 for (int c = 0; c < SP_COILS; c++) {
-                    acc_re[c][i] = fmaf(amp.re, cv[c].re, fmaf(-amp.im, cv[c].im, acc_re[c][i]));
-                    acc_im[c][i] = fmaf(amp.re, cv[c].im, fmaf(amp.im, cv[c].re, acc_im[c][i]));
+                    acc_re[c][i] = fmaf(amp.re, in.C[c].re, fmaf(-amp.im, in.C[c].im, acc_re[c][i]));
+                    acc_im[c][i] = fmaf(amp.re, in.C[c].im, fmaf(amp.im, in.C[c].re, acc_im[c][i]));
                 }
                 cur = dev::cmul(cur, step);
             }
@@ -123,7 +156,7 @@ __global__ void __launch_bounds__(256) signal_spiral_direct(
 #pragma unroll
     for (int c = 0; c < SP_COILS; c++)
 #pragma unroll
-        for (int i = 0; i < SP_SAMPLES; i++) {
+        for (int i = 0; i < SPS; i++) {
             float re = acc_re[c][i], im = acc_im[c][i];
 #pragma unroll
             for (int off = 16; off > 0; off >>= 1) {
@@ -136,9 +169,9 @@ __global__ void __launch_bounds__(256) signal_spiral_direct(
     if (vl == 0 && s0 < ns) {
 #pragma unroll
         for (int c = 0; c < SP_COILS; c++) {
-            if (c0 + c >= nc) continue;
+            if (c >= ncl) continue;
 #pragma unroll
-            for (int i = 0; i < SP_SAMPLES; i++) {
+            for (int i = 0; i < SPS; i++) {
                 if (s0 + i < ns) signal[(size_t(c0 + c) * nr + r) * ns + s0 + i] = {acc_re[c][i], acc_im[c][i]};
             }
         }
@@ -195,14 +228,34 @@ void spiral_sum(
     int nc, const SpiralTrajectory& t, cfloat* out) {
     const int nr = t.nreadouts, ns = t.samples_per_readout;
     if (nc == 0 || nr == 0 || ns == 0) return;
+    // samples per thread: 8 measured fastest at 4 coils x 1120 x 512 over
+    // 50176 voxels (24.0 ms against 30.8 ms for 16: more waves resident);
+    // COMPAS_SPIRAL_SAMPLES=4 or 16 selects another instance
+    static const int sps = [] {
+        const char* e = std::getenv("COMPAS_SPIRAL_SAMPLES");
+        const int v = e ? std::atoi(e) : 8;
+        return (v == 4 || v == 16) ? v : 8;
+    }();
     KernelScope ks(ctx, "signal_spiral_direct");
-    dim3 grid(nr, dev::div_ceil(ns, 64), dev::div_ceil(nc, kernels::SP_COILS));
+    dim3 grid(nr, dev::div_ceil(ns, 8 * sps), dev::div_ceil(nc, kernels::SP_COILS));
+    const auto tv = dev::tissue_view(p);
+    auto launch = [&](auto two, auto samples) {
+        constexpr bool TWO = decltype(two)::value;
+        constexpr int SPS = decltype(samples)::value;
+        kernels::signal_spiral_direct<TWO, SPS><<<grid, 256, 0, ctx.stream()>>>(
+            tv, A, TWO ? Ab : nullptr, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
+    };
+    using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
+    using I16 = std::integral_constant<int, 16>;
     if (Ab) {
-        kernels::signal_spiral_direct<true><<<grid, 256, 0, ctx.stream()>>>(
-            dev::tissue_view(p), A, Ab, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
+        if (sps == 4) launch(std::true_type {}, I4 {});
+        else if (sps == 16) launch(std::true_type {}, I16 {});
+        else launch(std::true_type {}, I8 {});
     } else {
-        kernels::signal_spiral_direct<false><<<grid, 256, 0, ctx.stream()>>>(
-            dev::tissue_view(p), A, nullptr, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
+        if (sps == 4) launch(std::false_type {}, I4 {});
+        else if (sps == 16) launch(std::false_type {}, I16 {});
+        else launch(std::false_type {}, I8 {});
     }
     check_launch("signal_spiral_direct");
 }

@@ -176,6 +176,22 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
     const CompasArrayHandle* delta_echos_B0, const CompasObjectHandle* parameters,
     const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* vector);
 
+/* Forward-mode derivative of the FISP echoes [nreadouts][nvoxels] with
+ * respect to one tissue parameter (field: 0 T1, 1 T2, 2 B1, 3 B0): the exact
+ * derivative of the discrete recursion, computed beside it in one pass.  No
+ * counterpart in the reference, which differentiates by FP32 finite
+ * differences (compas_simulate_magnetization_derivative_fisp). */
+CompasArrayHandle* compas_simulate_magnetization_tangent_fisp(
+    const CompasContextHandle* ctx, int field, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* sequence);
+
+/* The same for the pSSFP isochromat simulation (FP64 state, FP32 result);
+ * replaces compas_simulate_magnetization_derivative_pssfp's finite
+ * differences (reference julia-bindings/src/bindings.cpp:376). */
+CompasArrayHandle* compas_simulate_magnetization_tangent_pssfp(
+    const CompasContextHandle* ctx, int field, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* sequence);
+
 /* Request server (no counterpart in the reference, whose host is a Julia
  * session calling one operation at a time): continuous batching on one GPU.
  * `slots` requests run concurrently, each on a private stream and memory

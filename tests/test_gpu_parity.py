@@ -443,6 +443,93 @@ def test_fisp_finite_difference_derivative(ctx, oracle, field):
     assert rel(got, want) < 5e-3
 
 
+def central_difference(simulate, t, seq, name):
+    """FP64 oracle central difference of the echoes w.r.t. one tissue field.
+
+    Steps are exact in float32 (the oracle reads float32 tissue): 2^-16 of the
+    value's binade for T1/T2, 2^-16 for B1, 2^-14 Hz for B0; the denominator is
+    the difference actually taken.  At these steps the truncation error (h^2/6
+    times the third derivative) and the FP64 rounding of the two runs divided
+    by 2h both stay below 1e-8 relative.
+    """
+    v = getattr(t, name).astype(np.float32)
+    if name in ("T1", "T2"):
+        h = np.exp2(np.floor(np.log2(v)) - 16).astype(np.float32)
+    elif name == "B1":
+        h = np.full_like(v, 2.0 ** -16)
+    else:
+        h = np.full_like(v, 2.0 ** -14)
+    vp = (v + h).astype(np.float32)
+    vm = (v - h).astype(np.float32)
+    fp = simulate(t.replace(**{name: vp}), seq, "f64")
+    fm = simulate(t.replace(**{name: vm}), seq, "f64")
+    return (fp - fm) / (vp.astype(np.float64) - vm.astype(np.float64))[None, :]
+
+
+TANGENT_TOL = 1e-4  # FP32 recursion against the FP64 derivative
+
+
+@pytest.mark.parametrize("field", [ca.T1, ca.T2, ca.B1, ca.B0])
+@pytest.mark.parametrize("N", [3, 16, 25, 100])
+def test_fisp_tangent_derivative(ctx, oracle, field, N):
+    name = ["T1", "T2", "B1", "B0"][field]
+    banner(f"fisp tangent {name} N={N}")
+    t, seq = small_fisp(nv=97, nTR=40, N=N)
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    s = ca.FispSequence.from_sequence(seq)
+    got = ca.simulate_magnetization_tangent(field, p, s).collect()
+    assert got.shape == (40, 97)
+    assert rel(got, central_difference(oracle.fisp, t, seq, name)) < TANGENT_TOL
+
+
+@pytest.mark.parametrize("field", [ca.T1, ca.T2, ca.B1, ca.B0])
+def test_fisp_tangent_sequence_options(ctx, oracle, field):
+    name = ["T1", "T2", "B1", "B0"][field]
+    banner(f"fisp tangent {name} usf/reps/TW/slices")
+    t, seq = small_fisp(nv=70, nTR=30, N=12)
+    seq.undersampling = 2
+    seq.repetitions = 3
+    seq.TW = 0.12
+    seq.profiles = np.stack([np.linspace(0.6, 1.1, 30), np.full(30, 0.9 - 0.2j)]).astype(np.complex64)
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    for inversion, spoil in ((True, True), (False, False)):
+        seq.inversion = inversion
+        seq.wait_spoiling = spoil
+        s = ca.FispSequence.from_sequence(seq)
+        got = ca.simulate_magnetization_tangent(field, p, s).collect()
+        assert got.shape == (60, 70)
+        assert rel(got, central_difference(oracle.fisp, t, seq, name)) < TANGENT_TOL
+
+
+def test_fisp_tangent_b0_zero_and_errors(ctx, oracle):
+    t, seq = small_fisp(nv=50, nTR=30, N=10, b0=None)
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    s = ca.FispSequence.from_sequence(seq)
+    got = ca.simulate_magnetization_tangent(ca.B0, p, s).collect()
+    assert rel(got, central_difference(oracle.fisp, t, seq, "B0")) < TANGENT_TOL
+    with pytest.raises(ca.CompasError, match="tangent"):
+        ca.simulate_magnetization_tangent(4, p, s)
+
+
+@pytest.mark.parametrize("field", [ca.T1, ca.T2, ca.B1, ca.B0])
+@pytest.mark.parametrize("nz,nRF", [(1, 5), (7, 25), (35, 3), (40, 2)])
+def test_pssfp_tangent_derivative(ctx, oracle, field, nz, nRF):
+    # nz = 40 runs two passes of 32 slice positions; nRF = 2 at 60 degrees
+    # takes the sincos branch of the rotation and its closed-form derivative
+    name = ["T1", "T2", "B1", "B0"][field]
+    banner(f"pssfp tangent {name} nz={nz} nRF={nRF}")
+    t = W.make_tissue(W.SplitMix64(6), 9, 20.0, 40.0)
+    seq = W.make_pssfp(24, 60.0, nz=nz, nRF=nRF)
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    got = ca.simulate_magnetization_tangent(field, p, ca.pSSFPSequence.from_sequence(seq)).collect()
+    assert got.shape == (24, 81)
+    assert rel(got, central_difference(oracle.pssfp, t, seq, name)) < TANGENT_TOL
+
+
 def test_residual(ctx):
     rng = W.SplitMix64(3)
     a = (rng.normal(2 * 30 * 17) + 1j * rng.normal(2 * 30 * 17)).reshape(2, 30, 17).astype(np.complex64)
