@@ -35,6 +35,7 @@ import compas_amd as ca  # noqa: E402
 from compas_amd import workloads as W  # noqa: E402
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (published)
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (published)
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16/FP16 MFMA (published)
 PEAK_HBM_GBS = 8000.0
 
@@ -304,9 +305,12 @@ def spiral_bench(args):
     rooflines = {}
     for k in ("pssfp_iso", "spiral_hermitian"):
         if k in kernels:
+            # the isochromat state is FP64, so its roof is the FP64 vector peak
+            peak = PEAK_FP64_TFLOPS if k == "pssfp_iso" else PEAK_FP32_TFLOPS
             ach = work[k] / (kernels[k]["avg_ms"] * 1e-3) / 1e12
-            rooflines[k] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                            "frac": ach / PEAK_FP32_TFLOPS}
+            rooflines[k] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
+            if k == "pssfp_iso":
+                rooflines[k]["note"] = "FP64 isochromat state; the FP32 vector peak is twice this"
     if "signal_spiral_direct" in kernels:
         flop = (8.0 * nc + 6) * rsv + (8.0 * nc + 12 + 2) * rsv  # signal + J.v launches of one step
         ach = flop / (kernels["signal_spiral_direct"]["total_ms"] / args.steps * 1e-3) / 1e12
@@ -314,6 +318,24 @@ def spiral_bench(args):
                                              "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
     roof = dict(rooflines.get(dominant, {}), kernel=dominant, traffic=None) if dominant in rooflines else None
+
+    # derivatives of the echoes w.r.t. T1, T2, B1, B0: one tangent pass per field
+    # against the reference's forward differences (one extra simulation per field)
+    fields = (ca.T1, ca.T2, ca.B1, ca.B0)
+
+    def timed(fn):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = [fn(f) for f in fields]
+        ctx.synchronize()
+        return res, 1e3 * (time.perf_counter() - t0)
+
+    timed(lambda f: ca.simulate_magnetization_tangent(f, params, seq))  # warm-up
+    tangents, t_tan = timed(lambda f: ca.simulate_magnetization_tangent(f, params, seq))
+    fds, t_fd = timed(lambda f: ca.simulate_magnetization_derivative(f, echos0, params, seq))
+    derivatives = {"fields": "T1 T2 B1 B0", "tangent_ms": t_tan, "finite_difference_ms": t_fd,
+                   "note": "tangent: exact derivative of the recursion in one pass per field; finite "
+                           "difference: the reference's method (delta 1e-4), one more simulation per field"}
 
     e, s, jv, jh = (a.collect() for a in out)
     dsn = [d.collect() for d in ds]
@@ -338,11 +360,13 @@ def spiral_bench(args):
         "kernels": kernels,
         "rooflines": rooflines,
         "roofline": roof,
+        "derivatives": derivatives,
     }
     if not args.no_cpu_baseline:
         base, acc = spiral_cpu_baseline(wl, e, s, jv, jh, dsn)
         line["cpu_baseline"] = base
         line["accuracy"] = acc
+        derivatives.update(derivative_accuracy(wl, [t.collect() for t in tangents], [f.collect() for f in fds]))
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))
@@ -430,6 +454,33 @@ def spiral_cpu_baseline(wl, e, s, jv, jh, ds, seconds_budget=16.0):
         "note": "signal, J.v and J^H.y are checked against the FP64 oracle on the GPU echoes and derivatives",
     }
     return base, acc
+
+
+def derivative_accuracy(wl, tangents, fds, nsample=48):
+    """rel-L2 of the GPU derivatives (tangent and forward difference) against the
+    FP64 oracle's central difference, on a strided sample of voxels.  Steps are
+    exact in float32: 2^-16 of the value's binade for T1/T2, 2^-16 for B1,
+    2^-14 Hz for B0."""
+    import pyoracle
+
+    threads = int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    nv = wl.tissue.nvoxels
+    vi = np.linspace(0, nv - 1, nsample).astype(int)
+    sub = wl.tissue.replace(**{f: getattr(wl.tissue, f)[vi] for f in ("T1", "T2", "B1", "B0", "rho", "x", "y", "z")})
+    out = {"sample": f"{nsample} voxels (strided), FP64 central differences"}
+    for k, name in enumerate(("T1", "T2", "B1", "B0")):
+        v = getattr(sub, name).astype(np.float32)
+        if name in ("T1", "T2"):
+            h = np.exp2(np.floor(np.log2(v)) - 16).astype(np.float32)
+        else:
+            h = np.full_like(v, 2.0 ** (-16 if name == "B1" else -14))
+        vp, vm = (v + h).astype(np.float32), (v - h).astype(np.float32)
+        fp = pyoracle.pssfp(sub.replace(**{name: vp}), wl.sequence, "f64", threads)
+        fm = pyoracle.pssfp(sub.replace(**{name: vm}), wl.sequence, "f64", threads)
+        want = (fp - fm) / (vp.astype(np.float64) - vm.astype(np.float64))[None, :]
+        out[f"{name}_tangent_rel_l2"] = pyoracle.rel_l2(tangents[k][:, vi], want)
+        out[f"{name}_finite_difference_rel_l2"] = pyoracle.rel_l2(fds[k][:, vi], want)
+    return out
 
 
 SERVE_NS = 128  # samples per readout of the serving requests (Cartesian and spiral)

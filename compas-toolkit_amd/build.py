@@ -59,9 +59,10 @@ def compile_one(src, incremental, hdr):
         key = hashlib.sha256(fh.read() + hdr.encode() + " ".join(COMMON + HIP_ONLY).encode()).hexdigest()
     if incremental and os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == key:
         return obj, None
-    cmd = [HIPCC] + COMMON + [f"-I{i}" for i in INCLUDES]
+    # hipcc compiles .cpp as HIP too: give every file the gfx950 target
+    cmd = [HIPCC] + COMMON + [f"-I{i}" for i in INCLUDES] + HIP_ONLY
     if src.endswith(".hip"):
-        cmd += HIP_ONLY + ["-x", "hip"]
+        cmd += ["-x", "hip"]
     cmd += ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -1,0 +1,213 @@
+"""Gauss-Newton reconstruction of tissue parameters from k-space data.
+
+The reference stops at the products a reconstruction needs and leaves the loop
+to its Julia users: the objective ||S(p) - y||^2 and residual
+(CompasToolkit.jl:1015-1036, src/utils/residual.cu:8-46), J.v
+(CompasToolkit.jl compute_jacobian) and J^H.y (compute_jacobian_hermitian).
+This module is that caller loop, run on the HIP path:
+
+    per outer iteration
+      m, dm/dp       simulate_magnetization + simulate_magnetization_tangent
+                     (exact derivatives, one pass per parameter)
+      r = S(m) - y   magnetization_to_signal + compute_residual
+      g = Re J^H r   compute_jacobian_hermitian
+      (Re J^H J + lambda I) delta = -g    conjugate gradients, one J.v and one
+                                          J^H.y per iteration
+      p <- p (+) delta, halving the step while the objective does not decrease
+
+Parameterisation follows the products: the T1 and T2 rows of J are scaled by
+T1 and T2 (product_kernels.cuh:57-58), so their updates are relative
+(T <- T exp(delta)); B1 and B0 (spiral only) and rho_x, rho_y are absolute.
+Cartesian data use the four-parameter products (T1, T2, rho_x, rho_y); spiral
+data the six-parameter ones (T1, T2, B1, B0, rho_x, rho_y), where B1 and B0
+may be held fixed.  With phase_encode (Cartesian), echoes and derivatives pass
+through phase_encoding before the signal, as the reference's examples do.
+
+Vectors of the normal equations ([rows][nvoxels] real) live on the host; every
+product runs on the GPU.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import (B0, B1, T1, T2, CartesianTrajectory, CompasArray, SpiralTrajectory, TissueParameters,
+               as_array, compute_jacobian, compute_jacobian_hermitian, compute_jacobian_hermitian_spiral,
+               compute_jacobian_spiral, compute_residual, magnetization_to_signal, phase_encoding,
+               simulate_magnetization, simulate_magnetization_tangent)
+
+
+@dataclass
+class Estimate:
+    """Tissue parameters of a reconstruction (float32 arrays of nvoxels)."""
+    T1: np.ndarray
+    T2: np.ndarray
+    B1: np.ndarray
+    B0: np.ndarray
+    rho: np.ndarray  # complex64
+    x: np.ndarray
+    y: np.ndarray
+    z: Optional[np.ndarray] = None
+
+    def copy(self) -> "Estimate":
+        return Estimate(*(None if a is None else np.array(a, copy=True) for a in
+                          (self.T1, self.T2, self.B1, self.B0, self.rho, self.x, self.y, self.z)))
+
+    def parameters(self) -> TissueParameters:
+        return TissueParameters(self.T1.shape[0], self.T1, self.T2, self.B1, self.B0, self.rho.real,
+                                self.rho.imag, self.x, self.y, self.z)
+
+
+@dataclass
+class GaussNewtonResult:
+    estimate: Estimate
+    objective: List[float] = field(default_factory=list)  # before each iteration, then the final value
+    cg_iterations: List[int] = field(default_factory=list)
+    steps: List[float] = field(default_factory=list)
+
+
+class _Model:
+    """S(p), the derivatives at p and the products J.v / Re J^H.y."""
+
+    def __init__(self, sequence, trajectory, coils, method, signal_method, phase_encode, fit_b1, fit_b0):
+        self.sequence = sequence
+        self.trajectory = trajectory
+        self.coils = as_array(coils, np.complex64)
+        self.method = method
+        self.signal_method = signal_method
+        self.spiral = isinstance(trajectory, SpiralTrajectory)
+        if not self.spiral and not isinstance(trajectory, CartesianTrajectory):
+            raise TypeError("trajectory must be Cartesian or spiral")
+        self.phase_encode = phase_encode and not self.spiral
+        self.fit_b1, self.fit_b0 = fit_b1, fit_b0
+        if not self.spiral and (fit_b1 or fit_b0):
+            raise ValueError("the Cartesian products have no B1 / B0 rows")
+        self.rows = 6 if self.spiral else 4
+
+    def _encode(self, a, params):
+        return phase_encoding(a, params, self.trajectory) if self.phase_encode else a
+
+    def signal(self, est: Estimate):
+        params = est.parameters()
+        m = self._encode(simulate_magnetization(params, self.sequence), params)
+        return params, m, magnetization_to_signal(m, params, self.trajectory, self.coils, method=self.signal_method)
+
+    def linearise(self, est: Estimate, params, m):
+        fields = (T1, T2, B1, B0) if self.spiral else (T1, T2)
+        self.params = params
+        self.m = m
+        self.d = [self._encode(simulate_magnetization_tangent(f, params, self.sequence), params) for f in fields]
+
+    def jv(self, v: np.ndarray) -> CompasArray:
+        vc = as_array(np.ascontiguousarray(v, dtype=np.complex64))
+        if self.spiral:
+            return compute_jacobian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, vc)
+        return compute_jacobian(self.m, self.d, self.params, self.trajectory, self.coils, vc, method=self.method)
+
+    def jhy(self, y) -> np.ndarray:
+        if self.spiral:
+            out = compute_jacobian_hermitian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, y)
+        else:
+            out = compute_jacobian_hermitian(self.m, self.d, self.params, self.trajectory, self.coils, y,
+                                             method=self.method)
+        g = out.collect().real.astype(np.float64)
+        return self.mask(g)
+
+    def mask(self, g: np.ndarray) -> np.ndarray:
+        if self.spiral:
+            if not self.fit_b1:
+                g[2] = 0.0
+            if not self.fit_b0:
+                g[3] = 0.0
+        return g
+
+    def normal(self, v: np.ndarray) -> np.ndarray:
+        return self.jhy(self.jv(v))
+
+
+def _cg(apply, b, iterations, tol, shift_rel=0.0):
+    """Conjugate gradients for (A + shift I) x = b, A symmetric positive
+    semi-definite; shift = shift_rel times the Rayleigh quotient of b."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    rr = float(np.vdot(r, r).real)
+    b_norm = np.sqrt(rr)
+    if b_norm == 0.0:
+        return x, 0
+    shift = None
+    k = 0
+    for k in range(1, iterations + 1):
+        Ap = apply(p)
+        if shift is None:
+            shift = shift_rel * float(np.vdot(p, Ap).real) / rr
+        Ap = Ap + shift * p
+        pAp = float(np.vdot(p, Ap).real)
+        if pAp <= 0.0:
+            break
+        alpha = rr / pAp
+        x += alpha * p
+        r -= alpha * Ap
+        rr_new = float(np.vdot(r, r).real)
+        if np.sqrt(rr_new) <= tol * b_norm:
+            break
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+    return x, k
+
+
+def _apply_step(est: Estimate, delta: np.ndarray, step: float, spiral: bool, max_log_step: float) -> Estimate:
+    out = est.copy()
+    d = step * delta
+    out.T1 = (est.T1 * np.exp(np.clip(d[0], -max_log_step, max_log_step))).astype(np.float32)
+    out.T2 = (est.T2 * np.exp(np.clip(d[1], -max_log_step, max_log_step))).astype(np.float32)
+    if spiral:
+        out.B1 = (est.B1 + d[2]).astype(np.float32)
+        out.B0 = (est.B0 + d[3]).astype(np.float32)
+        rx, ry = d[4], d[5]
+    else:
+        rx, ry = d[2], d[3]
+    out.rho = (est.rho + (rx + 1j * ry)).astype(np.complex64)
+    return out
+
+
+def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, iterations: int = 8,
+                 cg_iterations: int = 25, cg_tol: float = 1e-4, damping: float = 1e-6,
+                 method: str = "gemm_fast", signal_method: str = "matmul_fast", phase_encode: bool = True,
+                 fit_b1: bool = False, fit_b0: bool = False, max_log_step: float = 0.7,
+                 min_step: float = 1.0 / 64) -> GaussNewtonResult:
+    """Minimise ||S(p) - measured||^2 over the tissue parameters, starting from
+    `initial`.  `measured` is [ncoils][nreadouts][samples] complex64 (host or
+    device array).  `damping` is the Levenberg shift relative to the normal
+    operator's Rayleigh quotient along the gradient."""
+    model = _Model(sequence, trajectory, coils, method, signal_method, phase_encode, fit_b1, fit_b0)
+    y = as_array(measured, np.complex64)
+    est = initial.copy()
+    result = GaussNewtonResult(est)
+    params, m, s = model.signal(est)
+    obj, r = compute_residual(s, y)
+    for _ in range(iterations):
+        result.objective.append(obj)
+        model.linearise(est, params, m)
+        g = model.jhy(r)
+        delta, k = _cg(model.normal, -g, cg_iterations, cg_tol, damping)
+        result.cg_iterations.append(k)
+        step = 1.0
+        while True:
+            cand = _apply_step(est, delta, step, model.spiral, max_log_step)
+            c_params, c_m, c_s = model.signal(cand)
+            c_obj, c_r = compute_residual(c_s, y)
+            if c_obj < obj or step <= min_step:
+                break
+            step *= 0.5
+        if not c_obj < obj:  # no decrease along the Gauss-Newton direction: converged
+            result.steps.append(0.0)
+            break
+        result.steps.append(step)
+        est, params, m, obj, r = cand, c_params, c_m, c_obj, c_r
+    result.estimate = est
+    result.objective.append(obj)
+    return result

@@ -29,26 +29,31 @@ struct Vec3d {
     double x, y, z;
 };
 
-__device__ inline void rotate_small(Vec3d& m, double ax, double ay, double az, double axy2) {
-    const double u = fma(az, az, axy2);
-    double s1, s2;
-    if (u <= 0.0625) {
-        // Taylor series of sin(t)/t and (1-cos t)/t^2 in u = t^2, |t| <= 1/4: six
-        // terms each (first omitted terms below 1e-17).  Summed over the powers
-        // of u with the running sum as the FMA addend, so every step is one
-        // v_fmac with the coefficient from a scalar register and both series
-        // share the powers.
-        const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
-        s1 = fma(u, -1.0 / 6.0, 1.0);
-        s1 = fma(u2, 1.0 / 120.0, s1);
-        s1 = fma(u3, -1.0 / 5040.0, s1);
-        s1 = fma(u4, 1.0 / 362880.0, s1);
-        s1 = fma(u5, -1.0 / 39916800.0, s1);
-        s2 = fma(u, -1.0 / 24.0, 0.5);
-        s2 = fma(u2, 1.0 / 720.0, s2);
-        s2 = fma(u3, -1.0 / 40320.0, s2);
-        s2 = fma(u4, 1.0 / 3628800.0, s2);
-        s2 = fma(u5, -1.0 / 479001600.0, s2);
+// Angles up to sqrt(SERIES_LIMIT) = 1/4 rad use the Taylor series.
+constexpr double SERIES_LIMIT = 0.0625;
+
+// s1 = sin(t)/t and s2 = (1 - cos t)/t^2 of t = sqrt(u), u <= SERIES_LIMIT:
+// Taylor series in u, six terms each (first omitted terms below 1e-17),
+// summed over the powers of u with the running sum as the FMA addend, so
+// every step is one v_fmac and both series share the powers.
+__device__ inline void rotation_series(double u, double& s1, double& s2) {
+    const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
+    s1 = fma(u, -1.0 / 6.0, 1.0);
+    s1 = fma(u2, 1.0 / 120.0, s1);
+    s1 = fma(u3, -1.0 / 5040.0, s1);
+    s1 = fma(u4, 1.0 / 362880.0, s1);
+    s1 = fma(u5, -1.0 / 39916800.0, s1);
+    s2 = fma(u, -1.0 / 24.0, 0.5);
+    s2 = fma(u2, 1.0 / 720.0, s2);
+    s2 = fma(u3, -1.0 / 40320.0, s2);
+    s2 = fma(u4, 1.0 / 3628800.0, s2);
+    s2 = fma(u5, -1.0 / 479001600.0, s2);
+}
+
+// the same for any u: the series up to SERIES_LIMIT, FP64 sincos above
+__device__ inline void rotation_any(double u, double& s1, double& s2) {
+    if (u <= SERIES_LIMIT) {
+        rotation_series(u, s1, s2);
     } else {
         const double t = sqrt(u);
         double s, c;
@@ -56,6 +61,10 @@ __device__ inline void rotate_small(Vec3d& m, double ax, double ay, double az, d
         s1 = s / t;
         s2 = (1.0 - c) / u;
     }
+}
+
+// m' = cos t m + s1 (a x m) + s2 (a . m) a, with cos t = 1 - u s2
+__device__ inline void apply_rotation(Vec3d& m, double ax, double ay, double az, double u, double s1, double s2) {
     const double c = fma(-u, s2, 1.0);
     const double cx = ay * m.z - az * m.y;
     const double cy = az * m.x - ax * m.z;

@@ -84,6 +84,21 @@ void spiral_sum(
 // Per (slice, TR) RF table of the FISP kernels: (deg2rad |RF|, cos phi, sin phi, 0).
 Array<float4, 1> fisp_rf_table(const CompasContext& ctx, const FISPSequence& seq);
 
+// pSSFP launch layout: ZPL slice positions per lane, G = ceil(nz / ZPL) wave
+// groups per block of 64 voxels, at most per_launch groups per launch (the
+// partial echoes of groups 1.. are held in one buffer of <= 4 GiB).
+// waves_per_cu[zpl] is the resident waves per CU of the ZPL instance.
+struct PssfpLayout {
+    int zpl, groups, per_launch;
+};
+PssfpLayout pssfp_layout(const CompasContext& ctx, int nz, int nv, int nTR, const int* waves_per_cu, int max_zpl);
+
+// device bytes of partial echoes one pSSFP simulation of this size allocates
+int64_t pssfp_scratch_bytes(const CompasContext& ctx, int nz, int nv, int nTR);
+
+// echos[i] += sum over count partial planes of n elements, in FP64, in order
+void pssfp_sum_groups(const CompasContext& ctx, cfloat* echos, const cfloat* partial, int count, int64_t n);
+
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);
 

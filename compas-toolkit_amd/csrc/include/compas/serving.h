@@ -70,7 +70,7 @@ class Server {
     void set_kernel_timing(bool on);
     std::map<std::string, KernelStats> collect_timings();
 
-    static size_t estimate_bytes(const ServeRequest& r);
+    size_t estimate_bytes(const ServeRequest& r) const;
     double now() const;
     int slots() const { return int(slots_.size()); }
 

@@ -4,11 +4,19 @@
 // and src/operators/isochromat.cuh:9-128.
 //
 // Design (MI355X):
-//   * A workgroup covers 64 voxels (one per lane) and G = ceil(nz / ZPL)
-//     waves; wave g simulates slice positions [g*ZPL, g*ZPL + ZPL) of its
-//     voxel, ZPL independent isochromats per lane (instruction-level
-//     parallelism).  The sum over z at each echo goes through LDS, one
-//     barrier per TR (double-buffered partials).
+//   * One wave per workgroup: 64 voxels (one per lane) times one group of ZPL
+//     slice positions, ZPL independent isochromats per lane (instruction-level
+//     parallelism).  Waves share nothing, so the hardware packs them onto the
+//     SIMDs freely; a workgroup of all G = ceil(nz / ZPL) groups of a voxel
+//     block would have to be co-resident (at 35 positions and ZPL 7: five
+//     waves, one workgroup per CU, one SIMD with two waves and three with one,
+//     and four rounds of 256 workgroups for 784 blocks, the last nearly empty).
+//   * Group 0 writes the echoes; groups 1..G-1 write FP32 partial echoes that
+//     a second pass adds in FP64, in group order (deterministic).  That costs
+//     (G-1) x 8 x nTR x nv bytes written and read once, about 1 ms at 35
+//     positions and 1120 x 50176, against the whole simulation's ~100 ms.
+//   * ZPL is chosen per launch from the resident waves per CU of each
+//     instance (occupancy API) and the rounds of waves it takes.
 //   * The isochromat state is FP64.  On MI355X, FP64 FMA issues at the same
 //     rate as non-packed FP32, and the FP32 form of this recursion drifts
 //     ~3e-5 (relative L2) from the exact result over 1000+ TRs because each TR
@@ -19,9 +27,13 @@
 //     < 1e-17), cos t = 1 - u s2(u), and
 //       m' = cos t m + s1 (a x m) + s2 (a . m) a,
 //     which equals the reference's normalised form.  Larger angles fall back
-//     to FP64 sincos.
+//     to FP64 sincos; the choice is made once per RF sub-step for the whole
+//     wave, so the common case is branch-free and the compiler interleaves
+//     the ZPL independent FP64 dependency chains.
 //   * The RF sub-pulse table is staged in LDS as doubles (broadcast reads in
 //     the sub-step loop instead of a global load per sub-step).
+
+#include <algorithm>
 
 #include "compas/device/common.h"
 #include "compas/device/pssfp_common.h"
@@ -30,16 +42,19 @@
 namespace compas {
 namespace kernels {
 
+// grid (ceil(nv / 64), groups in this launch), block 64.  Group g of the
+// launch is group0 + blockIdx.y; blockIdx.y == 0 writes out0, blockIdx.y = j
+// > 0 writes rest + (j - 1) * nTR * nv.
 template<int ZPL>
-__global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
-    __shared__ double2 partial[2][8][64];
+__global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat* out0, cfloat* rest) {
     __shared__ double2 grf[RF_CHUNK];  // gamma_dt_RF as doubles (LDS broadcast reads)
-    const int lane = threadIdx.x & 63;
-    const int group = threadIdx.x >> 6;
-    const int groups = blockDim.x >> 6;
+    const int lane = threadIdx.x;
+    const int group = group0 + int(blockIdx.y);
     const int voxel = blockIdx.x * 64 + lane;
     const bool active = voxel < a.tissue.nvoxels;
     const int vx = active ? voxel : 0;
+    const size_t plane = size_t(a.nTR) * size_t(a.tissue.nvoxels);
+    cfloat* out = (blockIdx.y == 0) ? out0 : rest + size_t(blockIdx.y - 1) * plane;
 
     const double T1 = a.tissue.T1(vx), T2 = a.tissue.T2(vx);
     const double B1 = a.tissue.B1(vx), B0 = a.tissue.B0(vx);
@@ -66,7 +81,7 @@ __global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
     }
 
     auto stage_rf = [&](int k0) {
-        for (int k = threadIdx.x; k < RF_CHUNK && k0 + k < a.nRF; k += blockDim.x) {
+        for (int k = lane; k < RF_CHUNK && k0 + k < a.nRF; k += 64) {
             const cfloat g = a.gamma_dt_RF[k0 + k];
             grf[k] = make_double2(double(g.re), double(g.im));
         }
@@ -75,8 +90,6 @@ __global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
     if (rf_resident) stage_rf(0);
     __syncthreads();
 
-    // every thread of the block runs the same excitations, so the barriers of a
-    // chunked RF table are uniform
     auto excite = [&](double tre, double tim) {
         for (int k0 = 0; k0 < a.nRF; k0 += RF_CHUNK) {
             if (!rf_resident) {
@@ -91,10 +104,31 @@ __global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
                 const double zim = tre * g.y + tim * g.x;
                 const double ax = zre * B1, ay = -zim * B1;
                 const double axy2 = fma(ax, ax, ay * ay);
+                double u[ZPL];
+                bool large = false;
 #pragma unroll
                 for (int i = 0; i < ZPL; i++) {
-                    rotate_small(m[i], ax, ay, az[i], axy2);
-                    relax3(m[i], E1_ex, E2_ex, R1_ex);
+                    u[i] = fma(az[i], az[i], axy2);
+                    large = large || (u[i] > SERIES_LIMIT);
+                }
+                // wave-uniform choice: the common all-small case is straight-line
+                // code over the ZPL independent isochromats
+                if (__any(large)) {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        double s1, s2;
+                        rotation_any(u[i], s1, s2);
+                        apply_rotation(m[i], ax, ay, az[i], u[i], s1, s2);
+                        relax3(m[i], E1_ex, E2_ex, R1_ex);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        double s1, s2;
+                        rotation_series(u[i], s1, s2);
+                        apply_rotation(m[i], ax, ay, az[i], u[i], s1, s2);
+                        relax3(m[i], E1_ex, E2_ex, R1_ex);
+                    }
                 }
             }
         }
@@ -128,17 +162,7 @@ __global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
                 sy += m[i].y;
             }
         }
-        partial[n & 1][group][lane] = make_double2(sx, sy);
-        __syncthreads();
-        if (group == 0 && active) {
-            double tx = 0.0, ty = 0.0;
-            for (int g = 0; g < groups; g++) {
-                const double2 p = partial[n & 1][g][lane];
-                tx += p.x;
-                ty += p.y;
-            }
-            a.echos[size_t(n) * size_t(a.tissue.nvoxels) + voxel] = {float(tx), float(ty)};
-        }
+        if (active) out[size_t(n) * size_t(a.tissue.nvoxels) + voxel] = {float(sx), float(sy)};
 #pragma unroll
         for (int i = 0; i < ZPL; i++) {
             rotate_z(m[i], cpr[i], spr[i]);
@@ -147,16 +171,71 @@ __global__ void __launch_bounds__(512) pssfp_iso(PssfpArgs a) {
     }
 }
 
-}  // namespace kernels
-
-using PssfpLauncher = void (*)(kernels::PssfpArgs, unsigned, unsigned, hipStream_t);
-
-template<int ZPL>
-static void launch_pssfp(kernels::PssfpArgs a, unsigned blocks, unsigned threads, hipStream_t s) {
-    kernels::pssfp_iso<ZPL><<<blocks, threads, 0, s>>>(a);
+// echos[i] += sum_j partial[j][i], accumulated in FP64 in slot order
+__global__ void pssfp_sum_groups_kernel(
+    cfloat* __restrict__ echos, const cfloat* __restrict__ partial, int count, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const cfloat e = echos[i];
+        double re = e.re, im = e.im;
+        for (int j = 0; j < count; j++) {
+            const cfloat p = partial[size_t(j) * size_t(n) + size_t(i)];
+            re += p.re;
+            im += p.im;
+        }
+        echos[i] = {float(re), float(im)};
+    }
 }
 
-static PssfpLauncher pssfp_launcher(int zpl) {
+}  // namespace kernels
+
+void pssfp_sum_groups(const CompasContext& ctx, cfloat* echos, const cfloat* partial, int count, int64_t n) {
+    if (count <= 0 || n == 0) return;
+    KernelScope ks(ctx, "pssfp_sum_groups");
+    const int64_t blocks = std::min<int64_t>(dev::div_ceil(n, int64_t(256)), 8192);
+    kernels::pssfp_sum_groups_kernel<<<unsigned(blocks), 256, 0, ctx.stream()>>>(echos, partial, count, n);
+    check_launch("pssfp_sum_groups");
+}
+
+PssfpLayout pssfp_layout(const CompasContext& ctx, int nz, int nv, int nTR, const int* waves_per_cu, int max_zpl) {
+    hipDeviceProp_t prop;
+    COMPAS_HIP_CHECK(hipGetDeviceProperties(&prop, ctx.device()));
+    const int64_t cus = prop.multiProcessorCount;
+    const int64_t blocks = dev::div_ceil(int64_t(nv), int64_t(64));
+    // partial echoes held at once: at most 4 GiB, at least one group
+    const int64_t plane = int64_t(8) * nTR * std::max(nv, 1);
+    const int cap = int(std::max<int64_t>(1, std::min<int64_t>(1 << 15, (int64_t(4) << 30) / plane)));
+    PssfpLayout best {1, nz, 1};
+    double best_cost = 1e300;
+    for (int zpl = max_zpl; zpl >= 1; zpl--) {
+        const int wpc = std::max(1, waves_per_cu[zpl]);
+        const int groups = (nz + zpl - 1) / zpl;
+        const int per_launch = std::min(groups, cap + 1);  // launch 0 also writes the echoes
+        // every wave does the same work, issue-bound at (zpl * 50 + 12) FP64
+        // instructions per RF sub-step; wpc / 4 waves share a SIMD
+        double cost = 0.0;
+        for (int g0 = 0; g0 < groups; g0 += per_launch) {
+            const int64_t waves = blocks * std::min(per_launch, groups - g0);
+            const int64_t rounds = dev::div_ceil(waves, cus * wpc);
+            cost += double(rounds) * (wpc / 4.0) * (zpl * 50.0 + 12.0);
+        }
+        if (cost < best_cost * 0.999) {
+            best_cost = cost;
+            best = {zpl, groups, per_launch};
+        }
+    }
+    return best;
+}
+
+namespace {
+
+using PssfpLauncher = void (*)(kernels::PssfpArgs, int, cfloat*, cfloat*, dim3, hipStream_t);
+
+template<int ZPL>
+void launch_pssfp(kernels::PssfpArgs a, int group0, cfloat* out0, cfloat* rest, dim3 grid, hipStream_t s) {
+    kernels::pssfp_iso<ZPL><<<grid, 64, 0, s>>>(a, group0, out0, rest);
+}
+
+PssfpLauncher pssfp_launcher(int zpl) {
     switch (zpl) {
         case 1: return &launch_pssfp<1>;
         case 2: return &launch_pssfp<2>;
@@ -164,26 +243,41 @@ static PssfpLauncher pssfp_launcher(int zpl) {
         case 4: return &launch_pssfp<4>;
         case 5: return &launch_pssfp<5>;
         case 6: return &launch_pssfp<6>;
-        case 7: return &launch_pssfp<7>;
-        default: return nullptr;
+        default: return &launch_pssfp<7>;  // 8 per lane exceeds 256 VGPRs
     }
 }
 
-// Slice positions per lane: the smallest ZPL in 1..8 with at most 8 waves
-// per workgroup and the least padding, preferring 4..8 isochromats per lane.
-static int choose_zpl(int nz) {
-    int best = 0, best_pad = 1 << 30;
-    for (int zpl = 7; zpl >= 1; zpl--) {  // 8 per lane exceeds 256 VGPRs
-        int g = (nz + zpl - 1) / zpl;
-        if (g > 8) continue;
-        int pad = g * zpl - nz;
-        if (zpl < 4 && nz >= 4) pad += 1;  // fewer isochromats per lane cost per-lane overhead
-        if (pad < best_pad) {
-            best_pad = pad;
-            best = zpl;
-        }
-    }
-    return best;
+template<int ZPL>
+int resident_waves() {
+    static const int cached = [] {
+        int n = 0;
+        COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernels::pssfp_iso<ZPL>, 64, 0));
+        return std::max(1, n);
+    }();
+    return cached;
+}
+
+}  // namespace
+
+static PssfpLayout primal_layout(const CompasContext& ctx, int nz, int nv, int nTR) {
+    const int waves[8] = {0,
+                          resident_waves<1>(),
+                          resident_waves<2>(),
+                          resident_waves<3>(),
+                          resident_waves<4>(),
+                          resident_waves<5>(),
+                          resident_waves<6>(),
+                          resident_waves<7>()};
+    return pssfp_layout(ctx, nz, nv, nTR, waves, 7);
+}
+
+static int partial_slots(const PssfpLayout& L) {
+    return L.groups > L.per_launch ? L.per_launch : L.groups - 1;
+}
+
+int64_t pssfp_scratch_bytes(const CompasContext& ctx, int nz, int nv, int nTR) {
+    if (nz <= 0 || nv <= 0 || nTR <= 0) return 0;
+    return int64_t(partial_slots(primal_layout(ctx, nz, nv, nTR))) * nTR * nv * int64_t(sizeof(cfloat));
 }
 
 Array<cfloat, 2> simulate_magnetization(
@@ -206,6 +300,8 @@ Array<cfloat, 2> simulate_magnetization(
     a.tissue = dev::tissue_view(parameters);
     a.RF = seq.RF_train.data();
     a.gamma_dt_RF = seq.gamma_dt_RF.data();
+    a.z = seq.z.data();
+    a.nz = nz;
     a.nTR = nTR;
     a.nRF = int(seq.gamma_dt_RF.size(0));
     a.dt_ex = seq.dt.ex;
@@ -215,26 +311,23 @@ Array<cfloat, 2> simulate_magnetization(
     a.g_pr = seq.gamma_dt_GRz.pr;
     a.echos = echos.data();
 
-    // more than 56 slice positions: passes of <= 56 accumulate into a scratch
-    // sum (rare; the reference supports any nz by batching launches)
-    const int chunk = 56;  // 8 waves x 7 isochromats per lane
+    const PssfpLayout L = primal_layout(ctx, nz, nv, nTR);
+    const int64_t plane = int64_t(nTR) * nv;
+    const int slots = partial_slots(L);
     Array<cfloat, 2> part;
-    for (int z0 = 0; z0 < nz; z0 += chunk) {
-        const int nzc = std::min(chunk, nz - z0);
-        a.z = seq.z.data() + z0;
-        a.nz = nzc;
-        const int zpl = choose_zpl(nzc);
-        const int groups = (nzc + zpl - 1) / zpl;
-        if (z0 > 0) {
-            if (part.is_empty()) part = ctx.empty<cfloat, 2>({nTR, nv});
-            a.echos = part.data();
-        }
+    if (slots > 0) part = ctx.empty<cfloat, 2>({slots, plane});
+    const PssfpLauncher fn = pssfp_launcher(L.zpl);
+    const unsigned vblocks = unsigned(dev::div_ceil(nv, 64));
+    for (int g0 = 0; g0 < L.groups; g0 += L.per_launch) {
+        const int count = std::min(L.per_launch, L.groups - g0);
+        cfloat* out0 = g0 == 0 ? echos.data() : part.data();
+        cfloat* rest = g0 == 0 ? part.data() : part.data() + plane;
         {
             KernelScope ks(ctx, "pssfp_iso");
-            pssfp_launcher(zpl)(a, dev::div_ceil(nv, 64), unsigned(64 * groups), ctx.stream());
+            fn(a, g0, out0, rest, dim3(vblocks, unsigned(count)), ctx.stream());
             check_launch("pssfp_iso");
         }
-        if (z0 > 0) add_into(ctx, echos.data(), part.data(), int64_t(nTR) * nv);
+        pssfp_sum_groups(ctx, echos.data(), part.data(), g0 == 0 ? count - 1 : count, plane);
     }
     return echos;
 }

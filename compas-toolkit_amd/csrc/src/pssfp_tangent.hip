@@ -18,8 +18,10 @@
 // s1' = (c - s1) / (2u), s2' = (s1 - 2 s2) / (2u).
 // The result is the exact derivative of the discrete recursion (no step).
 //
-// Layout: as the primal kernel (64 voxels per workgroup, one wave per group
-// of ZPL slice positions, sum over z through LDS once per TR).
+// Layout: as the primal kernel (one wave per block of 64 voxels and group of
+// ZPL <= 4 slice positions, partial sums over z added by a second pass).
+
+#include <algorithm>
 
 #include "compas/device/common.h"
 #include "compas/device/pssfp_common.h"
@@ -30,52 +32,43 @@ namespace kernels {
 
 enum { PTAN_T1 = 0, PTAN_T2 = 1, PTAN_B1 = 2, PTAN_B0 = 3 };
 
+// s1'(u), s2'(u) from the Taylor series (term-wise derivatives, one more
+// term each; first omitted terms below 1e-18), u <= SERIES_LIMIT
+__device__ inline void rotation_series_derivative(double u, double& s1p, double& s2p) {
+    const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
+    s1p = fma(u, 2.0 / 120.0, -1.0 / 6.0);
+    s1p = fma(u2, -3.0 / 5040.0, s1p);
+    s1p = fma(u3, 4.0 / 362880.0, s1p);
+    s1p = fma(u4, -5.0 / 39916800.0, s1p);
+    s1p = fma(u5, 6.0 / 6227020800.0, s1p);
+    s2p = fma(u, 2.0 / 720.0, -1.0 / 24.0);
+    s2p = fma(u2, -3.0 / 40320.0, s2p);
+    s2p = fma(u3, 4.0 / 3628800.0, s2p);
+    s2p = fma(u4, -5.0 / 479001600.0, s2p);
+    s2p = fma(u5, 6.0 / 87178291200.0, s2p);
+}
+
 // Rot(a) m and its tangent: Rot(a) dm, plus (dRot/da . da) m when AXIS.
-template<bool AXIS>
+// ANY: u may exceed SERIES_LIMIT (closed forms there); else the series only.
+template<bool AXIS, bool ANY>
 __device__ inline void rotate_tangent(
     Vec3d& m,
     Vec3d& dm,
     double ax,
     double ay,
     double az,
-    double axy2,
+    double u,
     double dax,
     double day,
     double daz) {
-    const double u = fma(az, az, axy2);
     double s1, s2, s1p = 0.0, s2p = 0.0;
-    if (u <= 0.0625) {
-        const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
-        s1 = fma(u, -1.0 / 6.0, 1.0);
-        s1 = fma(u2, 1.0 / 120.0, s1);
-        s1 = fma(u3, -1.0 / 5040.0, s1);
-        s1 = fma(u4, 1.0 / 362880.0, s1);
-        s1 = fma(u5, -1.0 / 39916800.0, s1);
-        s2 = fma(u, -1.0 / 24.0, 0.5);
-        s2 = fma(u2, 1.0 / 720.0, s2);
-        s2 = fma(u3, -1.0 / 40320.0, s2);
-        s2 = fma(u4, 1.0 / 3628800.0, s2);
-        s2 = fma(u5, -1.0 / 479001600.0, s2);
-        if constexpr (AXIS) {
-            // term-wise derivatives, one more term each (first omitted < 1e-18)
-            s1p = fma(u, 2.0 / 120.0, -1.0 / 6.0);
-            s1p = fma(u2, -3.0 / 5040.0, s1p);
-            s1p = fma(u3, 4.0 / 362880.0, s1p);
-            s1p = fma(u4, -5.0 / 39916800.0, s1p);
-            s1p = fma(u5, 6.0 / 6227020800.0, s1p);
-            s2p = fma(u, 2.0 / 720.0, -1.0 / 24.0);
-            s2p = fma(u2, -3.0 / 40320.0, s2p);
-            s2p = fma(u3, 4.0 / 3628800.0, s2p);
-            s2p = fma(u4, -5.0 / 479001600.0, s2p);
-            s2p = fma(u5, 6.0 / 87178291200.0, s2p);
-        }
+    if (!ANY || u <= SERIES_LIMIT) {
+        rotation_series(u, s1, s2);
+        if constexpr (AXIS) rotation_series_derivative(u, s1p, s2p);
     } else {
-        const double t = sqrt(u);
-        double s, c;
-        sincos(t, &s, &c);
-        s1 = s / t;
-        s2 = (1.0 - c) / u;
+        rotation_any(u, s1, s2);
         if constexpr (AXIS) {
+            const double c = fma(-u, s2, 1.0);
             s1p = (c - s1) / (2.0 * u);
             s2p = (s1 - 2.0 * s2) / (2.0 * u);
         }
@@ -145,16 +138,19 @@ __device__ inline void rotate_z_tangent(Vec3d& m, Vec3d& dm, double c, double s,
     }
 }
 
+// grid (ceil(nv / 64), groups in this launch), block 64: one wave per voxel
+// block and group of ZPL slice positions, as the primal kernel; blockIdx.y
+// == 0 writes out0, blockIdx.y = j > 0 writes rest + (j - 1) * nTR * nv.
 template<int ZPL, int FIELD>
-__global__ void __launch_bounds__(512) pssfp_iso_tangent(PssfpArgs a, cfloat* __restrict__ dechos) {
-    __shared__ double2 partial[2][8][64];
+__global__ void __launch_bounds__(64) pssfp_iso_tangent(PssfpArgs a, int group0, cfloat* out0, cfloat* rest) {
     __shared__ double2 grf[RF_CHUNK];
-    const int lane = threadIdx.x & 63;
-    const int group = threadIdx.x >> 6;
-    const int groups = blockDim.x >> 6;
+    const int lane = threadIdx.x;
+    const int group = group0 + int(blockIdx.y);
     const int voxel = blockIdx.x * 64 + lane;
     const bool active = voxel < a.tissue.nvoxels;
     const int vx = active ? voxel : 0;
+    const size_t plane = size_t(a.nTR) * size_t(a.tissue.nvoxels);
+    cfloat* out = (blockIdx.y == 0) ? out0 : rest + size_t(blockIdx.y - 1) * plane;
 
     const double T1 = a.tissue.T1(vx), T2 = a.tissue.T2(vx);
     const double B1 = a.tissue.B1(vx), B0 = a.tissue.B0(vx);
@@ -198,7 +194,7 @@ __global__ void __launch_bounds__(512) pssfp_iso_tangent(PssfpArgs a, cfloat* __
     }
 
     auto stage_rf = [&](int k0) {
-        for (int k = threadIdx.x; k < RF_CHUNK && k0 + k < a.nRF; k += blockDim.x) {
+        for (int k = lane; k < RF_CHUNK && k0 + k < a.nRF; k += 64) {
             const cfloat g = a.gamma_dt_RF[k0 + k];
             grf[k] = make_double2(double(g.re), double(g.im));
         }
@@ -225,10 +221,25 @@ __global__ void __launch_bounds__(512) pssfp_iso_tangent(PssfpArgs a, cfloat* __
                 const double dax = (FIELD == PTAN_B1) ? zre : 0.0;
                 const double day = (FIELD == PTAN_B1) ? -zim : 0.0;
                 const double dz = (FIELD == PTAN_B0) ? daz : 0.0;
+                double u[ZPL];
+                bool large = false;
 #pragma unroll
                 for (int i = 0; i < ZPL; i++) {
-                    rotate_tangent<AXIS>(m[i], dm[i], ax, ay, az[i], axy2, dax, day, dz);
-                    relax_tangent<FIELD>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE_ex);
+                    u[i] = fma(az[i], az[i], axy2);
+                    large = large || (u[i] > SERIES_LIMIT);
+                }
+                if (__any(large)) {  // wave-uniform: the all-small case is branch-free
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        rotate_tangent<AXIS, true>(m[i], dm[i], ax, ay, az[i], u[i], dax, day, dz);
+                        relax_tangent<FIELD>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE_ex);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        rotate_tangent<AXIS, false>(m[i], dm[i], ax, ay, az[i], u[i], dax, day, dz);
+                        relax_tangent<FIELD>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE_ex);
+                    }
                 }
             }
         }
@@ -262,17 +273,7 @@ __global__ void __launch_bounds__(512) pssfp_iso_tangent(PssfpArgs a, cfloat* __
                 sy += dm[i].y;
             }
         }
-        partial[n & 1][group][lane] = make_double2(sx, sy);
-        __syncthreads();
-        if (group == 0 && active) {
-            double tx = 0.0, ty = 0.0;
-            for (int g = 0; g < groups; g++) {
-                const double2 p = partial[n & 1][g][lane];
-                tx += p.x;
-                ty += p.y;
-            }
-            dechos[size_t(n) * size_t(a.tissue.nvoxels) + voxel] = {float(tx), float(ty)};
-        }
+        if (active) out[size_t(n) * size_t(a.tissue.nvoxels) + voxel] = {float(sx), float(sy)};
 #pragma unroll
         for (int i = 0; i < ZPL; i++) {
             rotate_z_tangent<FIELD>(m[i], dm[i], cpr[i], spr[i], dth);
@@ -285,11 +286,11 @@ __global__ void __launch_bounds__(512) pssfp_iso_tangent(PssfpArgs a, cfloat* __
 
 namespace {
 
-using PssfpTangentLauncher = void (*)(kernels::PssfpArgs, cfloat*, unsigned, unsigned, hipStream_t);
+using PssfpTangentLauncher = void (*)(kernels::PssfpArgs, int, cfloat*, cfloat*, dim3, hipStream_t);
 
 template<int ZPL, int FIELD>
-void launch_pssfp_tangent(kernels::PssfpArgs a, cfloat* d, unsigned blocks, unsigned threads, hipStream_t s) {
-    kernels::pssfp_iso_tangent<ZPL, FIELD><<<blocks, threads, 0, s>>>(a, d);
+void launch_pssfp_tangent(kernels::PssfpArgs a, int group0, cfloat* out0, cfloat* rest, dim3 grid, hipStream_t s) {
+    kernels::pssfp_iso_tangent<ZPL, FIELD><<<grid, 64, 0, s>>>(a, group0, out0, rest);
 }
 
 template<int FIELD>
@@ -298,24 +299,29 @@ PssfpTangentLauncher pssfp_tangent_launcher(int zpl) {
         case 1: return &launch_pssfp_tangent<1, FIELD>;
         case 2: return &launch_pssfp_tangent<2, FIELD>;
         case 3: return &launch_pssfp_tangent<3, FIELD>;
-        default: return &launch_pssfp_tangent<4, FIELD>;
+        default: return &launch_pssfp_tangent<4, FIELD>;  // primal + tangent of 4 fit the registers
     }
 }
 
-// Slice positions per lane (1..4, primal + tangent state of four isochromats
-// fit the register file) with at most 8 waves per workgroup: the least padding.
-int choose_tangent_zpl(int nz) {
-    int best = 0, best_pad = 1 << 30;
-    for (int zpl = 4; zpl >= 1; zpl--) {
-        const int g = (nz + zpl - 1) / zpl;
-        if (g > 8) continue;
-        const int pad = g * zpl - nz;
-        if (pad < best_pad) {
-            best_pad = pad;
-            best = zpl;
-        }
-    }
-    return best;
+template<int ZPL, int FIELD>
+int resident_waves() {
+    static const int cached = [] {
+        int n = 0;
+        COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernels::pssfp_iso_tangent<ZPL, FIELD>, 64, 0));
+        return std::max(1, n);
+    }();
+    return cached;
+}
+
+template<int FIELD>
+void tangent_layout(const CompasContext& ctx, int nz, int nv, int nTR, PssfpLayout* L, PssfpTangentLauncher* fn) {
+    const int waves[5] = {0,
+                          resident_waves<1, FIELD>(),
+                          resident_waves<2, FIELD>(),
+                          resident_waves<3, FIELD>(),
+                          resident_waves<4, FIELD>()};
+    *L = pssfp_layout(ctx, nz, nv, nTR, waves, 4);
+    *fn = pssfp_tangent_launcher<FIELD>(L->zpl);
 }
 
 }  // namespace
@@ -344,6 +350,8 @@ Array<cfloat, 2> simulate_magnetization_tangent(
     a.tissue = dev::tissue_view(parameters);
     a.RF = seq.RF_train.data();
     a.gamma_dt_RF = seq.gamma_dt_RF.data();
+    a.z = seq.z.data();
+    a.nz = nz;
     a.nTR = nTR;
     a.nRF = int(seq.gamma_dt_RF.size(0));
     a.dt_ex = seq.dt.ex;
@@ -353,33 +361,29 @@ Array<cfloat, 2> simulate_magnetization_tangent(
     a.g_pr = seq.gamma_dt_GRz.pr;
     a.echos = nullptr;
 
-    // more than 32 slice positions: passes of <= 32 accumulate into a scratch sum
-    const int chunk = 32;  // 8 waves x 4 isochromats per lane
+    PssfpLayout L;
+    PssfpTangentLauncher fn;
+    switch (field) {
+        case TissueParameterField::T1: tangent_layout<kernels::PTAN_T1>(ctx, nz, nv, nTR, &L, &fn); break;
+        case TissueParameterField::T2: tangent_layout<kernels::PTAN_T2>(ctx, nz, nv, nTR, &L, &fn); break;
+        case TissueParameterField::B1: tangent_layout<kernels::PTAN_B1>(ctx, nz, nv, nTR, &L, &fn); break;
+        default: tangent_layout<kernels::PTAN_B0>(ctx, nz, nv, nTR, &L, &fn); break;
+    }
+    const int64_t plane = int64_t(nTR) * nv;
+    const int slots = L.groups > L.per_launch ? L.per_launch : L.groups - 1;
     Array<cfloat, 2> part;
-    for (int z0 = 0; z0 < nz; z0 += chunk) {
-        const int nzc = std::min(chunk, nz - z0);
-        a.z = seq.z.data() + z0;
-        a.nz = nzc;
-        const int zpl = choose_tangent_zpl(nzc);
-        const int groups = (nzc + zpl - 1) / zpl;
-        cfloat* out = d.data();
-        if (z0 > 0) {
-            if (part.is_empty()) part = ctx.empty<cfloat, 2>({nTR, nv});
-            out = part.data();
-        }
-        PssfpTangentLauncher fn;
-        switch (field) {
-            case TissueParameterField::T1: fn = pssfp_tangent_launcher<kernels::PTAN_T1>(zpl); break;
-            case TissueParameterField::T2: fn = pssfp_tangent_launcher<kernels::PTAN_T2>(zpl); break;
-            case TissueParameterField::B1: fn = pssfp_tangent_launcher<kernels::PTAN_B1>(zpl); break;
-            default: fn = pssfp_tangent_launcher<kernels::PTAN_B0>(zpl); break;
-        }
+    if (slots > 0) part = ctx.empty<cfloat, 2>({slots, plane});
+    const unsigned vblocks = unsigned(dev::div_ceil(nv, 64));
+    for (int g0 = 0; g0 < L.groups; g0 += L.per_launch) {
+        const int count = std::min(L.per_launch, L.groups - g0);
+        cfloat* out0 = g0 == 0 ? d.data() : part.data();
+        cfloat* rest = g0 == 0 ? part.data() : part.data() + plane;
         {
             KernelScope ks(ctx, "pssfp_iso_tangent");
-            fn(a, out, dev::div_ceil(nv, 64), unsigned(64 * groups), ctx.stream());
+            fn(a, g0, out0, rest, dim3(vblocks, unsigned(count)), ctx.stream());
             check_launch("pssfp_iso_tangent");
         }
-        if (z0 > 0) add_into(ctx, d.data(), part.data(), int64_t(nTR) * nv);
+        pssfp_sum_groups(ctx, d.data(), part.data(), g0 == 0 ? count - 1 : count, plane);
     }
     return d;
 }

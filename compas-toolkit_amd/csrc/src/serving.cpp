@@ -59,12 +59,16 @@ double Server::now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
 }
 
-size_t Server::estimate_bytes(const ServeRequest& r) {
+size_t Server::estimate_bytes(const ServeRequest& r) const {
     const auto& t = trajectory_of(r);
     const size_t nv = size_t(r.parameters.nvoxels), nr = size_t(t.nreadouts), ns = size_t(t.samples_per_readout);
     const size_t nc = size_t(r.coils.size(0));
     const size_t echo = 8 * nr * nv;
     size_t bytes = echo + (size_t(16) << 20);
+    if (r.pssfp) {  // partial echoes of the slice-position groups
+        bytes += size_t(pssfp_scratch_bytes(slots_.front(), int(r.pssfp->z.size(0)), int(nv),
+                                            int(r.pssfp->RF_train.size(0))));
+    }
     if (r.kind == RequestKind::SimulateSignal) {
         bytes += 8 * nc * nr * ns * 17;  // signal + split-K partials
         bytes += r.cartesian ? 8 * nc * ns * nv : echo;  // operand W, or the spiral amplitudes

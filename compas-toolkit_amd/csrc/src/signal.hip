@@ -13,11 +13,8 @@
 // reduced to [-pi, pi) before the float sincos.
 //
 // Spiral:  S[c][r][s] = sum_v A[r][v] C[c][v] E_rv^s with per-readout
-// exponents, evaluated by a direct kernel: FP64-anchored powers E^s0 every 16
+// exponents, evaluated by a direct kernel: FP64-anchored powers E^s0 every 8
 // samples, complex recurrence in between.
-
-#include <cstdlib>
-#include <type_traits>
 
 #include "compas/device/common.h"
 #include "compas/internal.h"
@@ -228,34 +225,19 @@ void spiral_sum(
     int nc, const SpiralTrajectory& t, cfloat* out) {
     const int nr = t.nreadouts, ns = t.samples_per_readout;
     if (nc == 0 || nr == 0 || ns == 0) return;
-    // samples per thread: 8 measured fastest at 4 coils x 1120 x 512 over
-    // 50176 voxels (24.0 ms against 30.8 ms for 16: more waves resident);
-    // COMPAS_SPIRAL_SAMPLES=4 or 16 selects another instance
-    static const int sps = [] {
-        const char* e = std::getenv("COMPAS_SPIRAL_SAMPLES");
-        const int v = e ? std::atoi(e) : 8;
-        return (v == 4 || v == 16) ? v : 8;
-    }();
+    // 8 samples per thread: measured fastest at 4 coils x 1120 x 512 over 50176
+    // voxels (24.7 ms, against 35.1 ms for 4 and 30.8 ms for 16, which needs
+    // 256 VGPRs and leaves one wave per SIMD)
+    constexpr int SPS = 8;
     KernelScope ks(ctx, "signal_spiral_direct");
-    dim3 grid(nr, dev::div_ceil(ns, 8 * sps), dev::div_ceil(nc, kernels::SP_COILS));
+    dim3 grid(nr, dev::div_ceil(ns, 8 * SPS), dev::div_ceil(nc, kernels::SP_COILS));
     const auto tv = dev::tissue_view(p);
-    auto launch = [&](auto two, auto samples) {
-        constexpr bool TWO = decltype(two)::value;
-        constexpr int SPS = decltype(samples)::value;
-        kernels::signal_spiral_direct<TWO, SPS><<<grid, 256, 0, ctx.stream()>>>(
-            tv, A, TWO ? Ab : nullptr, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
-    };
-    using I4 = std::integral_constant<int, 4>;
-    using I8 = std::integral_constant<int, 8>;
-    using I16 = std::integral_constant<int, 16>;
     if (Ab) {
-        if (sps == 4) launch(std::true_type {}, I4 {});
-        else if (sps == 16) launch(std::true_type {}, I16 {});
-        else launch(std::true_type {}, I8 {});
+        kernels::signal_spiral_direct<true, SPS><<<grid, 256, 0, ctx.stream()>>>(
+            tv, A, Ab, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
     } else {
-        if (sps == 4) launch(std::false_type {}, I4 {});
-        else if (sps == 16) launch(std::false_type {}, I16 {});
-        else launch(std::false_type {}, I8 {});
+        kernels::signal_spiral_direct<false, SPS><<<grid, 256, 0, ctx.stream()>>>(
+            tv, A, nullptr, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, out);
     }
     check_launch("signal_spiral_direct");
 }
