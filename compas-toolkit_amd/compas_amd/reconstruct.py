@@ -175,7 +175,7 @@ def _apply_step(est: Estimate, delta: np.ndarray, step: float, spiral: bool, max
 
 
 def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, iterations: int = 8,
-                 cg_iterations: int = 25, cg_tol: float = 1e-4, damping: float = 1e-6,
+                 cg_iterations: int = 100, cg_tol: float = 1e-6, damping: float = 1e-6,
                  method: str = "gemm_fast", signal_method: str = "matmul_fast", phase_encode: bool = True,
                  fit_b1: bool = False, fit_b0: bool = False, max_log_step: float = 0.7,
                  min_step: float = 1.0 / 64) -> GaussNewtonResult:

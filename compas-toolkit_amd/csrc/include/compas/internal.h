@@ -86,7 +86,7 @@ Array<float4, 1> fisp_rf_table(const CompasContext& ctx, const FISPSequence& seq
 
 // pSSFP launch layout: ZPL slice positions per lane, G = ceil(nz / ZPL) wave
 // groups per block of 64 voxels, at most per_launch groups per launch (the
-// partial echoes of groups 1.. are held in one buffer of <= 4 GiB).
+// partial echoes of groups 1.. are held in one buffer of <= 8 GiB).
 // waves_per_cu[zpl] is the resident waves per CU of the ZPL instance.
 struct PssfpLayout {
     int zpl, groups, per_launch;

@@ -13,8 +13,8 @@
 //     and four rounds of 256 workgroups for 784 blocks, the last nearly empty).
 //   * Group 0 writes the echoes; groups 1..G-1 write FP32 partial echoes that
 //     a second pass adds in FP64, in group order (deterministic).  That costs
-//     (G-1) x 8 x nTR x nv bytes written and read once, about 1 ms at 35
-//     positions and 1120 x 50176, against the whole simulation's ~100 ms.
+//     (G-1) x 8 x nTR x nv bytes written and read once: 0.5 ms for the sum
+//     at 35 positions and 1120 x 50176, against 73 ms for the simulation.
 //   * ZPL is chosen per launch from the resident waves per CU of each
 //     instance (occupancy API) and the rounds of waves it takes.
 //   * The isochromat state is FP64.  On MI355X, FP64 FMA issues at the same
@@ -201,15 +201,16 @@ PssfpLayout pssfp_layout(const CompasContext& ctx, int nz, int nv, int nTR, cons
     COMPAS_HIP_CHECK(hipGetDeviceProperties(&prop, ctx.device()));
     const int64_t cus = prop.multiProcessorCount;
     const int64_t blocks = dev::div_ceil(int64_t(nv), int64_t(64));
-    // partial echoes held at once: at most 4 GiB, at least one group
+    // partial echoes held at once (one plane per group of a launch, launch 0
+    // writing its first group into the echoes): at most 8 GiB, at least one plane
     const int64_t plane = int64_t(8) * nTR * std::max(nv, 1);
-    const int cap = int(std::max<int64_t>(1, std::min<int64_t>(1 << 15, (int64_t(4) << 30) / plane)));
+    const int cap = int(std::max<int64_t>(1, std::min<int64_t>(1 << 15, (int64_t(8) << 30) / plane)));
     PssfpLayout best {1, nz, 1};
     double best_cost = 1e300;
     for (int zpl = max_zpl; zpl >= 1; zpl--) {
         const int wpc = std::max(1, waves_per_cu[zpl]);
         const int groups = (nz + zpl - 1) / zpl;
-        const int per_launch = std::min(groups, cap + 1);  // launch 0 also writes the echoes
+        const int per_launch = std::min(groups, cap);
         // every wave does the same work, issue-bound at (zpl * 50 + 12) FP64
         // instructions per RF sub-step; wpc / 4 waves share a SIMD
         double cost = 0.0;
