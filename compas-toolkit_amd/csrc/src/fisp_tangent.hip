@@ -338,8 +338,9 @@ Array<cfloat, 2> simulate_magnetization_tangent(
     if (field != TissueParameterField::T1 && field != TissueParameterField::T2 &&
         field != TissueParameterField::B1 && field != TissueParameterField::B0)
         COMPAS_ERROR("tangent derivatives are available for T1, T2, B1 and B0");
-    if ((field == TissueParameterField::B1 && !parameters.has_b1) ||
-        (field == TissueParameterField::B0 && !parameters.has_b0))
+    // (with no voxels every field reads as absent: the empty result is returned below)
+    if (nv > 0 && ((field == TissueParameterField::B1 && !parameters.has_b1) ||
+                   (field == TissueParameterField::B0 && !parameters.has_b0)))
         COMPAS_ERROR("cannot differentiate with respect to an absent parameter");
     const int nslices = int(seq.sliceprofiles.size(0));
     const int nr = nTR * seq.undersampling_factor;

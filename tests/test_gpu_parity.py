@@ -530,6 +530,57 @@ def test_pssfp_tangent_derivative(ctx, oracle, field, nz, nRF):
     assert rel(got, central_difference(oracle.pssfp, t, seq, name)) < TANGENT_TOL
 
 
+def test_empty_inputs(ctx):
+    # sums over no voxels are zero; per-voxel outputs are empty; no slice
+    # profiles (FISP) or slice positions (pSSFP) give zero echoes
+    banner("empty inputs")
+    context()
+    t0 = W.make_tissue(W.SplitMix64(2), 4, 20.0, 30.0).slice(0, 0)
+    p0 = ca.TissueParameters.from_tissue(t0)
+    _, fseq = small_fisp(nv=4, nTR=12, N=6)
+    fs = ca.FispSequence.from_sequence(fseq)
+    pseq = ca.pSSFPSequence.from_sequence(W.make_pssfp(12, 40.0, nz=3, nRF=4))
+    assert ca.simulate_magnetization(p0, fs).collect().shape == (12, 0)
+    assert ca.simulate_magnetization(p0, pseq).collect().shape == (12, 0)
+    for f in (ca.T1, ca.T2, ca.B1, ca.B0):
+        assert ca.simulate_magnetization_tangent(f, p0, fs).collect().shape == (12, 0)
+        assert ca.simulate_magnetization_tangent(f, p0, pseq).collect().shape == (12, 0)
+
+    nc, nr, ns = 2, 12, 5
+    echos = np.zeros((nr, 0), np.complex64)
+    coils = np.zeros((nc, 0), np.complex64)
+    cart = ca.CartesianTrajectory.from_trajectory(
+        W.CartesianTraj(nr, ns, 5e-6, np.zeros(nr, np.complex64), float(np.float32(2 * np.pi / 20.0))))
+    spiral = ca.SpiralTrajectory.from_trajectory(W.make_spiral(nr, ns, 20.0, 4, 5e-6))
+    for method in ("direct", "matmul_fast"):
+        s = ca.magnetization_to_signal(echos, p0, cart, coils, method=method).collect()
+        assert s.shape == (nc, nr, ns) and not np.any(s)
+    s = ca.magnetization_to_signal(echos, p0, spiral, coils).collect()
+    assert s.shape == (nc, nr, ns) and not np.any(s)
+    v4 = np.zeros((4, 0), np.complex64)
+    y = np.ones((nc, nr, ns), np.complex64)
+    for method in ("direct", "gemm_fast"):
+        jv = ca.compute_jacobian(echos, (echos, echos), p0, cart, coils, v4, method=method).collect()
+        assert jv.shape == (nc, nr, ns) and not np.any(jv)
+        assert ca.compute_jacobian_hermitian(echos, (echos, echos), p0, cart, coils, y,
+                                             method=method).collect().shape == (4, 0)
+    d4 = [echos] * 4
+    jv = ca.compute_jacobian_spiral(echos, d4, p0, spiral, coils, np.zeros((6, 0), np.complex64)).collect()
+    assert jv.shape == (nc, nr, ns) and not np.any(jv)
+    assert ca.compute_jacobian_hermitian_spiral(echos, d4, p0, spiral, coils, y).collect().shape == (6, 0)
+
+    # no slice profiles / no slice positions: zero echoes
+    t, fseq = small_fisp(nv=70, nTR=12, N=6)
+    fseq.profiles = np.zeros((0, 12), np.complex64)
+    p = ca.TissueParameters.from_tissue(t)
+    e = ca.simulate_magnetization(p, ca.FispSequence.from_sequence(fseq)).collect()
+    assert e.shape == (12, 70) and not np.any(e)
+    pz = W.make_pssfp(12, 40.0, nz=3, nRF=4)
+    pz.z = np.zeros(0, np.float32)
+    e = ca.simulate_magnetization(p, ca.pSSFPSequence.from_sequence(pz)).collect()
+    assert e.shape == (12, 70) and not np.any(e)
+
+
 def test_residual(ctx):
     rng = W.SplitMix64(3)
     a = (rng.normal(2 * 30 * 17) + 1j * rng.normal(2 * 30 * 17)).reshape(2, 30, 17).astype(np.complex64)
