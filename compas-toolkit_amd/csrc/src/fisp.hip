@@ -23,6 +23,8 @@
 #include <cfloat>
 #include <cmath>
 
+#include <cstdlib>
+
 #include "compas/device/common.h"
 #include "compas/device/fisp_common.h"
 #include "compas/internal.h"
@@ -310,9 +312,14 @@ static int simd_count(int device) {
     return n;
 }
 
-// Lanes per voxel: pick the split with the lowest modelled time
+// Lanes per voxel: pick the split with the lowest modelled time,
 // ceil(waves / SIMDs) * per-wave cost, cost ~ 26 VALU ops per local state per
 // TR plus ~90 ops of per-TR matrix set-up (duplicated in each lane of a voxel).
+// A wave alone on its SIMD issues at about half the rate of two co-resident
+// ones (50176 voxels x 25 states back to back: 1.90 ms in one lane per voxel,
+// 1.45 ms in two), but in the simulate + signal loop the one-lane layout is
+// faster (2.00-2.03 against 2.03-2.10 ms per launch after the contraction,
+// 2.30 against 2.43 ms per pipelined step): the model keeps one lane there.
 static void choose_fisp_layout(int N, int nvoxels, int simds, int* spl, int* lpv) {
     if (N > 64) {
         *spl = 32;
@@ -329,9 +336,9 @@ static void choose_fisp_layout(int N, int nvoxels, int simds, int* spl, int* lpv
         int s = (N + l - 1) / l;
         if (l == 2 && s > 16) continue;
         if (l == 4 && s > 8) continue;
-        double waves = double(nvoxels) * l / 64.0;
-        double rounds = std::ceil(waves / simds);
-        double cost = rounds * (26.0 * s + 90.0 + (l > 1 ? 12.0 : 0.0));
+        const double waves = double(nvoxels) * l / 64.0;
+        const double rounds = std::ceil(waves / simds);
+        const double cost = rounds * (26.0 * s + 90.0 + (l > 1 ? 12.0 : 0.0));
         if (cost < best - 1e-9) {
             best = cost;
             *spl = s;
@@ -382,6 +389,14 @@ Array<cfloat, 2> simulate_magnetization(
 
     int spl = 0, lpv = 0;
     choose_fisp_layout(seq.max_state, nv, simd_count(ctx.device()), &spl, &lpv);
+    if (const char* e = std::getenv("COMPAS_FISP_LPV")) {  // layout override (1, 2 or 4 lanes per voxel)
+        const int l = std::atoi(e);
+        const int s = (seq.max_state + l - 1) / l;
+        if ((l == 1 && s <= 32) || (l == 2 && (s <= 16 || s == 32)) || (l == 4 && (s <= 8 || s == 32))) {
+            spl = (l > 1 && s > (l == 2 ? 16 : 8)) ? 32 : s;
+            lpv = l;
+        }
+    }
     FispLauncher fn = lpv == 1 ? fisp_launcher_lpv1(spl) : lpv == 2 ? fisp_launcher_lpv2(spl) : fisp_launcher_lpv4(spl);
     COMPAS_CHECK(fn != nullptr);
     unsigned blocks = dev::div_ceil(int64_t(nv) * lpv, 64);
