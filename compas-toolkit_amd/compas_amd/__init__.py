@@ -85,6 +85,8 @@ _SIGNATURES = {
     "compas_set_contraction_precision": (None, [_i]),
     "compas_compute_jacobian_spiral": (_vp, [_vp] * 10),
     "compas_compute_jacobian_hermitian_spiral": (_vp, [_vp] * 10),
+    "compas_compute_jacobian_spiral_method": (_vp, [_vp] * 10 + [_i]),
+    "compas_compute_jacobian_hermitian_spiral_method": (_vp, [_vp] * 10 + [_i]),
     "compas_simulate_magnetization_tangent_fisp": (_vp, [_vp, _i, _vp, _vp]),
     "compas_simulate_magnetization_tangent_pssfp": (_vp, [_vp, _i, _vp, _vp]),
     "compas_server_create": (_vp, [_vp, _i, C.c_int64]),
@@ -480,28 +482,36 @@ def _spiral_derivs(dechos):
     return [as_array(d, np.complex64) for d in dechos]
 
 
-def compute_jacobian_spiral(echos, dechos, parameters, trajectory, coils, v) -> CompasArray:
+def compute_jacobian_spiral(echos, dechos, parameters, trajectory, coils, v, method=None) -> CompasArray:
     """J.v on a spiral trajectory, six parameters (T1, T2, B1, B0, rho_x, rho_y).
     dechos: derivatives of the echoes for T1, T2, B1, B0 (dict or sequence);
-    v: [6][nv] complex.  Returns [nc][nr][ns]."""
+    v: [6][nv] complex.  Returns [nc][nr][ns].  method: a JACOBIAN_METHODS key
+    (default gemm_fast, as the C-ABI entry point)."""
     ctx = get_context()
     e = as_array(echos, np.complex64)
     d = _spiral_derivs(dechos)
     c = as_array(coils, np.complex64)
     vv = as_array(v, np.complex64)
-    return _wrap(lib().compas_compute_jacobian_spiral(ctx.ptr, e.ptr, *[x.ptr for x in d], parameters.ptr,
-                                                      trajectory.ptr, c.ptr, vv.ptr))
+    args = (ctx.ptr, e.ptr, *[x.ptr for x in d], parameters.ptr, trajectory.ptr, c.ptr, vv.ptr)
+    if method is None:
+        return _wrap(lib().compas_compute_jacobian_spiral(*args))
+    m = JACOBIAN_METHODS[method] if isinstance(method, str) else int(method)
+    return _wrap(lib().compas_compute_jacobian_spiral_method(*args, m))
 
 
-def compute_jacobian_hermitian_spiral(echos, dechos, parameters, trajectory, coils, y) -> CompasArray:
-    """J^H.y on a spiral trajectory, six parameters; y: [nc][nr][ns].  Returns [6][nv]."""
+def compute_jacobian_hermitian_spiral(echos, dechos, parameters, trajectory, coils, y, method=None) -> CompasArray:
+    """J^H.y on a spiral trajectory, six parameters; y: [nc][nr][ns].  Returns
+    [6][nv].  method: a JACOBIAN_METHODS key (default gemm_fast: MFMA)."""
     ctx = get_context()
     e = as_array(echos, np.complex64)
     d = _spiral_derivs(dechos)
     c = as_array(coils, np.complex64)
     yy = as_array(y, np.complex64)
-    return _wrap(lib().compas_compute_jacobian_hermitian_spiral(ctx.ptr, e.ptr, *[x.ptr for x in d], parameters.ptr,
-                                                                trajectory.ptr, c.ptr, yy.ptr))
+    args = (ctx.ptr, e.ptr, *[x.ptr for x in d], parameters.ptr, trajectory.ptr, c.ptr, yy.ptr)
+    if method is None:
+        return _wrap(lib().compas_compute_jacobian_hermitian_spiral(*args))
+    m = JACOBIAN_METHODS[method] if isinstance(method, str) else int(method)
+    return _wrap(lib().compas_compute_jacobian_hermitian_spiral_method(*args, m))
 
 
 class Server:

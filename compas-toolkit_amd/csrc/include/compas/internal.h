@@ -93,6 +93,17 @@ struct PssfpLayout {
 };
 PssfpLayout pssfp_layout(const CompasContext& ctx, int nz, int nv, int nTR, const int* waves_per_cu, int max_zpl);
 
+// J^H.y on the spiral trajectory with the sample sums on MFMA (BF16 hi + lo,
+// or hi only for the single-product precisions); false when the precision is
+// FP32 or there are more than eight coils (the caller runs the VALU form).
+bool spiral_hermitian_mfma(
+    const CompasContext& ctx, const TissueParameters& p, const cfloat* m, const cfloat* d1, const cfloat* d2,
+    const cfloat* db1, const cfloat* db0, const cfloat* coils, int nc, const cfloat* y, const SpiralTrajectory& traj,
+    ContractionPrecision precision, cfloat* JHy);
+
+// JHy[p][v] = sum over `chunks` partial [6][nv] planes (FP64)
+void spiral_hermitian_reduce_partials(const CompasContext& ctx, const cfloat* partial, int chunks, int nv, cfloat* JHy);
+
 // device bytes of partial echoes one pSSFP simulation of this size allocates
 int64_t pssfp_scratch_bytes(const CompasContext& ctx, int nz, int nv, int nTR);
 

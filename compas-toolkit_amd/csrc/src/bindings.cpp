@@ -198,6 +198,35 @@ int format_timings(const std::map<std::string, KernelStats>& stats, char* buf, i
 
 }  // namespace
 
+namespace {
+
+template<bool ADJOINT>
+CompasArrayHandle* spiral_product(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasArrayHandle* dB1, const CompasArrayHandle* dB0,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory, const CompasArrayHandle* coils,
+    const CompasArrayHandle* vector, JacobianComputeMethod method) {
+    return catch_exceptions([&] {
+        auto e = array(echos, "echos").as<cfloat, 2>("echos");
+        auto d1 = array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1");
+        auto d2 = array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2");
+        auto db1 = array(dB1, "delta_echos_B1").as<cfloat, 2>("delta_echos_B1");
+        auto db0 = array(dB0, "delta_echos_B0").as<cfloat, 2>("delta_echos_B0");
+        const auto& p = unwrap<TissueParameters>(parameters, "parameters");
+        const auto& t = unwrap<SpiralTrajectory>(trajectory, "trajectory");
+        auto c = array(coils, "coils").as<cfloat, 2>("coils");
+        if constexpr (ADJOINT) {
+            return new_array(compute_jacobian_hermitian(context(ctx), e, d1, d2, db1, db0, p, t, c,
+                                                        array(vector, "vector").as<cfloat, 3>("vector"), method));
+        } else {
+            return new_array(compute_jacobian(context(ctx), e, d1, d2, db1, db0, p, t, c,
+                                              array(vector, "vector").as<cfloat, 2>("vector"), method));
+        }
+    });
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* compas_version() {
@@ -508,16 +537,8 @@ CompasArrayHandle* compas_compute_jacobian_spiral(
     const CompasArrayHandle* dT2, const CompasArrayHandle* dB1, const CompasArrayHandle* dB0,
     const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory, const CompasArrayHandle* coils,
     const CompasArrayHandle* vector) {
-    return catch_exceptions([&] {
-        return new_array(compute_jacobian(
-            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
-            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
-            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
-            array(dB1, "delta_echos_B1").as<cfloat, 2>("delta_echos_B1"),
-            array(dB0, "delta_echos_B0").as<cfloat, 2>("delta_echos_B0"),
-            unwrap<TissueParameters>(parameters, "parameters"), unwrap<SpiralTrajectory>(trajectory, "trajectory"),
-            array(coils, "coils").as<cfloat, 2>("coils"), array(vector, "vector").as<cfloat, 2>("vector")));
-    });
+    return spiral_product<false>(ctx, echos, dT1, dT2, dB1, dB0, parameters, trajectory, coils, vector,
+                                 JacobianComputeMethod::GemmFast);
 }
 
 CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
@@ -525,15 +546,29 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
     const CompasArrayHandle* dT2, const CompasArrayHandle* dB1, const CompasArrayHandle* dB0,
     const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory, const CompasArrayHandle* coils,
     const CompasArrayHandle* vector) {
+    return spiral_product<true>(ctx, echos, dT1, dT2, dB1, dB0, parameters, trajectory, coils, vector,
+                                JacobianComputeMethod::GemmFast);
+}
+
+CompasArrayHandle* compas_compute_jacobian_spiral_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasArrayHandle* dB1, const CompasArrayHandle* dB0,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory, const CompasArrayHandle* coils,
+    const CompasArrayHandle* vector, int32_t method) {
     return catch_exceptions([&] {
-        return new_array(compute_jacobian_hermitian(
-            context(ctx), array(echos, "echos").as<cfloat, 2>("echos"),
-            array(dT1, "delta_echos_T1").as<cfloat, 2>("delta_echos_T1"),
-            array(dT2, "delta_echos_T2").as<cfloat, 2>("delta_echos_T2"),
-            array(dB1, "delta_echos_B1").as<cfloat, 2>("delta_echos_B1"),
-            array(dB0, "delta_echos_B0").as<cfloat, 2>("delta_echos_B0"),
-            unwrap<TissueParameters>(parameters, "parameters"), unwrap<SpiralTrajectory>(trajectory, "trajectory"),
-            array(coils, "coils").as<cfloat, 2>("coils"), array(vector, "vector").as<cfloat, 3>("vector")));
+        return spiral_product<false>(ctx, echos, dT1, dT2, dB1, dB0, parameters, trajectory, coils, vector,
+                                     jacobian_method(method));
+    });
+}
+
+CompasArrayHandle* compas_compute_jacobian_hermitian_spiral_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* dT1,
+    const CompasArrayHandle* dT2, const CompasArrayHandle* dB1, const CompasArrayHandle* dB0,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* trajectory, const CompasArrayHandle* coils,
+    const CompasArrayHandle* vector, int32_t method) {
+    return catch_exceptions([&] {
+        return spiral_product<true>(ctx, echos, dT1, dT2, dB1, dB0, parameters, trajectory, coils, vector,
+                                    jacobian_method(method));
     });
 }
 

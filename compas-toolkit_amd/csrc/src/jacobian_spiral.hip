@@ -18,8 +18,10 @@
 //   U = sum_s y conj(Phi),  U' = sum_s (s - ns/2) y conj(Phi)
 // reduced over readouts against conj(m) and conj(dm_p).
 //
-// The spiral contraction has a different sample exponent per readout, so it
-// has no GEMM form; both products run on FP32 VALU with FP64 phase anchors.
+// The spiral sample factor differs per readout, so neither product is one
+// GEMM over all readouts.  The FP32 forms here run on VALU with FP64 phase
+// anchors; with a mixed-precision method J^H.y runs as one GEMM per readout
+// with a generated operand on MFMA (jacobian_spiral_mfma.hip).
 
 #include "compas/device/common.h"
 #include "compas/internal.h"
@@ -237,6 +239,13 @@ __global__ void spiral_hermitian_reduce(const cfloat* __restrict__ partial, int 
 
 }  // namespace kernels
 
+void spiral_hermitian_reduce_partials(const CompasContext& ctx, const cfloat* partial, int chunks, int nv, cfloat* JHy) {
+    KernelScope ks(ctx, "spiral_hermitian_reduce");
+    const int64_t n = 6 * int64_t(nv);
+    kernels::spiral_hermitian_reduce<<<dev::div_ceil(n, 256), 256, 0, ctx.stream()>>>(partial, chunks, nv, JHy);
+    check_launch("spiral_hermitian_reduce");
+}
+
 namespace {
 
 void check_spiral_jacobian_inputs(
@@ -305,7 +314,7 @@ Array<cfloat, 2> compute_jacobian_hermitian(
     Array<cfloat, 3> vector,
     JacobianComputeMethod kind) {
     ctx.activate();
-    (void)jacobian_precision(kind);
+    const ContractionPrecision precision = effective_precision(jacobian_precision(kind));
     check_spiral_jacobian_inputs(echos, dT1, dT2, dB1, dB0, p, traj, coils);
     const int nr = traj.nreadouts, ns = traj.samples_per_readout, nv = p.nvoxels, nc = int(coils.size(0));
     COMPAS_CHECK(vector.size(0) == nc);
@@ -317,7 +326,11 @@ Array<cfloat, 2> compute_jacobian_hermitian(
         ctx.fill_zero(JHy.data(), JHy.nbytes());
         return JHy;
     }
-    // readout chunks so that the grid fills the device: about 8 workgroups per CU
+    // mixed precision: the sample sums on MFMA (up to eight coils)
+    if (spiral_hermitian_mfma(ctx, p, echos.data(), dT1.data(), dT2.data(), dB1.data(), dB0.data(), coils.data(), nc,
+                              vector.data(), traj, precision, JHy.data()))
+        return JHy;
+    // FP32 on VALU; readout chunks so that the grid fills the device: about 8 workgroups per CU
     const int vblocks = int(dev::div_ceil(nv, kernels::JHS_THREADS));
     const int want_chunks = std::max(1, std::min(nr, int(dev::div_ceil(8 * 256, vblocks))));
     const int r_chunk = int(dev::div_ceil(nr, want_chunks));
@@ -331,13 +344,7 @@ Array<cfloat, 2> compute_jacobian_hermitian(
             vector.data(), traj.k_start.data(), traj.delta_k.data(), nr, ns, traj.delta_t, r_chunk, partial.data());
         check_launch("spiral_hermitian");
     }
-    {
-        KernelScope ks(ctx, "spiral_hermitian_reduce");
-        const int64_t n = 6 * int64_t(nv);
-        kernels::spiral_hermitian_reduce<<<dev::div_ceil(n, 256), 256, 0, ctx.stream()>>>(
-            partial.data(), chunks, nv, JHy.data());
-        check_launch("spiral_hermitian_reduce");
-    }
+    spiral_hermitian_reduce_partials(ctx, partial.data(), chunks, nv, JHy.data());
     return JHy;
 }
 

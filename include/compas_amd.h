@@ -164,7 +164,12 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_method(
  * rho rows multiply m and i m) to the derivatives of its spiral signal.
  * echos and the four derivative arrays are [nreadouts][nvoxels]; J.v takes
  * vector [6][nvoxels] and returns [ncoils][nreadouts][nsamples]; J^H.y takes
- * [ncoils][nreadouts][nsamples] and returns [6][nvoxels]. */
+ * [ncoils][nreadouts][nsamples] and returns [6][nvoxels].  These two use
+ * JacobianComputeMethod GemmFast (mixed precision, as the reference's
+ * bindings choose for the Cartesian products when low_precision is 0); the
+ * _method variants take the enumerator (0 Naive .. 4 GemmLow).  Naive,
+ * Direct and Gemm sum on FP32 VALU; GemmFast and GemmLow run J^H.y on MFMA
+ * (BF16 hi + lo, or hi only). */
 CompasArrayHandle* compas_compute_jacobian_spiral(
     const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
     const CompasArrayHandle* delta_echos_T2, const CompasArrayHandle* delta_echos_B1,
@@ -175,6 +180,18 @@ CompasArrayHandle* compas_compute_jacobian_hermitian_spiral(
     const CompasArrayHandle* delta_echos_T2, const CompasArrayHandle* delta_echos_B1,
     const CompasArrayHandle* delta_echos_B0, const CompasObjectHandle* parameters,
     const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* vector);
+CompasArrayHandle* compas_compute_jacobian_spiral_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasArrayHandle* delta_echos_B1,
+    const CompasArrayHandle* delta_echos_B0, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* vector,
+    int32_t method);
+CompasArrayHandle* compas_compute_jacobian_hermitian_spiral_method(
+    const CompasContextHandle* ctx, const CompasArrayHandle* echos, const CompasArrayHandle* delta_echos_T1,
+    const CompasArrayHandle* delta_echos_T2, const CompasArrayHandle* delta_echos_B1,
+    const CompasArrayHandle* delta_echos_B0, const CompasObjectHandle* parameters,
+    const CompasObjectHandle* trajectory, const CompasArrayHandle* coils, const CompasArrayHandle* vector,
+    int32_t method);
 
 /* Forward-mode derivative of the FISP echoes [nreadouts][nvoxels] with
  * respect to one tissue parameter (field: 0 T1, 1 T2, 2 B1, 3 B0): the exact

@@ -267,22 +267,43 @@ def spiral6_inputs(nc, nr, ns, nv, seed):
     return t, traj, coils, echos, ds, v, y
 
 
-@pytest.mark.parametrize("nc,nr,ns,nv", [(4, 70, 64, 784), (1, 33, 50, 333), (5, 41, 37, 1000), (8, 16, 130, 257)])
-def test_spiral_jacobian_six_parameters(ctx, oracle, nc, nr, ns, nv):
+# shapes: coil groups of four (1, 4, 5, 8 coils: one or two groups on MFMA) and
+# more than eight (the VALU form), samples across the 256-sample LDS chunks and
+# not a multiple of 16, voxels not a multiple of the 32-voxel wave tile
+SPIRAL_SHAPES = [(4, 70, 64, 784), (1, 33, 50, 333), (5, 41, 37, 1000), (8, 16, 130, 257), (3, 9, 600, 97),
+                 (10, 7, 40, 70)]
+
+
+@pytest.mark.parametrize("method", ["direct", "gemm_fast"])
+@pytest.mark.parametrize("nc,nr,ns,nv", SPIRAL_SHAPES)
+def test_spiral_jacobian_six_parameters(ctx, oracle, nc, nr, ns, nv, method):
     # J.v and J^H.y on a golden-angle spiral with (T1, T2, B1, B0, rho_x, rho_y)
-    banner(f"spiral J 6 params coils={nc} nr={nr} ns={ns} nv={nv}")
+    banner(f"spiral J 6 params coils={nc} nr={nr} ns={ns} nv={nv} {method}")
     t, traj, coils, echos, ds, v, y = spiral6_inputs(nc, nr, ns, nv, 7 * nc + nr)
     context()
     params = ca.TissueParameters.from_tissue(t)
     straj = ca.SpiralTrajectory.from_trajectory(traj)
-    jv = ca.compute_jacobian_spiral(echos, ds, params, straj, coils, v).collect()
+    jv = ca.compute_jacobian_spiral(echos, ds, params, straj, coils, v, method=method).collect()
     assert rel(jv, oracle.jacobian_spiral(t, traj, coils, echos, *ds, v, "f64")) < SIG_TOL
-    jh = ca.compute_jacobian_hermitian_spiral(echos, ds, params, straj, coils, y).collect()
+    jh = ca.compute_jacobian_hermitian_spiral(echos, ds, params, straj, coils, y, method=method).collect()
     assert rel(jh, oracle.jacobian_hermitian_spiral(t, traj, coils, echos, *ds, y, "f64")) < SIG_TOL
     # the two GPU products are adjoint
     lhs = np.vdot(y.ravel().astype(np.complex128), jv.ravel())
     rhs = np.vdot(jh.ravel().astype(np.complex128), v.ravel().astype(np.complex128))
     assert abs(lhs - rhs) / abs(lhs) < SIG_TOL
+
+
+def test_spiral_hermitian_single_product_is_coarse(ctx, oracle):
+    # gemm_low: BF16 hi only on MFMA -- a few parts in 1e3, far from the hi + lo result
+    t, traj, coils, echos, ds, v, y = spiral6_inputs(4, 20, 64, 300, 3)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    straj = ca.SpiralTrajectory.from_trajectory(traj)
+    want = oracle.jacobian_hermitian_spiral(t, traj, coils, echos, *ds, y, "f64")
+    low = ca.compute_jacobian_hermitian_spiral(echos, ds, params, straj, coils, y, method="gemm_low").collect()
+    fast = ca.compute_jacobian_hermitian_spiral(echos, ds, params, straj, coils, y).collect()
+    assert rel(fast, want) < SIG_TOL
+    assert 1e-4 < rel(low, want) < 3e-2
 
 
 def test_spiral_jacobian_shape_errors(ctx):
