@@ -104,12 +104,14 @@ class _Model:
     def jv(self, v: np.ndarray) -> CompasArray:
         vc = as_array(np.ascontiguousarray(v, dtype=np.complex64))
         if self.spiral:
-            return compute_jacobian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, vc)
+            return compute_jacobian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, vc,
+                                           method=self.method)
         return compute_jacobian(self.m, self.d, self.params, self.trajectory, self.coils, vc, method=self.method)
 
     def jhy(self, y) -> np.ndarray:
         if self.spiral:
-            out = compute_jacobian_hermitian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, y)
+            out = compute_jacobian_hermitian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, y,
+                                                    method=self.method)
         else:
             out = compute_jacobian_hermitian(self.m, self.d, self.params, self.trajectory, self.coils, y,
                                              method=self.method)

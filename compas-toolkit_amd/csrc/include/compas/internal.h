@@ -79,7 +79,14 @@ bool jacobian_hermitian_mfma(
 // nullptr).  A and Ab already carry the readout factor (rewind, prephaser).
 void spiral_sum(
     const CompasContext& ctx, const TissueParameters& p, const cfloat* A, const cfloat* Ab, const cfloat* coils,
-    int nc, const SpiralTrajectory& t, cfloat* out);
+    int nc, const SpiralTrajectory& t, ContractionPrecision precision, cfloat* out);
+
+// The same sums with the voxel contraction on MFMA (BF16 hi + lo, or hi only);
+// false when the precision is FP32 or the coils exceed one 16-row group
+// (eight with one amplitude, four with two): the caller runs the VALU form.
+bool spiral_sum_mfma(
+    const CompasContext& ctx, const TissueParameters& p, const cfloat* A, const cfloat* Ab, const cfloat* coils,
+    int nc, const SpiralTrajectory& t, ContractionPrecision precision, cfloat* out);
 
 // Per (slice, TR) RF table of the FISP kernels: (deg2rad |RF|, cos phi, sin phi, 0).
 Array<float4, 1> fisp_rf_table(const CompasContext& ctx, const FISPSequence& seq);

@@ -20,8 +20,9 @@
 //
 // The spiral sample factor differs per readout, so neither product is one
 // GEMM over all readouts.  The FP32 forms here run on VALU with FP64 phase
-// anchors; with a mixed-precision method J^H.y runs as one GEMM per readout
-// with a generated operand on MFMA (jacobian_spiral_mfma.hip).
+// anchors; with a mixed-precision method both run as one GEMM per readout
+// with a generated operand on MFMA (signal_spiral_mfma.hip for J.v through
+// the spiral sum, jacobian_spiral_mfma.hip for J^H.y).
 
 #include "compas/device/common.h"
 #include "compas/internal.h"
@@ -276,7 +277,7 @@ Array<cfloat, 3> compute_jacobian(
     Array<cfloat, 2> vector,
     JacobianComputeMethod kind) {
     ctx.activate();
-    (void)jacobian_precision(kind);  // validates the enumerator; the spiral sums run on VALU
+    const ContractionPrecision precision = effective_precision(jacobian_precision(kind));
     check_spiral_jacobian_inputs(echos, dT1, dT2, dB1, dB0, p, traj, coils);
     const int nr = traj.nreadouts, ns = traj.samples_per_readout, nv = p.nvoxels, nc = int(coils.size(0));
     COMPAS_CHECK(vector.size(0) == 6);
@@ -297,7 +298,7 @@ Array<cfloat, 3> compute_jacobian(
             traj.k_start.data(), nr, ns, traj.delta_t, Aa.data(), Ab.data());
         check_launch("spiral_jv_amplitudes");
     }
-    spiral_sum(ctx, p, Aa.data(), Ab.data(), coils.data(), nc, traj, Jv.data());
+    spiral_sum(ctx, p, Aa.data(), Ab.data(), coils.data(), nc, traj, precision, Jv.data());
     return Jv;
 }
 

@@ -222,9 +222,10 @@ static Array<cfloat, 3> signal_cartesian(
 
 void spiral_sum(
     const CompasContext& ctx, const TissueParameters& p, const cfloat* A, const cfloat* Ab, const cfloat* coils,
-    int nc, const SpiralTrajectory& t, cfloat* out) {
+    int nc, const SpiralTrajectory& t, ContractionPrecision precision, cfloat* out) {
     const int nr = t.nreadouts, ns = t.samples_per_readout;
     if (nc == 0 || nr == 0 || ns == 0) return;
+    if (spiral_sum_mfma(ctx, p, A, Ab, coils, nc, t, precision, out)) return;
     // 8 samples per thread: measured fastest at 4 coils x 1120 x 512 over 50176
     // voxels (24.7 ms, against 35.1 ms for 4 and 30.8 ms for 16, which needs
     // 256 VGPRs and leaves one wave per SIMD)
@@ -244,7 +245,7 @@ void spiral_sum(
 
 static Array<cfloat, 3> signal_spiral(
     const CompasContext& ctx, const Array<cfloat, 2>& echos, const TissueParameters& p,
-    const SpiralTrajectory& t, const Array<cfloat, 2>& coils) {
+    const SpiralTrajectory& t, const Array<cfloat, 2>& coils, SimulateSignalMethod method) {
     const int nc = int(coils.size(0)), nr = t.nreadouts, ns = t.samples_per_readout, nv = p.nvoxels;
     COMPAS_CHECK(t.k_start.size(0) == nr && t.delta_k.size(0) == nr);
     auto signal = ctx.empty<cfloat, 3>({nc, nr, ns});
@@ -261,7 +262,8 @@ static Array<cfloat, 3> signal_spiral(
             dev::tissue_view(p), echos.data(), t.k_start.data(), nr, ns, t.delta_t, A.data());
         check_launch("signal_spiral_factors");
     }
-    spiral_sum(ctx, p, A.data(), nullptr, coils.data(), nc, t, signal.data());
+    spiral_sum(ctx, p, A.data(), nullptr, coils.data(), nc, t, effective_precision(signal_precision(method)),
+               signal.data());
     return signal;
 }
 
@@ -278,7 +280,7 @@ Array<cfloat, 3> magnetization_to_signal(
         (void)signal_precision(method);  // validates the enumerator
         return signal_cartesian(ctx, echos, parameters, *cart, coils, method);
     } else if (const auto* sp = dynamic_cast<const SpiralTrajectory*>(&trajectory)) {
-        return signal_spiral(ctx, echos, parameters, *sp, coils);
+        return signal_spiral(ctx, echos, parameters, *sp, coils, method);
     }
     COMPAS_ERROR("invalid trajectory type");
 }

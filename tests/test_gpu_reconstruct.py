@@ -70,11 +70,13 @@ def test_gauss_newton_spiral_with_b1():
     coils = W.make_coils(n, fov, 2)
     truth = estimate_of(t)
     params = truth.parameters()
-    y = ca.magnetization_to_signal(ca.simulate_magnetization(params, seq), params, traj, coils).collect()
+    y = ca.magnetization_to_signal(ca.simulate_magnetization(params, seq), params, traj, coils,
+                                   method="direct").collect()
 
     start = estimate_of(t, T1=(t.T1 * 0.85).astype(np.float32), T2=(t.T2 * 1.15).astype(np.float32),
                         B1=(t.B1 + 0.05).astype(np.float32), rho=(t.rho * 1.1).astype(np.complex64))
-    res = R.gauss_newton(y, start, seq, traj, coils, iterations=10, cg_iterations=200, cg_tol=1e-7, fit_b1=True)
+    res = R.gauss_newton(y, start, seq, traj, coils, iterations=10, cg_iterations=200, cg_tol=1e-7, fit_b1=True,
+                         method="direct", signal_method="direct")
     obj = res.objective
     print("objective", obj, "cg", res.cg_iterations, "steps", res.steps, flush=True)
     assert obj[-1] < 1e-6 * obj[0], (obj, res.cg_iterations, res.steps)
@@ -82,3 +84,21 @@ def test_gauss_newton_spiral_with_b1():
     assert np.max(np.abs(est.T1 / t.T1 - 1)) < 5e-3
     assert np.max(np.abs(est.T2 / t.T2 - 1)) < 5e-3
     assert np.max(np.abs(est.B1 - t.B1)) < 5e-3
+
+
+def test_gauss_newton_mixed_precision_descends():
+    # the default mixed-precision products (BF16 hi + lo on MFMA) drive the same
+    # loop: the objective still falls by orders of magnitude
+    banner("gauss-newton spiral mixed precision")
+    context()
+    n, fov = 8, 8.0
+    t = phantom(n, fov, seed=13)
+    seq = ca.FispSequence.from_sequence(W.make_fisp(80, 60.0, 12))
+    traj = ca.SpiralTrajectory.from_trajectory(W.make_spiral(80, 32, fov, n, 5e-6))
+    coils = W.make_coils(n, fov, 2)
+    params = estimate_of(t).parameters()
+    y = ca.magnetization_to_signal(ca.simulate_magnetization(params, seq), params, traj, coils).collect()
+    start = estimate_of(t, T1=(t.T1 * 0.9).astype(np.float32), T2=(t.T2 * 1.1).astype(np.float32))
+    res = R.gauss_newton(y, start, seq, traj, coils, iterations=6)
+    assert res.objective[-1] < 1e-4 * res.objective[0], (res.objective, res.cg_iterations)
+    assert np.max(np.abs(res.estimate.T2 / t.T2 - 1)) < 1e-2
