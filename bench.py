@@ -271,11 +271,16 @@ def spiral_bench(args):
     ds = [ca.simulate_magnetization_derivative(f, echos0, params, seq) for f in (ca.T1, ca.T2, ca.B1, ca.B0)]
     ctx.synchronize()
 
+    # --method direct: the plain FP32 VALU sums; otherwise the signal method given and
+    # the mixed-precision products (MFMA)
+    jac_method = "direct" if args.method in ("direct", "naive") else ("gemm_low" if args.method == "matmul_low"
+                                                                       else "gemm_fast")
+
     def step():
         e = ca.simulate_magnetization(params, seq)
-        s = ca.magnetization_to_signal(e, params, traj, coils)
-        jv = ca.compute_jacobian_spiral(e, ds, params, traj, coils, v)
-        jh = ca.compute_jacobian_hermitian_spiral(e, ds, params, traj, coils, y)
+        s = ca.magnetization_to_signal(e, params, traj, coils, method=args.method)
+        jv = ca.compute_jacobian_spiral(e, ds, params, traj, coils, v, method=jac_method)
+        jh = ca.compute_jacobian_hermitian_spiral(e, ds, params, traj, coils, y, method=jac_method)
         return e, s, jv, jh
 
     for _ in range(args.warmup):
@@ -297,27 +302,36 @@ def spiral_bench(args):
     timings = ctx.read_kernel_timings()
     kernels = {k: {"launches": c, "avg_ms": tot / max(c, 1), "total_ms": tot} for k, (c, tot) in timings.items()}
 
-    # algorithmic work per launch (FP32 VALU): pSSFP (57 nRF + 28) flop per isochromat.TR; spiral
+    # algorithmic work per launch: pSSFP (57 nRF + 28) flop per isochromat.TR (FP64 state); spiral
     # sums 8 nc flop per (readout, sample, voxel) for the coil MACs + 6 for the sample recurrence
-    # (J.v: two amplitudes, 12); J^H.y: 8 nc (y conj(Phi)) + 4 nc (the (s - ns/2) moment) + 6
+    # (J.v: two amplitudes, 16 nc + 6); J^H.y: 8 nc (y conj(Phi)) + 8 nc (the (s - ns/2) moment) + 6.
+    # The spiral kernels run the contraction on MFMA (BF16 hi + lo: three v_mfma_f32_16x16x32_bf16
+    # per 16 voxels x 16 samples x 16 rows) and generate the sample factor on VALU, which binds:
+    # each entry gives the algorithmic rate against the FP32 vector peak and the issued MFMA
+    # fraction.
     rsv = float(nr) * ns * nv
-    work = {"pssfp_iso": (57.0 * nRF + 28) * nz * nTR * nv,
-            "signal_spiral_direct": None,  # two launches per step with different amplitude counts
-            "spiral_hermitian": (12.0 * nc + 6) * rsv}
     rooflines = {}
-    for k in ("pssfp_iso", "spiral_hermitian"):
+    if "pssfp_iso" in kernels:
+        ach = (57.0 * nRF + 28) * nz * nTR * nv / (kernels["pssfp_iso"]["avg_ms"] * 1e-3) / 1e12
+        rooflines["pssfp_iso"] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": ach / PEAK_FP64_TFLOPS,
+                                  "note": "FP64 isochromat state; the FP32 vector peak is twice this"}
+    issued = 2.0 * 16 * 2 * rsv * 3  # 16 real rows x 2 nv real K x ns x nr, three products
+    mfma_kernels = {"spiral_sum_mfma": ((8.0 * nc + 6) * rsv + (16.0 * nc + 6) * rsv, 2 * issued),  # signal + J.v
+                    "spiral_hermitian_mfma": ((16.0 * nc + 6) * rsv, issued)}
+    for k, (flop, iss) in mfma_kernels.items():
         if k in kernels:
-            # the isochromat state is FP64, so its roof is the FP64 vector peak
-            peak = PEAK_FP64_TFLOPS if k == "pssfp_iso" else PEAK_FP32_TFLOPS
-            ach = work[k] / (kernels[k]["avg_ms"] * 1e-3) / 1e12
-            rooflines[k] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
-            if k == "pssfp_iso":
-                rooflines[k]["note"] = "FP64 isochromat state; the FP32 vector peak is twice this"
-    if "signal_spiral_direct" in kernels:
-        flop = (8.0 * nc + 6) * rsv + (8.0 * nc + 12 + 2) * rsv  # signal + J.v launches of one step
-        ach = flop / (kernels["signal_spiral_direct"]["total_ms"] / args.steps * 1e-3) / 1e12
-        rooflines["signal_spiral_direct"] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS,
-                                             "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS}
+            step_s = kernels[k]["total_ms"] / args.steps * 1e-3
+            ach = flop / step_s / 1e12
+            rooflines[k] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                            "frac": ach / PEAK_FP32_TFLOPS,
+                            "mfma_issued_frac": iss / step_s / 1e12 / PEAK_BF16_TFLOPS}
+    for k, flop in (("signal_spiral_direct", (8.0 * nc + 6) * rsv + (16.0 * nc + 6) * rsv),
+                    ("spiral_hermitian", (16.0 * nc + 6) * rsv)):
+        if k in kernels:  # the FP32 VALU forms (plain methods)
+            ach = flop / (kernels[k]["total_ms"] / args.steps * 1e-3) / 1e12
+            rooflines[k] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                            "frac": ach / PEAK_FP32_TFLOPS}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
     roof = dict(rooflines.get(dominant, {}), kernel=dominant, traffic=None) if dominant in rooflines else None
 
@@ -355,7 +369,8 @@ def spiral_bench(args):
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 tissue maps, sinc-Gaussian RF, golden-angle spiral, Gaussian coils)",
         "config": {"workload": f"pSSFP {nv} voxels x {nTR} TRs x {nz} isochromats x {nRF} RF steps + spiral signal "
-                               f"{nc} coils x {nr} readouts x {ns} samples + J.v [6][{nv}] + J^H.y"},
+                               f"{nc} coils x {nr} readouts x {ns} samples + J.v [6][{nv}] + J^H.y",
+                   "signal_method": args.method, "jacobian_method": jac_method},
         "isochromat_tr_per_s": nv * nTR * nz * args.steps / t_total,
         "kspace_samples_per_s": 3 * nc * nr * ns * args.steps / t_total,
         "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p99": 1e3 * percentile(lat, 99)},
