@@ -13,6 +13,7 @@ no fallback: if the library is missing or no GPU is present the calls raise.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 from typing import Optional
@@ -89,6 +90,13 @@ _SIGNATURES = {
     "compas_compute_jacobian_hermitian_spiral_method": (_vp, [_vp] * 10 + [_i]),
     "compas_simulate_magnetization_tangent_fisp": (_vp, [_vp, _i, _vp, _vp]),
     "compas_simulate_magnetization_tangent_pssfp": (_vp, [_vp, _i, _vp, _vp]),
+    "compas_graph_begin": (None, [_vp]),
+    "compas_graph_end": (_vp, [_vp]),
+    "compas_graph_abort": (None, [_vp]),
+    "compas_graph_launch": (None, [_vp]),
+    "compas_graph_destroy": (None, [_vp]),
+    "compas_write_array_complex": (None, [_vp, _vp, _vp, C.c_int64]),
+    "compas_copy_array": (None, [_vp, _vp, _vp]),
     "compas_server_create": (_vp, [_vp, _i, C.c_int64]),
     "compas_server_destroy": (None, [_vp]),
     "compas_server_submit_simulate_signal": (C.c_int64, [_vp, _vp, _vp, _vp, _vp, _i]),
@@ -256,6 +264,19 @@ class CompasArray:
         sizes = (C.c_int64 * a.ndim)(*a.shape)
         ptr = _check(fn(ctx.ptr, a.ctypes.data_as(C.c_void_p), a.ndim, sizes))
         return cls(ptr, ctx)
+
+    def write(self, a) -> None:
+        """Overwrite the contents with a complex array of the same size (blocking upload)."""
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.complex64))
+        if a.size != int(np.prod(self.shape)) or self.dtype != np.complex64:
+            raise CompasError("write needs a complex array of the same size")
+        lib().compas_write_array_complex(self.context.ptr, self.ptr, a.ctypes.data_as(C.c_void_p), a.size)
+        _check()
+
+    def copy_from(self, src: "CompasArray") -> None:
+        """Stream-ordered device copy from an array of the same type and size."""
+        lib().compas_copy_array(self.context.ptr, self.ptr, src.ptr)
+        _check()
 
     def collect(self) -> np.ndarray:
         out = np.empty(self.shape, dtype=self.dtype)
@@ -512,6 +533,63 @@ def compute_jacobian_hermitian_spiral(echos, dechos, parameters, trajectory, coi
         return _wrap(lib().compas_compute_jacobian_hermitian_spiral(*args))
     m = JACOBIAN_METHODS[method] if isinstance(method, str) else int(method)
     return _wrap(lib().compas_compute_jacobian_hermitian_spiral_method(*args, m))
+
+
+class Graph:
+    """A captured sequence of library calls (see `capture`); launch() replays it
+    on the context's stream, rewriting the arrays the captured calls returned."""
+
+    def __init__(self, ctx: "Context"):
+        self.context = ctx
+        self.ptr = None
+
+    def launch(self) -> None:
+        if not self.ptr:
+            raise CompasError("graph is not captured")
+        lib().compas_graph_launch(self.ptr)
+        _check()
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().compas_graph_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@contextlib.contextmanager
+def capture():
+    """Record the library calls of the block into a Graph instead of running them:
+
+        with capture() as g:
+            echos = simulate_magnetization(p, s)
+            signal = magnetization_to_signal(echos, p, traj, coils)
+        g.launch()          # runs both; signal.collect() reads the result
+
+    The block must not read arrays back or upload new ones; run it once eagerly
+    first so that autotuning is done."""
+    ctx = get_context()
+    g = Graph(ctx)
+    lib().compas_graph_begin(ctx.ptr)
+    _check()
+    try:
+        yield g
+    except BaseException:
+        lib().compas_graph_abort(ctx.ptr)
+        _check_quiet()
+        raise
+    g.ptr = _check(lib().compas_graph_end(ctx.ptr))
+
+
+def _check_quiet():
+    try:
+        _check()
+    except CompasError:
+        pass
 
 
 class Server:

@@ -24,7 +24,11 @@ may be held fixed.  With phase_encode (Cartesian), echoes and derivatives pass
 through phase_encoding before the signal, as the reference's examples do.
 
 Vectors of the normal equations ([rows][nvoxels] real) live on the host; every
-product runs on the GPU.
+product runs on the GPU.  With use_graph (default) the normal operator
+J^H(J v) is captured as a HIP graph after each linearisation (one eager
+product first, for autotuning) and each CG step replays it with v written
+into the graph's input array: the same kernels and results, without the
+per-launch host overhead that dominates small problems.
 """
 
 from __future__ import annotations
@@ -34,8 +38,10 @@ from typing import List, Optional
 
 import numpy as np
 
+import time
+
 from . import (B0, B1, T1, T2, CartesianTrajectory, CompasArray, SpiralTrajectory, TissueParameters,
-               as_array, compute_jacobian, compute_jacobian_hermitian, compute_jacobian_hermitian_spiral,
+               as_array, capture, compute_jacobian, compute_jacobian_hermitian, compute_jacobian_hermitian_spiral,
                compute_jacobian_spiral, compute_residual, magnetization_to_signal, phase_encoding,
                simulate_magnetization, simulate_magnetization_tangent)
 
@@ -67,12 +73,15 @@ class GaussNewtonResult:
     objective: List[float] = field(default_factory=list)  # before each iteration, then the final value
     cg_iterations: List[int] = field(default_factory=list)
     steps: List[float] = field(default_factory=list)
+    normal_seconds: float = 0.0  # wall time in the normal operator J^H(J v) (host view)
+    normal_calls: int = 0
 
 
 class _Model:
     """S(p), the derivatives at p and the products J.v / Re J^H.y."""
 
-    def __init__(self, sequence, trajectory, coils, method, signal_method, phase_encode, fit_b1, fit_b0):
+    def __init__(self, sequence, trajectory, coils, method, signal_method, phase_encode, fit_b1, fit_b0,
+                 use_graph=True):
         self.sequence = sequence
         self.trajectory = trajectory
         self.coils = as_array(coils, np.complex64)
@@ -86,6 +95,10 @@ class _Model:
         if not self.spiral and (fit_b1 or fit_b0):
             raise ValueError("the Cartesian products have no B1 / B0 rows")
         self.rows = 6 if self.spiral else 4
+        self.use_graph = use_graph
+        self.graph = None
+        self.normal_seconds = 0.0
+        self.normal_calls = 0
 
     def _encode(self, a, params):
         return phase_encoding(a, params, self.trajectory) if self.phase_encode else a
@@ -100,23 +113,33 @@ class _Model:
         self.params = params
         self.m = m
         self.d = [self._encode(simulate_magnetization_tangent(f, params, self.sequence), params) for f in fields]
+        if self.graph is not None:
+            self.graph.close()
+            self.graph = None
+        if self.use_graph:
+            nv = est.T1.shape[0]
+            self.v_in = CompasArray.from_numpy(np.zeros((self.rows, nv), np.complex64))
+            self._jhy_dev(self.jv(self.v_in)).collect()  # eager once: autotuning outside the capture
+            with capture() as g:
+                self.normal_out = self._jhy_dev(self.jv(self.v_in))
+            self.graph = g
 
-    def jv(self, v: np.ndarray) -> CompasArray:
-        vc = as_array(np.ascontiguousarray(v, dtype=np.complex64))
+    def jv(self, v) -> CompasArray:
+        vc = v if isinstance(v, CompasArray) else as_array(np.ascontiguousarray(v, dtype=np.complex64))
         if self.spiral:
             return compute_jacobian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, vc,
                                            method=self.method)
         return compute_jacobian(self.m, self.d, self.params, self.trajectory, self.coils, vc, method=self.method)
 
-    def jhy(self, y) -> np.ndarray:
+    def _jhy_dev(self, y) -> CompasArray:
         if self.spiral:
-            out = compute_jacobian_hermitian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, y,
-                                                    method=self.method)
-        else:
-            out = compute_jacobian_hermitian(self.m, self.d, self.params, self.trajectory, self.coils, y,
-                                             method=self.method)
-        g = out.collect().real.astype(np.float64)
-        return self.mask(g)
+            return compute_jacobian_hermitian_spiral(self.m, self.d, self.params, self.trajectory, self.coils, y,
+                                                     method=self.method)
+        return compute_jacobian_hermitian(self.m, self.d, self.params, self.trajectory, self.coils, y,
+                                          method=self.method)
+
+    def jhy(self, y) -> np.ndarray:
+        return self.mask(self._jhy_dev(y).collect().real.astype(np.float64))
 
     def mask(self, g: np.ndarray) -> np.ndarray:
         if self.spiral:
@@ -127,7 +150,21 @@ class _Model:
         return g
 
     def normal(self, v: np.ndarray) -> np.ndarray:
-        return self.jhy(self.jv(v))
+        t0 = time.perf_counter()
+        if self.graph is not None:
+            self.v_in.write(np.asarray(v, dtype=np.complex64))
+            self.graph.launch()
+            out = self.mask(self.normal_out.collect().real.astype(np.float64))
+        else:
+            out = self.jhy(self.jv(v))
+        self.normal_seconds += time.perf_counter() - t0
+        self.normal_calls += 1
+        return out
+
+    def close(self):
+        if self.graph is not None:
+            self.graph.close()
+            self.graph = None
 
 
 def _cg(apply, b, iterations, tol, shift_rel=0.0):
@@ -180,12 +217,12 @@ def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, itera
                  cg_iterations: int = 100, cg_tol: float = 1e-6, damping: float = 1e-6,
                  method: str = "gemm_fast", signal_method: str = "matmul_fast", phase_encode: bool = True,
                  fit_b1: bool = False, fit_b0: bool = False, max_log_step: float = 0.7,
-                 min_step: float = 1.0 / 64) -> GaussNewtonResult:
+                 min_step: float = 1.0 / 64, use_graph: bool = True) -> GaussNewtonResult:
     """Minimise ||S(p) - measured||^2 over the tissue parameters, starting from
     `initial`.  `measured` is [ncoils][nreadouts][samples] complex64 (host or
     device array).  `damping` is the Levenberg shift relative to the normal
     operator's Rayleigh quotient along the gradient."""
-    model = _Model(sequence, trajectory, coils, method, signal_method, phase_encode, fit_b1, fit_b0)
+    model = _Model(sequence, trajectory, coils, method, signal_method, phase_encode, fit_b1, fit_b0, use_graph)
     y = as_array(measured, np.complex64)
     est = initial.copy()
     result = GaussNewtonResult(est)
@@ -210,6 +247,9 @@ def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, itera
             break
         result.steps.append(step)
         est, params, m, obj, r = cand, c_params, c_m, c_obj, c_r
+    model.close()
     result.estimate = est
     result.objective.append(obj)
+    result.normal_seconds = model.normal_seconds
+    result.normal_calls = model.normal_calls
     return result

@@ -10,11 +10,17 @@
 //     the same stream, which keeps allocation off the critical path and is
 //     safe because stream order serialises the reuse;
 //   * an optional kernel timer (HIP events around each launch) used by the
-//     benchmark to measure per-kernel durations on the library's own stream.
+//     benchmark to measure per-kernel durations on the library's own stream;
+//   * graph capture of the library calls enqueued on the stream
+//     (hipStreamBeginCapture, relaxed mode): blocks allocated while a graph
+//     is captured belong to it while it lives -- released arrays of the
+//     capture are held back from the pool until the graph is destroyed, so a
+//     replay never writes memory that other work has been given.
 
 #include <array>
 #include <cstdint>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -81,6 +87,17 @@ class Runtime {
     void end_kernel(int token);
     std::map<std::string, KernelStats> collect_timings();
 
+    // Graph capture on this runtime's stream (one capture at a time).  The
+    // calls between begin and end must not synchronise the host (reads of
+    // arrays, uploads, first-time autotuning); run the sequence once eagerly
+    // before capturing it.  Arrays the captured calls return are written by
+    // every launch of the graph.
+    void begin_capture();
+    hipGraphExec_t end_capture(int* id);  // instantiated graph; `id` names its memory
+    void abort_capture();
+    void release_graph(int id);  // the graph is gone: its held blocks return to the pool
+    bool capturing() const { return capture_id_ >= 0; }
+
   private:
     int device_;
     hipStream_t stream_ = nullptr;
@@ -88,6 +105,10 @@ class Runtime {
     std::multimap<size_t, void*> free_blocks_;
     size_t in_use_ = 0, cached_ = 0;
     bool timing_ = false;
+    bool timing_before_capture_ = false;
+    int capture_id_ = -1, next_graph_id_ = 0;
+    std::unordered_map<void*, int> graph_blocks_;  // block allocated during a capture -> its graph
+    std::map<int, std::vector<std::pair<void*, size_t>>> graph_held_;  // released blocks kept for a graph
     struct Pending {
         std::string name;
         hipEvent_t start, stop;
@@ -231,6 +252,24 @@ class CompasContext {
 };
 
 CompasContext make_context(int device = 0);
+
+// An instantiated graph of library calls captured on one runtime's stream;
+// launch() replays it on that stream.
+class CapturedGraph {
+  public:
+    CapturedGraph(std::shared_ptr<Runtime> rt, hipGraphExec_t exec, int id) : rt_(std::move(rt)), exec_(exec), id_(id) {}
+    ~CapturedGraph();
+    CapturedGraph(const CapturedGraph&) = delete;
+    CapturedGraph& operator=(const CapturedGraph&) = delete;
+    void launch() const;
+
+  private:
+    std::shared_ptr<Runtime> rt_;
+    hipGraphExec_t exec_;
+    int id_;
+};
+
+std::shared_ptr<CapturedGraph> end_capture(const CompasContext& ctx);
 
 // RAII helper that times one launch when the runtime's timer is enabled.
 struct KernelScope {

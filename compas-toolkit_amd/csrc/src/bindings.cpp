@@ -284,6 +284,59 @@ void compas_read_array_complex(
     });
 }
 
+void compas_write_array_complex(
+    const CompasContextHandle* ctx, const CompasArrayHandle* a, const compas_cfloat* src, int64_t length) {
+    catch_exceptions([&] {
+        const CompasContext& c = context(ctx);
+        const ArrayHandle& h = array(a, "array");
+        if (h.dtype != DT_COMPLEX) throw CompasException("array element type does not match the write function");
+        COMPAS_CHECK(length >= 0 && length <= h.numel());
+        if (length == 0) return;
+        COMPAS_CHECK(src != nullptr);
+        c.upload(h.buf, src, size_t(length) * sizeof(cfloat));
+    });
+}
+
+void compas_copy_array(const CompasContextHandle* ctx, const CompasArrayHandle* dst, const CompasArrayHandle* src) {
+    catch_exceptions([&] {
+        const CompasContext& c = context(ctx);
+        const ArrayHandle& d = array(dst, "dst");
+        const ArrayHandle& s = array(src, "src");
+        if (d.dtype != s.dtype || d.numel() != s.numel()) throw CompasException("arrays differ in type or size");
+        if (d.numel() == 0) return;
+        c.activate();
+        const size_t elem = d.dtype == DT_COMPLEX ? sizeof(cfloat) : sizeof(float);
+        COMPAS_HIP_CHECK(hipMemcpyAsync(d.buf->data(), s.buf->data(), size_t(d.numel()) * elem,
+                                        hipMemcpyDeviceToDevice, c.stream()));
+    });
+}
+
+void compas_graph_begin(const CompasContextHandle* ctx) {
+    catch_exceptions([&] { context(ctx).runtime()->begin_capture(); });
+}
+
+CompasGraphHandle* compas_graph_end(const CompasContextHandle* ctx) {
+    return catch_exceptions([&]() -> CompasGraphHandle* {
+        auto g = end_capture(context(ctx));
+        return reinterpret_cast<CompasGraphHandle*>(new std::shared_ptr<CapturedGraph>(std::move(g)));
+    });
+}
+
+void compas_graph_abort(const CompasContextHandle* ctx) {
+    catch_exceptions([&] { context(ctx).runtime()->abort_capture(); });
+}
+
+void compas_graph_launch(const CompasGraphHandle* graph) {
+    catch_exceptions([&] {
+        if (graph == nullptr) throw CompasException("null graph");
+        (*reinterpret_cast<const std::shared_ptr<CapturedGraph>*>(graph))->launch();
+    });
+}
+
+void compas_graph_destroy(const CompasGraphHandle* graph) {
+    catch_exceptions([&] { delete reinterpret_cast<const std::shared_ptr<CapturedGraph>*>(graph); });
+}
+
 void compas_destroy_array(const CompasArrayHandle* a) {
     catch_exceptions([&] { delete reinterpret_cast<const Handle*>(a); });
 }

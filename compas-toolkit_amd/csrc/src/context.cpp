@@ -82,12 +82,20 @@ void* Runtime::allocate(size_t nbytes) {
             free_blocks_.erase(it);
             cached_ -= sz;
             in_use_ += sz;
+            if (capture_id_ >= 0) graph_blocks_[p] = capture_id_;
             return p;
         }
     }
     activate();
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess && capturing()) {
+        // no trimming while a capture is open (it synchronises the stream)
+        (void)hipGetLastError();
+        std::stringstream ss;
+        ss << "device allocation of " << nbytes << " bytes failed during graph capture";
+        throw CompasException(ss.str());
+    }
     if (e != hipSuccess) {
         // out of memory: give the cache back and retry, then the caches of the
         // other runtimes on this device
@@ -120,9 +128,114 @@ void Runtime::release(void* ptr, size_t nbytes) {
     if (!ptr) return;
     size_t n = round_block(nbytes);
     std::lock_guard<std::mutex> lock(mutex_);
+    auto g = graph_blocks_.find(ptr);
+    if (g != graph_blocks_.end()) {  // a graph's replays still write it: hold it for the graph
+        graph_held_[g->second].emplace_back(ptr, n);
+        graph_blocks_.erase(g);
+        return;
+    }
     in_use_ -= n;
     cached_ += n;
     free_blocks_.emplace(n, ptr);
+}
+
+void Runtime::begin_capture() {
+    activate();
+    {
+        std::lock_guard<std::mutex> lock(mutex_);
+        if (capture_id_ >= 0) throw CompasException("a graph capture is already open on this context");
+        capture_id_ = next_graph_id_++;
+        timing_before_capture_ = timing_;
+        timing_ = false;  // no timing events inside a graph
+    }
+    hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lock(mutex_);
+        capture_id_ = -1;
+        timing_ = timing_before_capture_;
+        COMPAS_HIP_CHECK(e);
+    }
+}
+
+hipGraphExec_t Runtime::end_capture(int* id) {
+    activate();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(stream_, &graph);
+    int cid;
+    {
+        std::lock_guard<std::mutex> lock(mutex_);
+        cid = capture_id_;
+        capture_id_ = -1;
+        timing_ = timing_before_capture_;
+    }
+    if (e != hipSuccess || graph == nullptr) {
+        (void)hipGetLastError();
+        release_graph(cid);
+        std::stringstream ss;
+        ss << "graph capture failed (a call synchronised or was not capturable): " << hipGetErrorString(e);
+        throw CompasException(ss.str());
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        release_graph(cid);
+        COMPAS_HIP_CHECK(e);
+    }
+    *id = cid;
+    return exec;
+}
+
+void Runtime::abort_capture() {
+    activate();
+    hipGraph_t graph = nullptr;
+    (void)hipStreamEndCapture(stream_, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    int cid;
+    {
+        std::lock_guard<std::mutex> lock(mutex_);
+        cid = capture_id_;
+        capture_id_ = -1;
+        timing_ = timing_before_capture_;
+    }
+    if (cid >= 0) release_graph(cid);
+}
+
+void Runtime::release_graph(int id) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    auto h = graph_held_.find(id);
+    if (h != graph_held_.end()) {
+        // stream order: later work on this stream runs after the graph's last launch
+        for (auto& pn : h->second) {
+            in_use_ -= pn.second;
+            cached_ += pn.second;
+            free_blocks_.emplace(pn.second, pn.first);
+        }
+        graph_held_.erase(h);
+    }
+    // blocks of the capture still held by arrays return to the pool as usual
+    for (auto it = graph_blocks_.begin(); it != graph_blocks_.end();) {
+        if (it->second == id) it = graph_blocks_.erase(it);
+        else ++it;
+    }
+}
+
+CapturedGraph::~CapturedGraph() {
+    (void)hipSetDevice(rt_->device());
+    (void)hipGraphExecDestroy(exec_);
+    rt_->release_graph(id_);
+}
+
+void CapturedGraph::launch() const {
+    rt_->activate();
+    COMPAS_HIP_CHECK(hipGraphLaunch(exec_, rt_->stream()));
+}
+
+std::shared_ptr<CapturedGraph> end_capture(const CompasContext& ctx) {
+    int id = -1;
+    hipGraphExec_t exec = ctx.runtime()->end_capture(&id);
+    return std::make_shared<CapturedGraph>(ctx.runtime(), exec, id);
 }
 
 void Runtime::trim() {

@@ -268,6 +268,9 @@ int pick_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, con
         if (v >= 0 && v < NUM_VARIANTS) return v;
     }
     if (!autotune_enabled() || int64_t(s.M) * s.N * s.K < (int64_t(1) << 24)) return DEFAULT_VARIANT;
+    // timing trials synchronise the host: not inside a graph capture (the
+    // default tile is captured; a shape run once eagerly first uses its tuned one)
+    if (ctx.runtime()->capturing()) return DEFAULT_VARIANT;
     // time every variant into a scratch output (the caller's output is not touched)
     auto scratch = ctx.empty<cfloat, 1>({int64_t(s.M) * s.N});
     ContractionShape t = s;

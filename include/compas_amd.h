@@ -28,6 +28,7 @@ typedef struct CompasContextHandle CompasContextHandle; /* compas::CompasContext
 typedef struct CompasObjectHandle CompasObjectHandle;   /* parameters, sequences, trajectories */
 typedef struct CompasArrayHandle CompasArrayHandle;     /* device array */
 typedef struct CompasServerHandle CompasServerHandle;   /* request server (extension) */
+typedef struct CompasGraphHandle CompasGraphHandle;     /* captured graph (extension) */
 
 typedef struct {
     float re;
@@ -208,6 +209,27 @@ CompasArrayHandle* compas_simulate_magnetization_tangent_fisp(
 CompasArrayHandle* compas_simulate_magnetization_tangent_pssfp(
     const CompasContextHandle* ctx, int field, const CompasObjectHandle* parameters,
     const CompasObjectHandle* sequence);
+
+/* Graph capture (no counterpart in the reference, whose kmm runtime
+ * schedules every call afresh): the library calls made on `ctx` between
+ * compas_graph_begin and compas_graph_end are recorded, not run, into one HIP
+ * graph that compas_graph_launch replays on the context's stream.  Arrays
+ * those calls return are written by every launch; memory the captured calls
+ * allocate stays reserved for the graph until compas_graph_destroy.  The
+ * recorded calls must not synchronise the host (no reads or uploads of
+ * arrays, no compute_residual): run the sequence once before capturing it so
+ * that first-call autotuning is done.  compas_graph_abort ends a capture
+ * that failed.  compas_write_array_complex and compas_copy_array
+ * (stream-ordered device copy) refresh a graph's input arrays between
+ * launches. */
+void compas_graph_begin(const CompasContextHandle* ctx);
+CompasGraphHandle* compas_graph_end(const CompasContextHandle* ctx);
+void compas_graph_abort(const CompasContextHandle* ctx);
+void compas_graph_launch(const CompasGraphHandle* graph);
+void compas_graph_destroy(const CompasGraphHandle* graph);
+void compas_write_array_complex(
+    const CompasContextHandle* ctx, const CompasArrayHandle* array, const compas_cfloat* src, int64_t length);
+void compas_copy_array(const CompasContextHandle* ctx, const CompasArrayHandle* dst, const CompasArrayHandle* src);
 
 /* Request server (no counterpart in the reference, whose host is a Julia
  * session calling one operation at a time): continuous batching on one GPU.

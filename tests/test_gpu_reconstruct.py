@@ -102,3 +102,28 @@ def test_gauss_newton_mixed_precision_descends():
     res = R.gauss_newton(y, start, seq, traj, coils, iterations=6)
     assert res.objective[-1] < 1e-4 * res.objective[0], (res.objective, res.cg_iterations)
     assert np.max(np.abs(res.estimate.T2 / t.T2 - 1)) < 1e-2
+
+
+def test_gauss_newton_graph_replay_is_identical():
+    # the captured normal operator replays the same kernels: the whole fit is
+    # bitwise the same with and without graphs
+    banner("gauss-newton with and without graph replay")
+    context()
+    n, fov = 8, 8.0
+    t = phantom(n, fov, seed=14)
+    seq = ca.FispSequence.from_sequence(W.make_fisp(64, 60.0, 10))
+    traj = ca.CartesianTrajectory.from_trajectory(W.make_cartesian(n, 8, n, fov, 1e-5))
+    coils = W.make_coils(n, fov, 2)
+    params = estimate_of(t).parameters()
+    m = ca.phase_encoding(ca.simulate_magnetization(params, seq), params, traj)
+    y = ca.magnetization_to_signal(m, params, traj, coils).collect()
+    start = estimate_of(t, T1=(t.T1 * 1.2).astype(np.float32), T2=(t.T2 * 0.85).astype(np.float32))
+    a = R.gauss_newton(y, start, seq, traj, coils, iterations=4, use_graph=True)
+    b = R.gauss_newton(y, start, seq, traj, coils, iterations=4, use_graph=False)
+    print("normal operator ms per call: graph %.3f, eager %.3f" % (
+        1e3 * a.normal_seconds / a.normal_calls, 1e3 * b.normal_seconds / b.normal_calls), flush=True)
+    assert a.objective == b.objective
+    assert np.array_equal(a.estimate.T1, b.estimate.T1)
+    assert np.array_equal(a.estimate.T2, b.estimate.T2)
+    assert np.array_equal(a.estimate.rho, b.estimate.rho)
+
