@@ -335,23 +335,34 @@ def spiral_bench(args):
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
     roof = dict(rooflines.get(dominant, {}), kernel=dominant, traffic=None) if dominant in rooflines else None
 
-    # derivatives of the echoes w.r.t. T1, T2, B1, B0: one tangent pass per field
-    # against the reference's forward differences (one extra simulation per field)
+    # derivatives of the echoes w.r.t. T1, T2, B1, B0: the fused tangent pass
+    # (all four fields, optionally the echoes, one recursion), one tangent pass
+    # per field, and the reference's forward differences (one extra simulation
+    # per field beside the echoes)
     fields = (ca.T1, ca.T2, ca.B1, ca.B0)
 
     def timed(fn):
         ctx.synchronize()
         t0 = time.perf_counter()
-        res = [fn(f) for f in fields]
+        res = fn()
         ctx.synchronize()
         return res, 1e3 * (time.perf_counter() - t0)
 
-    timed(lambda f: ca.simulate_magnetization_tangent(f, params, seq))  # warm-up
-    tangents, t_tan = timed(lambda f: ca.simulate_magnetization_tangent(f, params, seq))
-    fds, t_fd = timed(lambda f: ca.simulate_magnetization_derivative(f, echos0, params, seq))
-    derivatives = {"fields": "T1 T2 B1 B0", "tangent_ms": t_tan, "finite_difference_ms": t_fd,
-                   "note": "tangent: exact derivative of the recursion in one pass per field; finite "
-                           "difference: the reference's method (delta 1e-4), one more simulation per field"}
+    timed(lambda: ca.simulate_magnetization_tangents(fields, params, seq, echos=True))  # warm-up
+    fused, t_fused = timed(lambda: ca.simulate_magnetization_tangents(fields, params, seq))
+    fused_e, t_fused_e = timed(lambda: ca.simulate_magnetization_tangents(fields, params, seq, echos=True))
+    del fused_e
+    timed(lambda: [ca.simulate_magnetization_tangent(f, params, seq) for f in fields])  # warm-up
+    tangents, t_tan = timed(lambda: [ca.simulate_magnetization_tangent(f, params, seq) for f in fields])
+    fds, t_fd = timed(lambda: [ca.simulate_magnetization_derivative(f, echos0, params, seq) for f in fields])
+    derivatives = {"fields": "T1 T2 B1 B0", "fused_ms": t_fused, "fused_with_echos_ms": t_fused_e,
+                   "tangent_per_field_ms": t_tan, "finite_difference_ms": t_fd,
+                   "note": "fused: exact derivatives of the recursion, all fields in one pass (with_echos: and "
+                           "the echoes); per field: one tangent pass each; finite difference: the reference's "
+                           "method (delta 1e-4), one more simulation per field, the echoes given"}
+    derivatives["fused_vs_per_field_rel_l2"] = max(
+        float(np.linalg.norm(a.collect() - b.collect()) / np.linalg.norm(b.collect())) for a, b in zip(fused, tangents))
+    del tangents
 
     e, s, jv, jh = (a.collect() for a in out)
     dsn = [d.collect() for d in ds]
@@ -383,7 +394,7 @@ def spiral_bench(args):
         base, acc = spiral_cpu_baseline(wl, e, s, jv, jh, dsn)
         line["cpu_baseline"] = base
         line["accuracy"] = acc
-        derivatives.update(derivative_accuracy(wl, [t.collect() for t in tangents], [f.collect() for f in fds]))
+        derivatives.update(derivative_accuracy(wl, [t.collect() for t in fused], [f.collect() for f in fds]))
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

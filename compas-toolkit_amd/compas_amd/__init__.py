@@ -90,6 +90,10 @@ _SIGNATURES = {
     "compas_compute_jacobian_hermitian_spiral_method": (_vp, [_vp] * 10 + [_i]),
     "compas_simulate_magnetization_tangent_fisp": (_vp, [_vp, _i, _vp, _vp]),
     "compas_simulate_magnetization_tangent_pssfp": (_vp, [_vp, _i, _vp, _vp]),
+    "compas_simulate_magnetization_tangents_fisp": (_i, [_vp, C.POINTER(C.c_int32), _i32, _i32, _vp, _vp,
+                                                         C.POINTER(_vp)]),
+    "compas_simulate_magnetization_tangents_pssfp": (_i, [_vp, C.POINTER(C.c_int32), _i32, _i32, _vp, _vp,
+                                                          C.POINTER(_vp)]),
     "compas_graph_begin": (None, [_vp]),
     "compas_graph_end": (_vp, [_vp]),
     "compas_graph_abort": (None, [_vp]),
@@ -441,6 +445,28 @@ def simulate_magnetization_tangent(field, parameters, sequence) -> CompasArray:
     else:
         raise TypeError("expected a FispSequence or pSSFPSequence")
     return _wrap(fn(ctx.ptr, int(field), parameters.ptr, sequence.ptr))
+
+
+def simulate_magnetization_tangents(fields, parameters, sequence, echos: bool = False):
+    """Forward-mode derivatives w.r.t. several distinct fields (T1, T2, B1,
+    B0) in one call: a list of arrays in the order of `fields`, preceded by
+    the echoes when `echos` is true.  For pSSFP, T1 + T2 and all four fields
+    (with or without the echoes) run as one pass that shares the isochromat
+    recursion among its tangents."""
+    ctx = get_context()
+    if isinstance(sequence, FispSequence):
+        fn = lib().compas_simulate_magnetization_tangents_fisp
+    elif isinstance(sequence, pSSFPSequence):
+        fn = lib().compas_simulate_magnetization_tangents_pssfp
+    else:
+        raise TypeError("expected a FispSequence or pSSFPSequence")
+    f = (C.c_int32 * max(1, len(fields)))(*[int(x) for x in fields])
+    out = (_vp * (len(fields) + 1))()
+    n = fn(ctx.ptr, f, len(fields), 1 if echos else 0, parameters.ptr, sequence.ptr, out)
+    _check()
+    if n != len(fields) + (1 if echos else 0):
+        raise CompasError("library returned an unexpected number of arrays")
+    return [CompasArray(out[i], ctx) for i in range(n)]
 
 
 def simulate_magnetization_derivatives(echos, parameters, sequence, delta=1e-4):

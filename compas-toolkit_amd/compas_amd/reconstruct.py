@@ -7,8 +7,8 @@ to its Julia users: the objective ||S(p) - y||^2 and residual
 This module is that caller loop, run on the HIP path:
 
     per outer iteration
-      m, dm/dp       simulate_magnetization + simulate_magnetization_tangent
-                     (exact derivatives, one pass per parameter)
+      m, dm/dp       simulate_magnetization + simulate_magnetization_tangents
+                     (exact derivatives; pSSFP: all parameters in one pass)
       r = S(m) - y   magnetization_to_signal + compute_residual
       g = Re J^H r   compute_jacobian_hermitian
       (Re J^H J + lambda I) delta = -g    conjugate gradients, one J.v and one
@@ -43,7 +43,7 @@ import time
 from . import (B0, B1, T1, T2, CartesianTrajectory, CompasArray, SpiralTrajectory, TissueParameters,
                as_array, capture, compute_jacobian, compute_jacobian_hermitian, compute_jacobian_hermitian_spiral,
                compute_jacobian_spiral, compute_residual, magnetization_to_signal, phase_encoding,
-               simulate_magnetization, simulate_magnetization_tangent)
+               simulate_magnetization, simulate_magnetization_tangents)
 
 
 @dataclass
@@ -112,7 +112,7 @@ class _Model:
         fields = (T1, T2, B1, B0) if self.spiral else (T1, T2)
         self.params = params
         self.m = m
-        self.d = [self._encode(simulate_magnetization_tangent(f, params, self.sequence), params) for f in fields]
+        self.d = [self._encode(d, params) for d in simulate_magnetization_tangents(fields, params, self.sequence)]
         if self.graph is not None:
             self.graph.close()
             self.graph = None

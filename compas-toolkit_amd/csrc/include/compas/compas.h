@@ -257,6 +257,24 @@ Array<cfloat, 2> simulate_magnetization_tangent(
 Array<cfloat, 2> simulate_magnetization_tangent(
     const CompasContext& ctx, int field, TissueParameters parameters, pSSFPSequence sequence);
 
+// Derivatives with respect to several fields (distinct T1, T2, B1, B0), one
+// array per field in the order given; with echos != nullptr the echoes too.
+// pSSFP: T1 + T2 and all four fields (each with or without the echoes) run as
+// one fused pass sharing the primal recursion; other sets as several passes.
+// FISP: one pass per field (and the plain simulation for the echoes).
+std::vector<Array<cfloat, 2>> simulate_magnetization_tangents(
+    const CompasContext& ctx,
+    const std::vector<int>& fields,
+    TissueParameters parameters,
+    pSSFPSequence sequence,
+    Array<cfloat, 2>* echos = nullptr);
+std::vector<Array<cfloat, 2>> simulate_magnetization_tangents(
+    const CompasContext& ctx,
+    const std::vector<int>& fields,
+    TissueParameters parameters,
+    FISPSequence sequence,
+    Array<cfloat, 2>* echos = nullptr);
+
 // ---------------------------------------------------------------- signal --
 
 // Same enumerators as the reference (signal.h:14-44).  On MI355X:

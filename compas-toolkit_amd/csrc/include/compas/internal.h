@@ -114,8 +114,10 @@ void spiral_hermitian_reduce_partials(const CompasContext& ctx, const cfloat* pa
 // device bytes of partial echoes one pSSFP simulation of this size allocates
 int64_t pssfp_scratch_bytes(const CompasContext& ctx, int nz, int nv, int nTR);
 
-// echos[i] += sum over count partial planes of n elements, in FP64, in order
-void pssfp_sum_groups(const CompasContext& ctx, cfloat* echos, const cfloat* partial, int count, int64_t n);
+// echos[i] += sum over count partial planes of n elements (plane j at
+// partial + j * stride), in FP64, in order
+void pssfp_sum_groups(
+    const CompasContext& ctx, cfloat* echos, const cfloat* partial, int count, int64_t n, int64_t stride);
 
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);

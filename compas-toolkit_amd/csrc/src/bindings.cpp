@@ -227,6 +227,26 @@ CompasArrayHandle* spiral_product(
 
 }  // namespace
 
+namespace {
+
+template<typename Sequence>
+int tangents_impl(
+    const CompasContextHandle* ctx, const int32_t* fields, int32_t nfields, int32_t with_echos,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* sequence, CompasArrayHandle** out) {
+    if (nfields < 0 || (nfields > 0 && fields == nullptr)) throw CompasException("invalid field list");
+    if (out == nullptr) throw CompasException("null output list");
+    std::vector<int> f(fields, fields + nfields);
+    Array<cfloat, 2> echos;
+    auto d = simulate_magnetization_tangents(context(ctx), f, unwrap<TissueParameters>(parameters, "parameters"),
+                                             unwrap<Sequence>(sequence, "sequence"), with_echos ? &echos : nullptr);
+    int n = 0;
+    if (with_echos) out[n++] = new_array(echos);
+    for (auto& a : d) out[n++] = new_array(a);
+    return n;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* compas_version() {
@@ -643,6 +663,20 @@ CompasArrayHandle* compas_simulate_magnetization_tangent_pssfp(
                                                         unwrap<TissueParameters>(parameters, "parameters"),
                                                         unwrap<pSSFPSequence>(sequence, "sequence")));
     });
+}
+
+int compas_simulate_magnetization_tangents_fisp(
+    const CompasContextHandle* ctx, const int32_t* fields, int32_t nfields, int32_t with_echos,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* sequence, CompasArrayHandle** out) {
+    return catch_exceptions(
+        [&] { return tangents_impl<FISPSequence>(ctx, fields, nfields, with_echos, parameters, sequence, out); });
+}
+
+int compas_simulate_magnetization_tangents_pssfp(
+    const CompasContextHandle* ctx, const int32_t* fields, int32_t nfields, int32_t with_echos,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* sequence, CompasArrayHandle** out) {
+    return catch_exceptions(
+        [&] { return tangents_impl<pSSFPSequence>(ctx, fields, nfields, with_echos, parameters, sequence, out); });
 }
 
 // ---- request server

@@ -27,6 +27,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <vector>
 
 #include "compas/device/common.h"
 #include "compas/device/fisp_common.h"
@@ -382,6 +383,25 @@ Array<cfloat, 2> simulate_magnetization_tangent(
     KernelScope ks(ctx, "fisp_epg_tangent");
     fn(a, d.data(), dev::div_ceil(int64_t(nv) * lpv, 64), ctx.stream());
     check_launch("fisp_epg_tangent");
+    return d;
+}
+
+}  // namespace compas
+
+namespace compas {
+
+std::vector<Array<cfloat, 2>> simulate_magnetization_tangents(
+    const CompasContext& ctx,
+    const std::vector<int>& fields,
+    TissueParameters parameters,
+    FISPSequence seq,
+    Array<cfloat, 2>* echos) {
+    for (size_t i = 0; i < fields.size(); i++)
+        for (size_t j = 0; j < i; j++)
+            if (fields[i] == fields[j]) COMPAS_ERROR("a field is listed twice");
+    std::vector<Array<cfloat, 2>> d;
+    for (int f : fields) d.push_back(simulate_magnetization_tangent(ctx, f, parameters, seq));
+    if (echos) *echos = simulate_magnetization(ctx, parameters, seq);
     return d;
 }
 

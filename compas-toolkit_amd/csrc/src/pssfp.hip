@@ -173,12 +173,12 @@ __global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat*
 
 // echos[i] += sum_j partial[j][i], accumulated in FP64 in slot order
 __global__ void pssfp_sum_groups_kernel(
-    cfloat* __restrict__ echos, const cfloat* __restrict__ partial, int count, int64_t n) {
+    cfloat* __restrict__ echos, const cfloat* __restrict__ partial, int count, int64_t n, int64_t stride) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
         const cfloat e = echos[i];
         double re = e.re, im = e.im;
         for (int j = 0; j < count; j++) {
-            const cfloat p = partial[size_t(j) * size_t(n) + size_t(i)];
+            const cfloat p = partial[size_t(j) * size_t(stride) + size_t(i)];
             re += p.re;
             im += p.im;
         }
@@ -188,11 +188,12 @@ __global__ void pssfp_sum_groups_kernel(
 
 }  // namespace kernels
 
-void pssfp_sum_groups(const CompasContext& ctx, cfloat* echos, const cfloat* partial, int count, int64_t n) {
+void pssfp_sum_groups(
+    const CompasContext& ctx, cfloat* echos, const cfloat* partial, int count, int64_t n, int64_t stride) {
     if (count <= 0 || n == 0) return;
     KernelScope ks(ctx, "pssfp_sum_groups");
     const int64_t blocks = std::min<int64_t>(dev::div_ceil(n, int64_t(256)), 8192);
-    kernels::pssfp_sum_groups_kernel<<<unsigned(blocks), 256, 0, ctx.stream()>>>(echos, partial, count, n);
+    kernels::pssfp_sum_groups_kernel<<<unsigned(blocks), 256, 0, ctx.stream()>>>(echos, partial, count, n, stride);
     check_launch("pssfp_sum_groups");
 }
 
@@ -202,9 +203,12 @@ PssfpLayout pssfp_layout(const CompasContext& ctx, int nz, int nv, int nTR, cons
     const int64_t cus = prop.multiProcessorCount;
     const int64_t blocks = dev::div_ceil(int64_t(nv), int64_t(64));
     // partial echoes held at once (one plane per group of a launch, launch 0
-    // writing its first group into the echoes): at most 8 GiB, at least one plane
+    // writing its first group into the echoes): at most a sixth of the
+    // device's memory (48 GB of 288) and at least 8 GiB, and at least one
+    // plane.  A launch split for memory costs whole extra rounds of waves.
     const int64_t plane = int64_t(8) * nTR * std::max(nv, 1);
-    const int cap = int(std::max<int64_t>(1, std::min<int64_t>(1 << 15, (int64_t(8) << 30) / plane)));
+    const int64_t budget = std::max<int64_t>(int64_t(8) << 30, int64_t(prop.totalGlobalMem / 6));
+    const int cap = int(std::max<int64_t>(1, std::min<int64_t>(1 << 15, budget / plane)));
     PssfpLayout best {1, nz, 1};
     double best_cost = 1e300;
     for (int zpl = max_zpl; zpl >= 1; zpl--) {
@@ -328,7 +332,7 @@ Array<cfloat, 2> simulate_magnetization(
             fn(a, g0, out0, rest, dim3(vblocks, unsigned(count)), ctx.stream());
             check_launch("pssfp_iso");
         }
-        pssfp_sum_groups(ctx, echos.data(), part.data(), g0 == 0 ? count - 1 : count, plane);
+        pssfp_sum_groups(ctx, echos.data(), part.data(), g0 == 0 ? count - 1 : count, plane, plane);
     }
     return echos;
 }

@@ -210,6 +210,22 @@ CompasArrayHandle* compas_simulate_magnetization_tangent_pssfp(
     const CompasContextHandle* ctx, int field, const CompasObjectHandle* parameters,
     const CompasObjectHandle* sequence);
 
+/* Derivatives with respect to nfields distinct fields (0 T1, 1 T2, 2 B1,
+ * 3 B0) in one call, and with with_echos != 0 the echoes as well: writes
+ * the echoes (if asked) and then one [nreadouts][nvoxels] array per field, in
+ * the order given, to out[] (room for nfields + 1 handles) and returns the
+ * number written (0 on error).  pSSFP runs T1 + T2 and all four fields, with
+ * or without the echoes, as one pass sharing the isochromat recursion -- where
+ * the reference runs one simulation per field plus the echoes
+ * (compas_simulate_magnetization_derivative_pssfp, julia-bindings/src/
+ * bindings.cpp:376, called once per field).  FISP runs one pass per field. */
+int compas_simulate_magnetization_tangents_fisp(
+    const CompasContextHandle* ctx, const int32_t* fields, int32_t nfields, int32_t with_echos,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* sequence, CompasArrayHandle** out);
+int compas_simulate_magnetization_tangents_pssfp(
+    const CompasContextHandle* ctx, const int32_t* fields, int32_t nfields, int32_t with_echos,
+    const CompasObjectHandle* parameters, const CompasObjectHandle* sequence, CompasArrayHandle** out);
+
 /* Graph capture (no counterpart in the reference, whose kmm runtime
  * schedules every call afresh): the library calls made on `ctx` between
  * compas_graph_begin and compas_graph_end are recorded, not run, into one HIP

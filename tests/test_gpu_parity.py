@@ -551,6 +551,61 @@ def test_pssfp_tangent_derivative(ctx, oracle, field, nz, nRF):
     assert rel(got, central_difference(oracle.pssfp, t, seq, name)) < TANGENT_TOL
 
 
+# field sets of one fused pass (all four, T1 + T2) and of several passes
+FIELD_SETS = [(ca.T1, ca.T2, ca.B1, ca.B0), (ca.T1, ca.T2), (ca.T2, ca.B0, ca.T1), (ca.B1,), (ca.B0, ca.T1)]
+
+
+@pytest.mark.parametrize("fields", FIELD_SETS, ids=lambda f: "".join("T1 T2 B1 B0".split()[i] for i in f))
+@pytest.mark.parametrize("echos", [False, True])
+@pytest.mark.parametrize("nz,nRF", [(7, 25), (40, 2)])
+def test_pssfp_fused_tangents(ctx, oracle, fields, echos, nz, nRF):
+    # several tangents (and the echoes) from one call agree with one field
+    # at a time, with the central differences of the oracle and, for the
+    # echoes, with the plain simulation
+    names = ["T1", "T2", "B1", "B0"]
+    banner(f"pssfp fused tangents {[names[f] for f in fields]} echos={echos} nz={nz} nRF={nRF}")
+    t = W.make_tissue(W.SplitMix64(6), 9, 20.0, 40.0)
+    seq = W.make_pssfp(24, 60.0, nz=nz, nRF=nRF)
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    s = ca.pSSFPSequence.from_sequence(seq)
+    out = ca.simulate_magnetization_tangents(fields, p, s, echos=echos)
+    assert len(out) == len(fields) + int(echos)
+    if echos:
+        e = out.pop(0).collect()
+        assert e.shape == (24, 81)
+        assert rel(e, ca.simulate_magnetization(p, s).collect()) < 1e-6
+    for f, d in zip(fields, out):
+        got = d.collect()
+        assert got.shape == (24, 81)
+        assert rel(got, ca.simulate_magnetization_tangent(f, p, s).collect()) < 1e-6
+        assert rel(got, central_difference(oracle.pssfp, t, seq, names[f])) < TANGENT_TOL
+
+
+def test_fused_tangents_fisp_and_errors(ctx):
+    t, seq = small_fisp(nv=50, nTR=30, N=10)
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    fs = ca.FispSequence.from_sequence(seq)
+    e, d2, d1 = (a.collect() for a in ca.simulate_magnetization_tangents((ca.T2, ca.T1), p, fs, echos=True))
+    assert np.array_equal(e, ca.simulate_magnetization(p, fs).collect())
+    assert np.array_equal(d1, ca.simulate_magnetization_tangent(ca.T1, p, fs).collect())
+    assert np.array_equal(d2, ca.simulate_magnetization_tangent(ca.T2, p, fs).collect())
+    assert ca.simulate_magnetization_tangents((), p, fs) == []
+
+    tp = W.make_tissue(W.SplitMix64(3), 4, 20.0, None)
+    pp = ca.TissueParameters.from_tissue(tp)
+    ps = ca.pSSFPSequence.from_sequence(W.make_pssfp(12, 40.0, nz=3, nRF=4))
+    with pytest.raises(ca.CompasError, match="twice"):
+        ca.simulate_magnetization_tangents((ca.T1, ca.T1), pp, ps)
+    with pytest.raises(ca.CompasError, match="twice"):
+        ca.simulate_magnetization_tangents((ca.B1, ca.B1), p, fs)
+    with pytest.raises(ca.CompasError, match="tangent"):
+        ca.simulate_magnetization_tangents((ca.T1, 4), pp, ps)
+    e, = ca.simulate_magnetization_tangents((), pp, ps, echos=True)
+    assert np.array_equal(e.collect(), ca.simulate_magnetization(pp, ps).collect())
+
+
 def test_empty_inputs(ctx):
     # sums over no voxels are zero; per-voxel outputs are empty; no slice
     # profiles (FISP) or slice positions (pSSFP) give zero echoes
