@@ -94,6 +94,15 @@ _SIGNATURES = {
                                                          C.POINTER(_vp)]),
     "compas_simulate_magnetization_tangents_pssfp": (_i, [_vp, C.POINTER(C.c_int32), _i32, _i32, _vp, _vp,
                                                           C.POINTER(_vp)]),
+    "compas_context_stream": (_vp, [_vp]),
+    "compas_context_wait_stream": (None, [_vp, _vp]),
+    "compas_stream_wait_context": (None, [_vp, _vp]),
+    "compas_wrap_device_array": (_vp, [_vp, _vp, _i32, _i32, _i64p, _vp, _vp]),
+    "compas_array_device_pointer": (_vp, [_vp]),
+    "compas_array_device": (_i32, [_vp]),
+    "compas_array_from_dlpack": (_vp, [_vp, _vp]),
+    "compas_array_to_dlpack": (_vp, [_vp]),
+    "compas_dlpack_delete": (None, [_vp]),
     "compas_graph_begin": (None, [_vp]),
     "compas_graph_end": (_vp, [_vp]),
     "compas_graph_abort": (None, [_vp]),
@@ -109,6 +118,9 @@ _SIGNATURES = {
     "compas_server_result": (_vp, [_vp, C.c_int64, _i]),
     "compas_server_release": (None, [_vp, C.c_int64]),
     "compas_server_now": (C.c_double, [_vp]),
+    "compas_server_submit_wire": (C.c_int64, [_vp, C.c_char_p, C.c_int64]),
+    "compas_server_result_wire": (C.c_int64, [_vp, C.c_int64, _vp, C.c_int64]),
+    "compas_wire_describe": (C.c_int64, [C.c_char_p, C.c_int64, C.c_char_p, C.c_int64]),
     "compas_server_set_kernel_timing": (None, [_vp, _i]),
     "compas_server_read_kernel_timings": (_i, [_vp, C.c_char_p, _i]),
 }
@@ -292,6 +304,96 @@ class CompasArray:
         _check()
         return out
 
+    # ---- zero-copy interop
+
+    @classmethod
+    def from_device(cls, device_ptr: int, dtype, shape, release=None,
+                    context: Optional[Context] = None) -> "CompasArray":
+        """An array over device memory owned elsewhere (no copy): 256-byte
+        aligned, row-major, on the context's GPU.  `release()` is called once
+        the last array over it is gone."""
+        ctx = context or get_context()
+        code = 1 if np.dtype(dtype) == np.complex64 else 0 if np.dtype(dtype) == np.float32 else None
+        if code is None:
+            raise CompasError("dtype must be float32 or complex64")
+        sizes = (C.c_int64 * len(shape))(*shape)
+        cb = None
+        if release is not None:
+            key = [None]
+
+            def hook(_user, key=key):
+                try:
+                    release()
+                finally:
+                    _RELEASE_HOOKS.pop(id(key), None)
+
+            cb = _RELEASE_FN(hook)
+            key[0] = cb
+            _RELEASE_HOOKS[id(key)] = key  # kept alive until the library calls it
+        ptr = lib().compas_wrap_device_array(ctx.ptr, C.c_void_p(device_ptr), code, len(shape), sizes,
+                                             C.cast(cb, C.c_void_p) if cb else None, None)
+        if not ptr and cb is not None:
+            _RELEASE_HOOKS.pop(id(key), None)
+        return _wrap_in(ptr, ctx)
+
+    @classmethod
+    def from_dlpack(cls, obj, context: Optional[Context] = None) -> "CompasArray":
+        """Zero-copy array over a DLPack producer's memory (a torch tensor on
+        the GPU, float32 or complex64, contiguous, 256-byte aligned).  The
+        producer's pending work is ordered before the library's stream."""
+        ctx = context or get_context()
+        if hasattr(obj, "__dlpack__"):
+            if hasattr(obj, "__dlpack_device__") and int(obj.__dlpack_device__()[0]) != 10:  # kDLROCM
+                raise CompasError("DLPack tensor is not ROCm device memory")
+            stream = lib().compas_context_stream(ctx.ptr)
+            _check()
+            capsule = obj.__dlpack__(stream=int(stream or 0))
+        else:
+            capsule = obj
+        if not _capsule_is_valid(capsule, _NAME_DLTENSOR):
+            raise CompasError("expected an unconsumed DLPack capsule (dltensor)")
+        managed = _capsule_get_pointer(capsule, _NAME_DLTENSOR)
+        ptr = lib().compas_array_from_dlpack(ctx.ptr, C.c_void_p(managed))
+        _check()
+        if not ptr:
+            raise CompasError("library returned no array")
+        _capsule_set_name(capsule, _NAME_USED)  # the array owns the tensor now
+        return cls(ptr, ctx)
+
+    def __dlpack__(self, *, stream=None, max_version=None, dl_device=None, copy=None):
+        """DLPack export (legacy capsule, kDLROCM).  `stream`: the consumer's
+        stream, made to wait for the library's queued work (None: the host
+        waits; -1: no ordering)."""
+        if copy:
+            raise BufferError("CompasArray exports its memory without copying")
+        if dl_device is not None and tuple(dl_device) != self.__dlpack_device__():
+            raise BufferError("CompasArray lives on another device")
+        if stream is None:
+            self.context.synchronize()
+        elif stream != -1:
+            lib().compas_stream_wait_context(self.context.ptr, C.c_void_p(int(stream)))
+            _check()
+        managed = lib().compas_array_to_dlpack(self.ptr)
+        _check()
+        return _capsule_new(managed, _NAME_DLTENSOR, _DLPACK_DTOR)
+
+    def __dlpack_device__(self):
+        dev = lib().compas_array_device(self.ptr)
+        _check()
+        return (10, int(dev))  # kDLROCM
+
+    def to_torch(self):
+        """Zero-copy torch tensor over this array's memory (shares it)."""
+        import torch
+
+        return torch.from_dlpack(self)
+
+    @property
+    def device_pointer(self) -> int:
+        p = lib().compas_array_device_pointer(self.ptr)
+        _check()
+        return int(p or 0)
+
     def __del__(self):
         if getattr(self, "ptr", None) and _lib is not None:
             _lib.compas_destroy_array(self.ptr)
@@ -299,6 +401,59 @@ class CompasArray:
 
     def __repr__(self):
         return f"CompasArray({np.dtype(self.dtype).name}, {self.shape})"
+
+
+# PyCapsule helpers for DLPack (ctypes over the CPython API).  A capsule keeps
+# the name pointer it is given, so the names live in module globals.
+_NAME_DLTENSOR = b"dltensor"
+_NAME_USED = b"used_dltensor"
+_RELEASE_FN = C.CFUNCTYPE(None, C.c_void_p)
+_RELEASE_HOOKS: dict = {}
+_capsule_new_fn = C.pythonapi.PyCapsule_New
+_capsule_new_fn.restype = C.py_object
+_capsule_new_fn.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+_capsule_ptr_fn = C.pythonapi.PyCapsule_GetPointer
+_capsule_ptr_fn.restype = C.c_void_p
+_capsule_ptr_fn.argtypes = [C.py_object, C.c_char_p]
+_capsule_valid_fn = C.pythonapi.PyCapsule_IsValid
+_capsule_valid_fn.restype = C.c_int
+_capsule_valid_fn.argtypes = [C.py_object, C.c_char_p]
+_capsule_setname_fn = C.pythonapi.PyCapsule_SetName
+_capsule_setname_fn.restype = C.c_int
+_capsule_setname_fn.argtypes = [C.py_object, C.c_char_p]
+_capsule_valid_raw = C.PYFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p)(("PyCapsule_IsValid", C.pythonapi))
+_capsule_ptr_raw = C.PYFUNCTYPE(C.c_void_p, C.c_void_p, C.c_char_p)(("PyCapsule_GetPointer", C.pythonapi))
+
+
+@C.CFUNCTYPE(None, C.c_void_p)
+def _DLPACK_DTOR(capsule):
+    # an exported capsule nobody consumed: free the tensor it holds
+    if _capsule_valid_raw(capsule, _NAME_DLTENSOR) and _lib is not None:
+        _lib.compas_dlpack_delete(_capsule_ptr_raw(capsule, _NAME_DLTENSOR))
+
+
+def _capsule_new(ptr, name, dtor):
+    return _capsule_new_fn(C.c_void_p(ptr), name, C.cast(dtor, C.c_void_p))
+
+
+def _capsule_get_pointer(capsule, name):
+    return _capsule_ptr_fn(capsule, name)
+
+
+def _capsule_is_valid(capsule, name):
+    return bool(_capsule_valid_fn(capsule, name))
+
+
+def _capsule_set_name(capsule, name):
+    if _capsule_setname_fn(capsule, name) != 0:
+        raise CompasError("could not mark the DLPack capsule as consumed")
+
+
+def _wrap_in(ptr, ctx) -> "CompasArray":
+    _check()
+    if not ptr:
+        raise CompasError("library returned no array")
+    return CompasArray(ptr, ctx)
 
 
 def as_array(a, dtype=None, context: Optional[Context] = None) -> CompasArray:
@@ -658,6 +813,34 @@ class Server:
     def release(self, ticket: int):
         lib().compas_server_release(self.ptr, int(ticket))
         _check()
+
+    def submit_wire(self, frame: bytes) -> int:
+        """Submit a request frame (compas_amd.wire); returns its ticket."""
+        frame = bytes(frame)
+        t = lib().compas_server_submit_wire(self.ptr, frame, len(frame))
+        _check()
+        return int(t)
+
+    def wait_done(self, ticket: int, poll: float = 1.0) -> bool:
+        """Block until the request has finished; True if it succeeded (a
+        failure is reported by its result frame, not raised)."""
+        while True:
+            rc = lib().compas_server_wait(self.ptr, int(ticket), float(poll), None)
+            if rc != 0:
+                ok = lib().compas_last_error() is None
+                lib().compas_clear_error()
+                return ok and rc == 1
+
+    def result_wire(self, ticket: int) -> bytes:
+        """Result frame of a finished request (its arrays read back)."""
+        n = lib().compas_server_result_wire(self.ptr, int(ticket), None, 0)
+        _check()
+        buf = C.create_string_buffer(int(n))
+        n2 = lib().compas_server_result_wire(self.ptr, int(ticket), buf, int(n))
+        _check()
+        if n2 != n:
+            raise CompasError("result frame changed size")
+        return buf.raw[:n]
 
     def now(self) -> float:
         return float(lib().compas_server_now(self.ptr))

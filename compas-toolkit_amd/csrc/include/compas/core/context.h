@@ -19,6 +19,7 @@
 
 #include <array>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <unordered_map>
 #include <memory>
@@ -121,22 +122,28 @@ class Runtime {
 
 std::shared_ptr<Runtime> runtime_for_device(int device);
 
-// Device buffer with shared ownership; released into the runtime's pool.
+// Device buffer with shared ownership; released into the runtime's pool, or,
+// for memory the library does not own (wrapped from another framework), handed
+// back through its release hook when the last reference goes.
 class Buffer {
   public:
     Buffer(std::shared_ptr<Runtime> rt, size_t nbytes);
+    Buffer(std::shared_ptr<Runtime> rt, void* external, size_t nbytes, std::function<void()> release);
     ~Buffer();
     Buffer(const Buffer&) = delete;
     Buffer& operator=(const Buffer&) = delete;
 
     void* data() const { return ptr_; }
     size_t nbytes() const { return nbytes_; }
+    bool external() const { return external_; }
     const std::shared_ptr<Runtime>& runtime() const { return rt_; }
 
   private:
     std::shared_ptr<Runtime> rt_;
     void* ptr_ = nullptr;
     size_t nbytes_ = 0;
+    bool external_ = false;
+    std::function<void()> release_;
 };
 
 // Asynchronous device array (row-major, last index fastest).  Copies share the
@@ -161,7 +168,7 @@ class Array {
 
     void copy_to(T* host) const;
     std::vector<T> copy_to_vector() const {
-        std::vector<T> out(size_t(numel()));
+        std::vector<T> out(static_cast<size_t>(numel()));
         copy_to(out.data());
         return out;
     }
@@ -246,6 +253,28 @@ class CompasContext {
 
     void upload(const std::shared_ptr<Buffer>& buf, const void* host, size_t nbytes) const;
     void fill_zero(void* ptr, size_t nbytes) const;
+
+    // Array over device memory owned elsewhere (no copy): `device_ptr` must be
+    // device memory of this context's GPU, aligned to 256 bytes; `release` runs
+    // once the last array over it is gone.
+    template<typename T, size_t N>
+    Array<T, N> wrap(void* device_ptr, std::array<int64_t, N> sizes, std::function<void()> release) const {
+        int64_t n = 1;
+        for (auto s : sizes) {
+            COMPAS_CHECK(s >= 0);
+            n *= s;
+        }
+        check_wrappable(device_ptr, size_t(n) * sizeof(T));
+        auto buf = std::make_shared<Buffer>(rt_, device_ptr, size_t(n) * sizeof(T), std::move(release));
+        return Array<T, N>(std::move(buf), sizes);
+    }
+    void check_wrappable(const void* device_ptr, size_t nbytes) const;
+
+    // Stream ordering with work outside the library: this context's stream
+    // waits for what is queued on `other` so far, or `other` waits for what
+    // is queued on this context's stream so far (an event, no host wait).
+    void wait_for_stream(hipStream_t other) const;
+    void make_stream_wait(hipStream_t other) const;
 
   private:
     std::shared_ptr<Runtime> rt_;

@@ -72,6 +72,7 @@ class Server {
 
     size_t estimate_bytes(const ServeRequest& r) const;
     double now() const;
+    const CompasContext& base() const { return base_; }  // where callers upload request data
     int slots() const { return int(slots_.size()); }
 
   private:
@@ -87,6 +88,7 @@ class Server {
     void dispatch_loop();
     void launch(Entry& e, int slot);
 
+    CompasContext base_;
     std::vector<CompasContext> slots_;
     std::vector<bool> slot_busy_;
     size_t budget_, in_flight_bytes_ = 0;

@@ -15,6 +15,7 @@
 #include "compas/compas.h"
 #include "compas/internal.h"
 #include "compas/serving.h"
+#include "compas/wire.h"
 #include "compas_amd.h"
 
 // Provided by libjulia when loaded from CompasToolkit.jl; absent otherwise.
@@ -247,6 +248,69 @@ int tangents_impl(
 
 }  // namespace
 
+namespace {
+
+// DLPack (legacy, unversioned) ABI: the exchange structs of dlpack.h v0.8,
+// restated here so the C header carries no dlpack.h dependency.
+struct DLDevice {
+    int32_t device_type;
+    int32_t device_id;
+};
+struct DLDataType {
+    uint8_t code;
+    uint8_t bits;
+    uint16_t lanes;
+};
+struct DLTensor {
+    void* data;
+    DLDevice device;
+    int32_t ndim;
+    DLDataType dtype;
+    int64_t* shape;
+    int64_t* strides;
+    uint64_t byte_offset;
+};
+struct DLManagedTensor {
+    DLTensor dl_tensor;
+    void* manager_ctx;
+    void (*deleter)(DLManagedTensor*);
+};
+constexpr int32_t DL_ROCM = 10;
+constexpr uint8_t DL_FLOAT = 2, DL_COMPLEX = 5;
+
+// An exported array: the DLPack view and a reference to the buffer
+struct ExportedArray {
+    DLManagedTensor managed;
+    int64_t shape[3];
+    std::shared_ptr<Buffer> buf;
+};
+
+void delete_exported(DLManagedTensor* m) {
+    delete static_cast<ExportedArray*>(m->manager_ctx);
+}
+
+ArrayHandle* wrap_external(
+    const CompasContext& ctx, void* ptr, int dtype, int rank, const int64_t* sizes, std::function<void()> release) {
+    if (dtype != DT_FLOAT && dtype != DT_COMPLEX) throw CompasException("dtype must be 0 (float) or 1 (complex)");
+    if (rank < 1 || rank > 3) COMPAS_ERROR("cannot support rank > 3");
+    COMPAS_CHECK(sizes != nullptr);
+    int64_t n = 1;
+    for (int i = 0; i < rank; i++) {
+        COMPAS_CHECK(sizes[i] >= 0);
+        n *= sizes[i];
+    }
+    const size_t nbytes = size_t(n) * (dtype == DT_COMPLEX ? sizeof(cfloat) : sizeof(float));
+    ctx.check_wrappable(ptr, nbytes);
+    auto h = std::make_unique<ArrayHandle>();
+    h->dtype = dtype;
+    h->rank = rank;
+    for (int i = 0; i < rank; i++) h->sizes[i] = sizes[i];
+    h->buf = std::make_shared<Buffer>(ctx.runtime(), ptr, nbytes, std::move(release));  // last: owns from here
+    return h.release();
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* compas_version() {
@@ -328,6 +392,94 @@ void compas_copy_array(const CompasContextHandle* ctx, const CompasArrayHandle* 
         const size_t elem = d.dtype == DT_COMPLEX ? sizeof(cfloat) : sizeof(float);
         COMPAS_HIP_CHECK(hipMemcpyAsync(d.buf->data(), s.buf->data(), size_t(d.numel()) * elem,
                                         hipMemcpyDeviceToDevice, c.stream()));
+    });
+}
+
+void* compas_context_stream(const CompasContextHandle* ctx) {
+    return catch_exceptions([&]() -> void* { return static_cast<void*>(context(ctx).stream()); });
+}
+
+void compas_context_wait_stream(const CompasContextHandle* ctx, void* stream) {
+    catch_exceptions([&] { context(ctx).wait_for_stream(static_cast<hipStream_t>(stream)); });
+}
+
+void compas_stream_wait_context(const CompasContextHandle* ctx, void* stream) {
+    catch_exceptions([&] { context(ctx).make_stream_wait(static_cast<hipStream_t>(stream)); });
+}
+
+const CompasArrayHandle* compas_wrap_device_array(
+    const CompasContextHandle* ctx, void* device_ptr, int32_t dtype, int32_t rank, const int64_t* sizes,
+    compas_release_fn release, void* user) {
+    return catch_exceptions([&]() -> const CompasArrayHandle* {
+        std::function<void()> hook;
+        if (release) hook = [release, user] { release(user); };
+        return reinterpret_cast<const CompasArrayHandle*>(
+            static_cast<const Handle*>(wrap_external(context(ctx), device_ptr, dtype, rank, sizes, std::move(hook))));
+    });
+}
+
+int32_t compas_array_device(const CompasArrayHandle* a) {
+    return catch_exceptions([&]() -> int32_t { return array(a, "array").buf->runtime()->device(); });
+}
+
+void compas_dlpack_delete(void* dl_managed_tensor) {
+    catch_exceptions([&] {
+        auto* m = static_cast<DLManagedTensor*>(dl_managed_tensor);
+        if (m && m->deleter) m->deleter(m);
+    });
+}
+
+void* compas_array_device_pointer(const CompasArrayHandle* a) {
+    return catch_exceptions([&]() -> void* { return array(a, "array").buf->data(); });
+}
+
+const CompasArrayHandle* compas_array_from_dlpack(const CompasContextHandle* ctx, void* managed_tensor) {
+    return catch_exceptions([&]() -> const CompasArrayHandle* {
+        if (managed_tensor == nullptr) throw CompasException("null DLPack tensor");
+        auto* m = static_cast<DLManagedTensor*>(managed_tensor);
+        const DLTensor& t = m->dl_tensor;
+        const CompasContext& c = context(ctx);
+        if (t.device.device_type != DL_ROCM) throw CompasException("DLPack tensor is not ROCm device memory");
+        if (t.device.device_id != c.device()) throw CompasException("DLPack tensor is on another GPU than the context's");
+        int dtype;
+        if (t.dtype.code == DL_FLOAT && t.dtype.bits == 32 && t.dtype.lanes == 1) dtype = DT_FLOAT;
+        else if (t.dtype.code == DL_COMPLEX && t.dtype.bits == 64 && t.dtype.lanes == 1) dtype = DT_COMPLEX;
+        else throw CompasException("DLPack tensor must be float32 or complex64");
+        if (t.ndim < 1 || t.ndim > 3) throw CompasException("DLPack tensor must have rank 1 to 3");
+        if (t.strides) {  // row-major and compact (size-1 axes may carry any stride)
+            int64_t expect = 1;
+            for (int i = t.ndim - 1; i >= 0; i--) {
+                if (t.shape[i] != 1 && t.strides[i] != expect) throw CompasException("DLPack tensor is not contiguous");
+                expect *= t.shape[i];
+            }
+        }
+        void* data = static_cast<char*>(t.data) + t.byte_offset;
+        // the array owns the tensor once built (the hook runs only from its
+        // buffer); when a check fails the caller still owns it
+        auto* h = wrap_external(c, data, dtype, t.ndim, t.shape, [m] {
+            if (m->deleter) m->deleter(m);
+        });
+        return reinterpret_cast<const CompasArrayHandle*>(static_cast<const Handle*>(h));
+    });
+}
+
+void* compas_array_to_dlpack(const CompasArrayHandle* a) {
+    return catch_exceptions([&]() -> void* {
+        const ArrayHandle& h = array(a, "array");
+        auto* e = new ExportedArray();
+        e->buf = h.buf;
+        for (int i = 0; i < 3; i++) e->shape[i] = h.sizes[i];
+        DLTensor& t = e->managed.dl_tensor;
+        t.data = h.buf->data();
+        t.device = {DL_ROCM, h.buf->runtime()->device()};
+        t.ndim = h.rank;
+        t.dtype = h.dtype == DT_COMPLEX ? DLDataType {DL_COMPLEX, 64, 1} : DLDataType {DL_FLOAT, 32, 1};
+        t.shape = e->shape;
+        t.strides = nullptr;  // row-major, compact
+        t.byte_offset = 0;
+        e->managed.manager_ctx = e;
+        e->managed.deleter = &delete_exported;
+        return &e->managed;
     });
 }
 
@@ -687,6 +839,12 @@ struct ServerHandle {
     std::unique_ptr<Server> server;
 };
 
+// The result frame of the last size query on this thread, so the usual
+// query-then-copy pair of compas_server_result_wire reads the result back once.
+thread_local Server* cached_server = nullptr;
+thread_local int64_t cached_ticket = -1;
+thread_local std::vector<uint8_t> cached;
+
 Server& server_of(CompasServerHandle* s) {
     if (s == nullptr) throw CompasException("null server");
     return *reinterpret_cast<ServerHandle*>(s)->server;
@@ -727,7 +885,15 @@ CompasServerHandle* compas_server_create(const CompasContextHandle* ctx, int slo
 }
 
 void compas_server_destroy(CompasServerHandle* server) {
-    catch_exceptions([&] { delete reinterpret_cast<ServerHandle*>(server); });
+    catch_exceptions([&] {
+        auto* h = reinterpret_cast<ServerHandle*>(server);
+        if (h && cached_server == h->server.get()) {
+            cached_server = nullptr;
+            cached_ticket = -1;
+            std::vector<uint8_t>().swap(cached);
+        }
+        delete h;
+    });
 }
 
 int64_t compas_server_submit_simulate_signal(
@@ -756,6 +922,51 @@ int64_t compas_server_submit_jacobian_hermitian(
         return server_of(server).submit(std::move(r));
     });
     return g_last_error.empty() ? t : -1;
+}
+
+int64_t compas_server_submit_wire(CompasServerHandle* server, const void* frame, int64_t nbytes) {
+    const int64_t t = catch_exceptions([&]() -> int64_t {
+        Server& s = server_of(server);
+        if (nbytes < 0) throw CompasException("negative frame size");
+        const wire::Frame f = wire::parse(frame, size_t(nbytes), wire::REQUEST_MAGIC);
+        return s.submit(wire::upload_request(s.base(), f));
+    });
+    return g_last_error.empty() ? t : -1;
+}
+
+int64_t compas_server_result_wire(CompasServerHandle* server, int64_t ticket, void* buf, int64_t capacity) {
+    const int64_t n = catch_exceptions([&]() -> int64_t {
+        Server& s = server_of(server);
+        if (cached_server != &s || cached_ticket != ticket) {
+            if (!s.wait(ticket, 0.0)) throw CompasException("request has not finished");
+            cached = wire::encode_result(ticket, s.result(ticket));
+            cached_server = &s;
+            cached_ticket = ticket;
+        }
+        const int64_t n = int64_t(cached.size());
+        if (buf != nullptr && capacity >= n) {
+            std::memcpy(buf, cached.data(), cached.size());
+            cached_server = nullptr;
+            cached_ticket = -1;
+            std::vector<uint8_t>().swap(cached);
+        }
+        return n;
+    });
+    return g_last_error.empty() ? n : -1;
+}
+
+int64_t compas_wire_describe(const void* frame, int64_t nbytes, char* out, int64_t capacity) {
+    const int64_t n = catch_exceptions([&]() -> int64_t {
+        if (nbytes < 0) throw CompasException("negative frame size");
+        const std::string s = wire::describe_request(wire::parse(frame, size_t(nbytes), wire::REQUEST_MAGIC));
+        if (out != nullptr && capacity > 0) {
+            const size_t k = std::min(size_t(capacity - 1), s.size());
+            std::memcpy(out, s.data(), k);
+            out[k] = '\0';
+        }
+        return int64_t(s.size() + 1);
+    });
+    return g_last_error.empty() ? n : -1;
 }
 
 int compas_server_wait(CompasServerHandle* server, int64_t ticket, double timeout_seconds, double* times) {

@@ -328,8 +328,61 @@ Buffer::Buffer(std::shared_ptr<Runtime> rt, size_t nbytes) : rt_(std::move(rt)),
     ptr_ = rt_->allocate(nbytes);
 }
 
+Buffer::Buffer(std::shared_ptr<Runtime> rt, void* external, size_t nbytes, std::function<void()> release) :
+    rt_(std::move(rt)),
+    ptr_(external),
+    nbytes_(nbytes),
+    external_(true),
+    release_(std::move(release)) {}
+
 Buffer::~Buffer() {
-    rt_->release(ptr_, nbytes_);
+    if (!external_) {
+        rt_->release(ptr_, nbytes_);
+        return;
+    }
+    if (release_) {
+        try {
+            // queued kernels may still read the memory: let them finish before
+            // its owner may reuse it (arrays a captured graph reads must outlive
+            // the graph, wrapped or not)
+            if (!rt_->capturing()) rt_->synchronize();
+            release_();
+        } catch (...) {  // a destructor must not throw; the owner's hook failed
+        }
+    }
+}
+
+void CompasContext::check_wrappable(const void* device_ptr, size_t nbytes) const {
+    if (nbytes == 0) return;
+    if (device_ptr == nullptr) COMPAS_ERROR("null device pointer");
+    if (reinterpret_cast<uintptr_t>(device_ptr) % 256 != 0)
+        COMPAS_ERROR("wrapped device memory must be aligned to 256 bytes");
+    activate();
+    hipPointerAttribute_t attr {};
+    if (hipPointerGetAttributes(&attr, device_ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        COMPAS_ERROR("pointer is not device memory known to HIP");
+    }
+    if (attr.type != hipMemoryTypeDevice) COMPAS_ERROR("pointer is not device memory");
+    if (attr.device != device()) COMPAS_ERROR("device memory belongs to another GPU than the context's");
+}
+
+void CompasContext::wait_for_stream(hipStream_t other) const {
+    activate();
+    hipEvent_t ev;
+    COMPAS_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    COMPAS_HIP_CHECK(hipEventRecord(ev, other));
+    COMPAS_HIP_CHECK(hipStreamWaitEvent(stream(), ev, 0));
+    COMPAS_HIP_CHECK(hipEventDestroy(ev));
+}
+
+void CompasContext::make_stream_wait(hipStream_t other) const {
+    activate();
+    hipEvent_t ev;
+    COMPAS_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    COMPAS_HIP_CHECK(hipEventRecord(ev, stream()));
+    COMPAS_HIP_CHECK(hipStreamWaitEvent(other, ev, 0));
+    COMPAS_HIP_CHECK(hipEventDestroy(ev));
 }
 
 void copy_device_to_host(const std::shared_ptr<Buffer>& buf, void* host, size_t nbytes) {

@@ -38,7 +38,7 @@ Array<cfloat, 2> derivative(const CompasContext& ctx, int field, const Array<cfl
 }  // namespace
 
 Server::Server(const CompasContext& base, int slots, size_t memory_budget) :
-    budget_(memory_budget), t0_(std::chrono::steady_clock::now()) {
+    base_(base), budget_(memory_budget), t0_(std::chrono::steady_clock::now()) {
     COMPAS_CHECK(slots >= 1 && slots <= 32);
     base.activate();
     for (int i = 0; i < slots; i++) slots_.emplace_back(std::make_shared<Runtime>(base.device()));

@@ -247,6 +247,44 @@ void compas_write_array_complex(
     const CompasContextHandle* ctx, const CompasArrayHandle* array, const compas_cfloat* src, int64_t length);
 void compas_copy_array(const CompasContextHandle* ctx, const CompasArrayHandle* dst, const CompasArrayHandle* src);
 
+/* Zero-copy interop (no counterpart in the reference, where arrays cross
+ * only by copies: compas_make_array_* and compas_read_array_*, julia-
+ * bindings/src/bindings.cpp:50-107).
+ *
+ * compas_wrap_device_array: an array over device memory owned by the caller
+ * (a Julia ROCArray, a torch tensor, ...), no copy.  dtype 0 float, 1
+ * complex; the memory must be device memory of the context's GPU, aligned to
+ * 256 bytes, row-major.  `release(user)` runs once the last array over it is
+ * destroyed, after the context's stream has finished the work queued on it
+ * (NULL: nothing is called).
+ * compas_array_device_pointer: the device address of an array, for the
+ * caller to wrap (valid while the array handle lives).
+ * compas_array_from_dlpack / compas_array_to_dlpack: the same through DLPack
+ * (legacy DLManagedTensor, device type kDLROCM, float32 or complex64,
+ * compact row-major).  from_dlpack takes ownership of the tensor on success
+ * and calls its deleter when the last array over it is gone; on failure
+ * (NULL returned) the caller keeps it.  to_dlpack returns a new
+ * DLManagedTensor whose deleter drops its reference to the array's memory.
+ *
+ * Stream order: the library runs on the context's stream
+ * (compas_context_stream).  Before the library reads memory written on
+ * another stream, call compas_context_wait_stream(ctx, that stream); before
+ * another stream reads a library result, call compas_stream_wait_context(ctx,
+ * that stream).  Both record an event and enqueue a wait; the host does not
+ * block. */
+typedef void (*compas_release_fn)(void* user);
+void* compas_context_stream(const CompasContextHandle* ctx);
+void compas_context_wait_stream(const CompasContextHandle* ctx, void* stream);
+void compas_stream_wait_context(const CompasContextHandle* ctx, void* stream);
+const CompasArrayHandle* compas_wrap_device_array(
+    const CompasContextHandle* ctx, void* device_ptr, int32_t dtype, int32_t rank, const int64_t* sizes,
+    compas_release_fn release, void* user);
+void* compas_array_device_pointer(const CompasArrayHandle* array);
+int32_t compas_array_device(const CompasArrayHandle* array); /* GPU ordinal of the array's memory */
+void compas_dlpack_delete(void* dl_managed_tensor);         /* calls its deleter (unconsumed exports) */
+const CompasArrayHandle* compas_array_from_dlpack(const CompasContextHandle* ctx, void* dl_managed_tensor);
+void* compas_array_to_dlpack(const CompasArrayHandle* array);
+
 /* Request server (no counterpart in the reference, whose host is a Julia
  * session calling one operation at a time): continuous batching on one GPU.
  * `slots` requests run concurrently, each on a private stream and memory
@@ -281,6 +319,22 @@ double compas_server_now(CompasServerHandle* server);
  * has the format of compas_read_kernel_timings (merged over the slots). */
 void compas_server_set_kernel_timing(CompasServerHandle* server, int enabled);
 int compas_server_read_kernel_timings(CompasServerHandle* server, char* buf, int buflen);
+
+/* Wire format of the server (no counterpart in the reference, which has no
+ * wire or on-disk format): a request arrives as one self-describing binary
+ * frame of named, typed arrays (layout in csrc/include/compas/wire.h) and
+ * its result leaves as one.  compas_server_submit_wire parses and checks the
+ * frame (every size against the buffer), uploads its arrays and submits the
+ * request: a ticket, or -1 (compas_last_error) for a malformed or
+ * inconsistent frame.  compas_server_result_wire encodes a finished
+ * request's result (or its error) into buf when `capacity` suffices and
+ * returns the frame's size in bytes (-1 if the request is unknown or not
+ * finished).  compas_wire_describe checks a request frame without a GPU and
+ * writes a one-line JSON summary (kind, sizes, payload sums) into out;
+ * returns the bytes needed, -1 on a malformed frame. */
+int64_t compas_server_submit_wire(CompasServerHandle* server, const void* frame, int64_t nbytes);
+int64_t compas_server_result_wire(CompasServerHandle* server, int64_t ticket, void* buf, int64_t capacity);
+int64_t compas_wire_describe(const void* frame, int64_t nbytes, char* out, int64_t capacity);
 
 /* Per-kernel timing with HIP events on the context's stream.  While enabled,
  * every library launch is bracketed by events; compas_read_kernel_timings
