@@ -240,8 +240,37 @@ def products_bench(args):
         "kernels": kernels,
         "contraction_tuning": ca.contraction_tuning_report().strip().splitlines(),
         "roofline": roof,
+        "derivatives": fisp_derivative_timing(ctx, params, seq, echos),
     }
     print(json.dumps(line))
+
+
+def fisp_derivative_timing(ctx, params, seq, echos):
+    """Wall time of the FISP echo derivatives at these shapes: the fused tangent
+    pass (T1 + T2, and all four fields), one tangent pass per field, and the
+    reference's forward differences (one more simulation per field)."""
+    def timed(fn):
+        fn()  # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = fn()
+        ctx.synchronize()
+        return res, 1e3 * (time.perf_counter() - t0)
+
+    two, four = (ca.T1, ca.T2), (ca.T1, ca.T2, ca.B1, ca.B0)
+    fused2, t_f2 = timed(lambda: ca.simulate_magnetization_tangents(two, params, seq))
+    _, t_f2e = timed(lambda: ca.simulate_magnetization_tangents(two, params, seq, echos=True))
+    _, t_f4 = timed(lambda: ca.simulate_magnetization_tangents(four, params, seq))
+    per, t_per = timed(lambda: [ca.simulate_magnetization_tangent(f, params, seq) for f in two])
+    _, t_fd = timed(lambda: [ca.simulate_magnetization_derivative(f, echos, params, seq) for f in two])
+    _, t_sim = timed(lambda: ca.simulate_magnetization(params, seq))
+    diff = max(float(np.linalg.norm(a.collect() - b.collect()) / np.linalg.norm(b.collect()))
+               for a, b in zip(fused2, per))
+    return {"T1T2_fused_ms": t_f2, "T1T2_fused_with_echos_ms": t_f2e, "T1T2B1B0_fused_ms": t_f4,
+            "T1T2_per_field_ms": t_per, "T1T2_finite_difference_ms": t_fd, "echos_ms": t_sim,
+            "fused_vs_per_field_rel_l2": diff,
+            "note": "FISP at the products shapes; finite difference: the reference's method (delta 1e-4), one "
+                    "more simulation per field, the echoes given"}
 
 
 def spiral_bench(args):

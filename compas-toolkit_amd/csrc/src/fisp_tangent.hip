@@ -2,11 +2,15 @@
 //
 // The reference differentiates by forward finite differences in FP32
 // (src/simulate/derivative.cu:29-59, delta = 1e-4), which loses about three
-// digits to cancellation.  This kernel runs the recursion of fisp.hip together
-// with its tangent with respect to one tissue parameter p in {T1, T2, B1, B0}:
-// every state (F+, F-, Z) carries dF+/dp, dF-/dp, dZ/dp, updated by the same
-// per-TR matrix plus the derivative of that matrix applied to the state.  The
-// result is the exact derivative of the discrete recursion (no step size).
+// digits to cancellation and costs one more simulation per parameter.  This
+// kernel runs the recursion of fisp.hip together with its tangents with
+// respect to any set of tissue parameters p in {T1, T2, B1, B0}, in one pass:
+// every state (F+, F-, Z) carries dF+/dp, dF-/dp, dZ/dp per parameter,
+// updated by the same per-TR matrix plus the derivative of that matrix
+// applied to the state.  The result is the exact derivative of the discrete
+// recursion (no step size).  The per-TR matrices, the primal state and its
+// next value are shared by all tangents of a pass; the echoes can be written
+// by the same pass.
 //
 // How each parameter enters one TR (M = D(TR) R(alpha), rows 0/1 of M mix the
 // transverse components with the complex decay d = E2 exp(i 2 pi t B0), row 2
@@ -21,30 +25,82 @@
 // The TE sample, the inversion (T1 only: the inversion ignores B1) and the
 // end-of-repetition decay over TW differentiate the same way.
 //
-// Layout: LPV lanes per voxel (a power of two up to 32) with SPL = 4 states
-// each, so the primal and tangent states (12 floats per state) stay in
-// registers; max_state <= 128.
+// Layout: LPV lanes per voxel (a power of two up to 64) with SPL states each:
+// 4 with up to two tangents, 2 with more, so the primal and tangent states
+// (6 floats per state and vector) stay in registers; max_state <= 128.
 
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "compas/device/common.h"
 #include "compas/device/fisp_common.h"
+#include "compas/device/tangent_set.h"
 #include "compas/internal.h"
 
 namespace compas {
 namespace kernels {
-
-enum { TAN_T1 = 0, TAN_T2 = 1, TAN_B1 = 2, TAN_B0 = 3 };
 
 // i z
 __device__ inline cfloat times_i(cfloat z) {
     return {-z.im, z.re};
 }
 
-template<int SPL, int LPV, int FIELD>
-__global__ void __launch_bounds__(64) fisp_epg_tangent(FispArgs a, cfloat* __restrict__ dechos) {
+struct FispPlanes {
+    cfloat* p[5];  // [nTR * usf][nvoxels] each, in the order of TangentSet
+};
+
+// The three rows of a 3x3 real matrix applied to an EPG state (F+, F-, Z)
+// in the (S = F+ + F-, D = F+ - F-) form the recursion uses: rows 0/1 act on
+// S with half weights h, on D through i, row 2 likewise (fisp.hip).
+struct Rows {
+    float h00, h01, m02, h10, h11, m12, h20, h21, m22;
+};
+
+__device__ inline void apply_rows(const Rows& r, cfloat S, cfloat D, cfloat z, cfloat& Pn, cfloat& Qn, cfloat& Zn) {
+    const float xr = fmaf(r.h00, S.re, fmaf(r.h01, D.im, r.m02 * z.re));
+    const float xi = fmaf(r.h00, S.im, fmaf(-r.h01, D.re, r.m02 * z.im));
+    const float yr = fmaf(r.h10, S.re, fmaf(r.h11, D.im, r.m12 * z.re));
+    const float yi = fmaf(r.h10, S.im, fmaf(-r.h11, D.re, r.m12 * z.im));
+    const float zr = fmaf(r.h20, S.re, fmaf(r.h21, D.im, r.m22 * z.re));
+    const float zi = fmaf(r.h20, S.im, fmaf(-r.h21, D.re, r.m22 * z.im));
+    Pn = {xr - yi, xi + yr};
+    Qn = {xr + yi, xi - yr};
+    Zn = {zr, zi};
+}
+
+// rows 0/1 only (the TE sample: the echo is F+ of state 0)
+__device__ inline cfloat sample_rows(const Rows& r, cfloat S, cfloat D, cfloat z) {
+    const float xr = fmaf(r.h00, S.re, fmaf(r.h01, D.im, r.m02 * z.re));
+    const float xi = fmaf(r.h00, S.im, fmaf(-r.h01, D.re, r.m02 * z.im));
+    const float yr = fmaf(r.h10, S.re, fmaf(r.h11, D.im, r.m12 * z.re));
+    const float yi = fmaf(r.h10, S.im, fmaf(-r.h11, D.re, r.m12 * z.im));
+    return {xr - yi, xi + yr};
+}
+
+// rows 0/1 of D(d) R and row 2 of E1 R, with the half weights on columns 0/1
+__device__ inline Rows decay_rows(
+    cfloat d, float E1, float R00, float R01, float R02, float R10, float R11, float R12, float R20, float R21,
+    float R22) {
+    Rows r;
+    r.h00 = 0.5f * (d.re * R00 - d.im * R10);
+    r.h01 = 0.5f * (d.re * R01 - d.im * R11);
+    r.m02 = d.re * R02 - d.im * R12;
+    r.h10 = 0.5f * (d.im * R00 + d.re * R10);
+    r.h11 = 0.5f * (d.im * R01 + d.re * R11);
+    r.m12 = d.im * R02 + d.re * R12;
+    r.h20 = 0.5f * E1 * R20;
+    r.h21 = 0.5f * E1 * R21;
+    r.m22 = E1 * R22;
+    return r;
+}
+
+template<int SPL, int LPV, int MASK>
+__global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes out) {
+    using TS = TangentSet<MASK>;
+    constexpr int NT = TS::nt, NP = TS::planes, OFS = int(TS::echo);
+    static_assert(NT >= 1, "a tangent pass carries at least one tangent");
     const int sub = (LPV > 1) ? int(threadIdx.x % LPV) : 0;
     const int voxel = (blockIdx.x * 64 + threadIdx.x) / LPV;
     if (voxel >= a.tissue.nvoxels) return;  // whole LPV groups leave together
@@ -70,19 +126,23 @@ __global__ void __launch_bounds__(64) fisp_epg_tangent(FispArgs a, cfloat* __res
     const float dE1_TE = a.TE * iT1s * E1_TE, dE1_TRmTE = (a.TR - a.TE) * iT1s * E1_TRmTE;
     const float dE1_TI = a.TI * iT1s * E1_TI, dE1_W = a.TW * iT1s * E1_W;
     const float dG = dE1_TRmTE * (1.0f - E1_TE) - E1_TRmTE * dE1_TE - dE1_TRmTE;
-    const float kTR = (FIELD == TAN_T2) ? a.TR * iT2s : dev::TWO_PI_F * a.TR;  // T2 / B0 factor over TR
-    const float kTE = (FIELD == TAN_T2) ? a.TE * iT2s : dev::TWO_PI_F * a.TE;
-    const float kTW = (FIELD == TAN_T2) ? a.TW * iT2s : dev::TWO_PI_F * a.TW;
-    const float lTR = a.TR * iT1s;  // T1 factor of M row 2
+    const float lTR = a.TR * iT1s;  // T1: factor of M row 2
+    // T2: d/dT2 of the decay d over t is (t/T2^2) d; B0: i 2 pi t d
+    const float k2TR = a.TR * iT2s, k2TE = a.TE * iT2s, k2TW = a.TW * iT2s;
+    const float k0TR = dev::TWO_PI_F * a.TR, k0TE = dev::TWO_PI_F * a.TE, k0TW = dev::TWO_PI_F * a.TW;
 
     const bool owns_zero = (sub == 0);
     const int k0 = sub * SPL;
 
     for (int slice = 0; slice < a.nslices; slice++) {
         const float4* tab = a.table + size_t(slice) * size_t(a.nTR);
-        cfloat P[SPL], Q[SPL], Z[SPL], dP[SPL], dQ[SPL], dZ[SPL];
+        cfloat P[SPL], Q[SPL], Z[SPL], dP[NT][SPL], dQ[NT][SPL], dZ[NT][SPL];
 #pragma unroll
-        for (int i = 0; i < SPL; i++) P[i] = Q[i] = Z[i] = dP[i] = dQ[i] = dZ[i] = {0, 0};
+        for (int i = 0; i < SPL; i++) {
+            P[i] = Q[i] = Z[i] = {0, 0};
+#pragma unroll
+            for (int k = 0; k < NT; k++) dP[k][i] = dQ[k][i] = dZ[k][i] = {0, 0};
+        }
         if (owns_zero) Z[0] = {1.0f, 0.0f};
 
         for (int rep = 0; rep < a.repetitions; rep++) {
@@ -90,14 +150,18 @@ __global__ void __launch_bounds__(64) fisp_epg_tangent(FispArgs a, cfloat* __res
             if (a.inversion) {  // invert (ignores B1), spoil, decay(TI), regrow
 #pragma unroll
                 for (int i = 0; i < SPL; i++) {
-                    P[i] = Q[i] = dP[i] = dQ[i] = {0, 0};
-                    if constexpr (FIELD == TAN_T1) dZ[i] = dZ[i] * (-E1_TI) - Z[i] * dE1_TI;
-                    else dZ[i] = dZ[i] * (-E1_TI);
+                    P[i] = Q[i] = {0, 0};
+#pragma unroll
+                    for (int k = 0; k < NT; k++) {
+                        dP[k][i] = dQ[k][i] = {0, 0};
+                        dZ[k][i] = dZ[k][i] * (-E1_TI);
+                        if (TS::t1 && k == TS::it1) dZ[k][i] = dZ[k][i] - Z[i] * dE1_TI;
+                    }
                     Z[i] = Z[i] * (-E1_TI);
                 }
                 if (owns_zero) {
                     Z[0].re += 1.0f - E1_TI;
-                    if constexpr (FIELD == TAN_T1) dZ[0].re -= dE1_TI;
+                    if constexpr (TS::t1) dZ[TS::it1][0].re -= dE1_TI;
                 }
             }
 
@@ -112,188 +176,137 @@ __global__ void __launch_bounds__(64) fisp_epg_tangent(FispArgs a, cfloat* __res
                 const float R00 = 1.0f - omc * sp * sp, R01 = omc * cp * sp, R02 = sa * sp;
                 const float R10 = R01, R11 = 1.0f - omc * cp * cp, R12 = -sa * cp;
                 const float R20 = -sa * sp, R21 = sa * cp, R22 = 1.0f - omc;
-                const float dr = d_TR.re, di = d_TR.im;
-                const float M00 = dr * R00 - di * R10, M01 = dr * R01 - di * R11, M02 = dr * R02 - di * R12;
-                const float M10 = di * R00 + dr * R10, M11 = di * R01 + dr * R11, M12 = di * R02 + dr * R12;
-                const float M20 = E1_TR * R20, M21 = E1_TR * R21, M22 = E1_TR * R22;
-                const float h00 = 0.5f * M00, h01 = 0.5f * M01;
-                const float h10 = 0.5f * M10, h11 = 0.5f * M11;
-                const float h20 = 0.5f * M20, h21 = 0.5f * M21;
-                // dR/dB1 = |RF| dR/dalpha (d sin(alpha) = cos(alpha), d(1 - cos(alpha)) = sin(alpha))
-                float N00 = 0, N01 = 0, N02 = 0, N10 = 0, N11 = 0, N12 = 0, N20 = 0, N21 = 0, N22 = 0;
-                float S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S12 = 0;
-                if constexpr (FIELD == TAN_B1) {
+                const Rows M = decay_rows(d_TR, E1_TR, R00, R01, R02, R10, R11, R12, R20, R21, R22);
+                // B1: alpha = B1 |RF|, dR/dB1 = |RF| dR/dalpha (d sin = cos, d(1 - cos) = sin)
+                Rows N {}, Sb {};
+                if constexpr (TS::b1) {
                     const float ca = 1.0f - omc;
                     const float dR00 = -sa * sp * sp * t.x, dR01 = sa * cp * sp * t.x, dR02 = ca * sp * t.x;
                     const float dR10 = dR01, dR11 = -sa * cp * cp * t.x, dR12 = -ca * cp * t.x;
                     const float dR20 = -ca * sp * t.x, dR21 = ca * cp * t.x, dR22 = -sa * t.x;
-                    N00 = 0.5f * (dr * dR00 - di * dR10);
-                    N01 = 0.5f * (dr * dR01 - di * dR11);
-                    N02 = dr * dR02 - di * dR12;
-                    N10 = 0.5f * (di * dR00 + dr * dR10);
-                    N11 = 0.5f * (di * dR01 + dr * dR11);
-                    N12 = di * dR02 + dr * dR12;
-                    N20 = 0.5f * E1_TR * dR20;
-                    N21 = 0.5f * E1_TR * dR21;
-                    N22 = E1_TR * dR22;
-                    // rows 0/1 of D(TE) dR for the sample
-                    const float er = d_TE.re, ei = d_TE.im;
-                    S00 = 0.5f * (er * dR00 - ei * dR10);
-                    S01 = 0.5f * (er * dR01 - ei * dR11);
-                    S02 = er * dR02 - ei * dR12;
-                    S10 = 0.5f * (ei * dR00 + er * dR10);
-                    S11 = 0.5f * (ei * dR01 + er * dR11);
-                    S12 = ei * dR02 + er * dR12;
+                    N = decay_rows(d_TR, E1_TR, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
+                    Sb = decay_rows(d_TE, 0.0f, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
                 }
 
                 if (sample && owns_zero) {
-                    const float er = d_TE.re, ei = d_TE.im;
-                    const float T00 = 0.5f * (er * R00 - ei * R10), T01 = 0.5f * (er * R01 - ei * R11);
-                    const float T02 = er * R02 - ei * R12;
-                    const float T10 = 0.5f * (ei * R00 + er * R10), T11 = 0.5f * (ei * R01 + er * R11);
-                    const float T12 = ei * R02 + er * R12;
-                    const cfloat S0 = P[0] + Q[0], D0 = P[0] - Q[0], z0 = Z[0];
-                    const cfloat dS0 = dP[0] + dQ[0], dD0 = dP[0] - dQ[0], dz0 = dZ[0];
-                    // primal echo (needed by the T2 / B0 terms) and the tangent of the state term
-                    const float xr = fmaf(T00, S0.re, fmaf(T01, D0.im, T02 * z0.re));
-                    const float xi = fmaf(T00, S0.im, fmaf(-T01, D0.re, T02 * z0.im));
-                    const float yr = fmaf(T10, S0.re, fmaf(T11, D0.im, T12 * z0.re));
-                    const float yi = fmaf(T10, S0.im, fmaf(-T11, D0.re, T12 * z0.im));
-                    const cfloat echo = {xr - yi, xi + yr};
-                    const float dxr = fmaf(T00, dS0.re, fmaf(T01, dD0.im, T02 * dz0.re));
-                    const float dxi = fmaf(T00, dS0.im, fmaf(-T01, dD0.re, T02 * dz0.im));
-                    const float dyr = fmaf(T10, dS0.re, fmaf(T11, dD0.im, T12 * dz0.re));
-                    const float dyi = fmaf(T10, dS0.im, fmaf(-T11, dD0.re, T12 * dz0.im));
-                    cfloat decho = {dxr - dyi, dxi + dyr};
-                    if constexpr (FIELD == TAN_T2) decho = decho + echo * kTE;
-                    if constexpr (FIELD == TAN_B0) decho = decho + times_i(echo) * kTE;
-                    if constexpr (FIELD == TAN_B1) {
-                        const float sxr = fmaf(S00, S0.re, fmaf(S01, D0.im, S02 * z0.re));
-                        const float sxi = fmaf(S00, S0.im, fmaf(-S01, D0.re, S02 * z0.im));
-                        const float syr = fmaf(S10, S0.re, fmaf(S11, D0.im, S12 * z0.re));
-                        const float syi = fmaf(S10, S0.im, fmaf(-S11, D0.re, S12 * z0.im));
-                        decho = decho + cfloat {sxr - syi, sxi + syr};
+                    const Rows T = decay_rows(d_TE, 0.0f, R00, R01, R02, R10, R11, R12, R20, R21, R22);
+                    const cfloat S0 = P[0] + Q[0], D0 = P[0] - Q[0];
+                    const cfloat echo = sample_rows(T, S0, D0, Z[0]);
+                    cfloat v[NP];
+                    if constexpr (TS::echo) v[0] = echo;
+#pragma unroll
+                    for (int k = 0; k < NT; k++) {
+                        cfloat dv = sample_rows(T, dP[k][0] + dQ[k][0], dP[k][0] - dQ[k][0], dZ[k][0]);
+                        if (TS::t2 && k == TS::it2) dv = dv + echo * k2TE;
+                        if (TS::b0 && k == TS::ib0) dv = dv + times_i(echo) * k0TE;
+                        if (TS::b1 && k == TS::ib1) dv = dv + sample_rows(Sb, S0, D0, Z[0]);
+                        v[OFS + k] = dv;
                     }
                     for (int u = 0; u < a.usf; u++) {
-                        cfloat* out = dechos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
-                        *out = (slice == 0) ? decho : (*out + decho);
+                        const size_t at = size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
+#pragma unroll
+                        for (int k = 0; k < NP; k++) out.p[k][at] = (slice == 0) ? v[k] : (out.p[k][at] + v[k]);
                     }
                 }
 
-                cfloat Pn[SPL], Qn[SPL], dPn[SPL], dQn[SPL];
+                cfloat Pn[SPL], Qn[SPL], dPn[NT][SPL], dQn[NT][SPL];
 #pragma unroll
                 for (int i = 0; i < SPL; i++) {
-                    const cfloat S = P[i] + Q[i], D = P[i] - Q[i], z = Z[i];
-                    const float xr = fmaf(h00, S.re, fmaf(h01, D.im, M02 * z.re));
-                    const float xi = fmaf(h00, S.im, fmaf(-h01, D.re, M02 * z.im));
-                    const float yr = fmaf(h10, S.re, fmaf(h11, D.im, M12 * z.re));
-                    const float yi = fmaf(h10, S.im, fmaf(-h11, D.re, M12 * z.im));
-                    const float zr = fmaf(h20, S.re, fmaf(h21, D.im, M22 * z.re));
-                    const float zi = fmaf(h20, S.im, fmaf(-h21, D.re, M22 * z.im));
-                    Pn[i] = {xr - yi, xi + yr};
-                    Qn[i] = {xr + yi, xi - yr};
-
-                    const cfloat dS = dP[i] + dQ[i], dD = dP[i] - dQ[i], dz = dZ[i];
-                    float txr = fmaf(h00, dS.re, fmaf(h01, dD.im, M02 * dz.re));
-                    float txi = fmaf(h00, dS.im, fmaf(-h01, dD.re, M02 * dz.im));
-                    float tyr = fmaf(h10, dS.re, fmaf(h11, dD.im, M12 * dz.re));
-                    float tyi = fmaf(h10, dS.im, fmaf(-h11, dD.re, M12 * dz.im));
-                    float tzr = fmaf(h20, dS.re, fmaf(h21, dD.im, M22 * dz.re));
-                    float tzi = fmaf(h20, dS.im, fmaf(-h21, dD.re, M22 * dz.im));
-                    if constexpr (FIELD == TAN_B1) {
-                        txr += fmaf(N00, S.re, fmaf(N01, D.im, N02 * z.re));
-                        txi += fmaf(N00, S.im, fmaf(-N01, D.re, N02 * z.im));
-                        tyr += fmaf(N10, S.re, fmaf(N11, D.im, N12 * z.re));
-                        tyi += fmaf(N10, S.im, fmaf(-N11, D.re, N12 * z.im));
-                        tzr += fmaf(N20, S.re, fmaf(N21, D.im, N22 * z.re));
-                        tzi += fmaf(N20, S.im, fmaf(-N21, D.re, N22 * z.im));
+                    const cfloat S = P[i] + Q[i], D = P[i] - Q[i];
+                    cfloat Zn;
+                    apply_rows(M, S, D, Z[i], Pn[i], Qn[i], Zn);
+#pragma unroll
+                    for (int k = 0; k < NT; k++) {
+                        cfloat tp, tq, tz;
+                        apply_rows(M, dP[k][i] + dQ[k][i], dP[k][i] - dQ[k][i], dZ[k][i], tp, tq, tz);
+                        if (TS::b1 && k == TS::ib1) {
+                            cfloat np, nq, nz;
+                            apply_rows(N, S, D, Z[i], np, nq, nz);
+                            tp = tp + np;
+                            tq = tq + nq;
+                            tz = tz + nz;
+                        }
+                        if (TS::t1 && k == TS::it1) tz = tz + Zn * lTR;
+                        if (TS::t2 && k == TS::it2) {
+                            tp = tp + Pn[i] * k2TR;
+                            tq = tq + Qn[i] * k2TR;
+                        }
+                        if (TS::b0 && k == TS::ib0) {
+                            tp = tp + times_i(Pn[i]) * k0TR;
+                            tq = tq - times_i(Qn[i]) * k0TR;
+                        }
+                        dPn[k][i] = tp;
+                        dQn[k][i] = tq;
+                        dZ[k][i] = tz;
                     }
-                    if constexpr (FIELD == TAN_T1) {
-                        tzr = fmaf(lTR, zr, tzr);
-                        tzi = fmaf(lTR, zi, tzi);
-                    }
-                    dPn[i] = {txr - tyi, txi + tyr};
-                    dQn[i] = {txr + tyi, txi - tyr};
-                    if constexpr (FIELD == TAN_T2) {
-                        dPn[i] = dPn[i] + Pn[i] * kTR;
-                        dQn[i] = dQn[i] + Qn[i] * kTR;
-                    }
-                    if constexpr (FIELD == TAN_B0) {
-                        dPn[i] = dPn[i] + times_i(Pn[i]) * kTR;
-                        dQn[i] = dQn[i] - times_i(Qn[i]) * kTR;
-                    }
-                    Z[i] = {zr, zi};
-                    dZ[i] = {tzr, tzi};
+                    Z[i] = Zn;
                 }
                 if (owns_zero) {
                     Z[0].re += G;
-                    if constexpr (FIELD == TAN_T1) dZ[0].re += dG;
+                    if constexpr (TS::t1) dZ[TS::it1][0].re += dG;
                 }
 
-                // dephasing, primal and tangent alike
-                cfloat p_in = {0, 0}, q_in = {0, 0}, dp_in = {0, 0}, dq_in = {0, 0};
-                if constexpr (LPV > 1) {
-                    p_in.re = __shfl_up(Pn[SPL - 1].re, 1, LPV);
-                    p_in.im = __shfl_up(Pn[SPL - 1].im, 1, LPV);
-                    q_in.re = __shfl_down(Qn[0].re, 1, LPV);
-                    q_in.im = __shfl_down(Qn[0].im, 1, LPV);
-                    dp_in.re = __shfl_up(dPn[SPL - 1].re, 1, LPV);
-                    dp_in.im = __shfl_up(dPn[SPL - 1].im, 1, LPV);
-                    dq_in.re = __shfl_down(dQn[0].re, 1, LPV);
-                    dq_in.im = __shfl_down(dQn[0].im, 1, LPV);
-                    if (sub == LPV - 1) q_in = dq_in = {0, 0};
-                }
+                // dephasing: F+ moves up one state, F- down one, primal and tangents alike
+                auto shift = [&](cfloat (&p)[SPL], cfloat (&q)[SPL], const cfloat (&pn)[SPL], const cfloat (&qn)[SPL]) {
+                    cfloat p_in = {0, 0}, q_in = {0, 0};
+                    if constexpr (LPV > 1) {
+                        p_in.re = __shfl_up(pn[SPL - 1].re, 1, LPV);
+                        p_in.im = __shfl_up(pn[SPL - 1].im, 1, LPV);
+                        q_in.re = __shfl_down(qn[0].re, 1, LPV);
+                        q_in.im = __shfl_down(qn[0].im, 1, LPV);
+                        if (sub == LPV - 1) q_in = {0, 0};
+                    }
 #pragma unroll
-                for (int i = SPL - 1; i >= 1; i--) {
-                    P[i] = Pn[i - 1];
-                    dP[i] = dPn[i - 1];
-                }
-                P[0] = p_in;
-                dP[0] = dp_in;
+                    for (int i = SPL - 1; i >= 1; i--) p[i] = pn[i - 1];
+                    p[0] = p_in;
 #pragma unroll
-                for (int i = 0; i < SPL - 1; i++) {
-                    Q[i] = Qn[i + 1];
-                    dQ[i] = dQn[i + 1];
-                }
-                Q[SPL - 1] = q_in;
-                dQ[SPL - 1] = dq_in;
+                    for (int i = 0; i < SPL - 1; i++) q[i] = qn[i + 1];
+                    q[SPL - 1] = q_in;
 #pragma unroll
-                for (int i = 0; i < SPL; i++) {
-                    if (k0 + i >= a.N - 1) Q[i] = dQ[i] = {0, 0};
-                }
-                if (owns_zero) {
-                    P[0] = conj(Q[0]);
-                    dP[0] = conj(dQ[0]);
-                }
+                    for (int i = 0; i < SPL; i++)
+                        if (k0 + i >= a.N - 1) q[i] = {0, 0};
+                    if (owns_zero) p[0] = conj(q[0]);
+                };
+                shift(P, Q, Pn, Qn);
+#pragma unroll
+                for (int k = 0; k < NT; k++) shift(dP[k], dQ[k], dPn[k], dQn[k]);
             }
 
             if (a.wait_spoiling) {
 #pragma unroll
-                for (int i = 0; i < SPL; i++) P[i] = Q[i] = dP[i] = dQ[i] = {0, 0};
+                for (int i = 0; i < SPL; i++) {
+                    P[i] = Q[i] = {0, 0};
+#pragma unroll
+                    for (int k = 0; k < NT; k++) dP[k][i] = dQ[k][i] = {0, 0};
+                }
             }
 #pragma unroll
             for (int i = 0; i < SPL; i++) {
-                cfloat tP = dev::cmul(dP[i], d_W), tQ = dev::cmul(dQ[i], conj(d_W)), tZ = dZ[i] * E1_W;
                 const cfloat pW = dev::cmul(P[i], d_W), qW = dev::cmul(Q[i], conj(d_W));
-                if constexpr (FIELD == TAN_T2) {
-                    tP = tP + pW * kTW;
-                    tQ = tQ + qW * kTW;
+#pragma unroll
+                for (int k = 0; k < NT; k++) {
+                    cfloat tP = dev::cmul(dP[k][i], d_W), tQ = dev::cmul(dQ[k][i], conj(d_W));
+                    cfloat tZ = dZ[k][i] * E1_W;
+                    if (TS::t2 && k == TS::it2) {
+                        tP = tP + pW * k2TW;
+                        tQ = tQ + qW * k2TW;
+                    }
+                    if (TS::b0 && k == TS::ib0) {
+                        tP = tP + times_i(pW) * k0TW;
+                        tQ = tQ - times_i(qW) * k0TW;
+                    }
+                    if (TS::t1 && k == TS::it1) tZ = tZ + Z[i] * dE1_W;
+                    dP[k][i] = tP;
+                    dQ[k][i] = tQ;
+                    dZ[k][i] = tZ;
                 }
-                if constexpr (FIELD == TAN_B0) {
-                    tP = tP + times_i(pW) * kTW;
-                    tQ = tQ - times_i(qW) * kTW;
-                }
-                if constexpr (FIELD == TAN_T1) tZ = tZ + Z[i] * dE1_W;
                 P[i] = pW;
                 Q[i] = qW;
                 Z[i] = Z[i] * E1_W;
-                dP[i] = tP;
-                dQ[i] = tQ;
-                dZ[i] = tZ;
             }
             if (owns_zero) {
                 Z[0].re += 1.0f - E1_W;
-                if constexpr (FIELD == TAN_T1) dZ[0].re -= dE1_W;
+                if constexpr (TS::t1) dZ[TS::it1][0].re -= dE1_W;
             }
         }
     }
@@ -303,32 +316,89 @@ __global__ void __launch_bounds__(64) fisp_epg_tangent(FispArgs a, cfloat* __res
 
 namespace {
 
-using TangentLauncher = void (*)(kernels::FispArgs, cfloat*, unsigned, hipStream_t);
+using kernels::TM_B0;
+using kernels::TM_B1;
+using kernels::TM_ECHO;
+using kernels::TM_T1;
+using kernels::TM_T2;
 
-template<int SPL, int LPV, int FIELD>
-void launch_tangent(kernels::FispArgs a, cfloat* d, unsigned blocks, hipStream_t s) {
-    kernels::fisp_epg_tangent<SPL, LPV, FIELD><<<blocks, 64, 0, s>>>(a, d);
+using TangentLauncher = void (*)(kernels::FispArgs, kernels::FispPlanes, unsigned, hipStream_t);
+
+template<int SPL, int LPV, int MASK>
+void launch_tangents(kernels::FispArgs a, kernels::FispPlanes o, unsigned blocks, hipStream_t s) {
+    kernels::fisp_epg_tangents<SPL, LPV, MASK><<<blocks, 64, 0, s>>>(a, o);
 }
 
-// Four states per lane keep primal + tangent state (48 floats) and the
-// next-state temporaries under 128 VGPRs (four waves per SIMD); eight states
-// per lane need 236+ and leave one.
-template<int FIELD>
-TangentLauncher tangent_launcher(int lpv) {
+// States per lane: four keep primal + two tangents (36 floats of state per
+// lane) and the next-state temporaries within 96-134 VGPRs; with four
+// tangents four states take 217 VGPRs (two waves per SIMD) and two states 136
+// (three waves, twice the lanes per voxel and the per-TR matrices computed
+// by twice the lanes).  COMPAS_FISP_TANGENT_SPL=2|4 picks for four tangents.
+template<int MASK, int SPL>
+TangentLauncher launcher_spl(int lpv) {
     switch (lpv) {
-        case 1: return &launch_tangent<4, 1, FIELD>;
-        case 2: return &launch_tangent<4, 2, FIELD>;
-        case 4: return &launch_tangent<4, 4, FIELD>;
-        case 8: return &launch_tangent<4, 8, FIELD>;
-        case 16: return &launch_tangent<4, 16, FIELD>;
-        default: return &launch_tangent<4, 32, FIELD>;
+        case 1: return &launch_tangents<SPL, 1, MASK>;
+        case 2: return &launch_tangents<SPL, 2, MASK>;
+        case 4: return &launch_tangents<SPL, 4, MASK>;
+        case 8: return &launch_tangents<SPL, 8, MASK>;
+        case 16: return &launch_tangents<SPL, 16, MASK>;
+        case 32: return &launch_tangents<SPL, 32, MASK>;
+        default: return &launch_tangents<SPL, 64, MASK>;
     }
+}
+
+int wide_states_per_lane() {
+    static const int spl = [] {
+        const char* e = std::getenv("COMPAS_FISP_TANGENT_SPL");
+        return (e && std::atoi(e) == 2) ? 2 : 4;
+    }();
+    return spl;
+}
+
+template<int MASK>
+TangentLauncher launcher_for(int lpv, int* spl) {
+    if constexpr (kernels::TangentSet<MASK>::nt <= 2) {
+        *spl = 4;
+        return launcher_spl<MASK, 4>(lpv);
+    } else {
+        *spl = wide_states_per_lane();
+        return *spl == 2 ? launcher_spl<MASK, 2>(lpv) : launcher_spl<MASK, 4>(lpv);
+    }
+}
+
+// field sets with a kernel of their own (as the pSSFP passes)
+bool launcher_for_mask(int mask, int lpv, TangentLauncher* fn, int* spl) {
+    switch (mask) {
+#define COMPAS_FISP_MASK(M)                   \
+    case M: *fn = launcher_for<M>(lpv, spl); return true;
+        COMPAS_FISP_MASK(TM_T1)
+        COMPAS_FISP_MASK(TM_T2)
+        COMPAS_FISP_MASK(TM_B1)
+        COMPAS_FISP_MASK(TM_B0)
+        COMPAS_FISP_MASK(TM_T1 | TM_T2)
+        COMPAS_FISP_MASK(TM_T1 | TM_T2 | TM_ECHO)
+        COMPAS_FISP_MASK(TM_T1 | TM_T2 | TM_B1 | TM_B0)
+        COMPAS_FISP_MASK(TM_T1 | TM_T2 | TM_B1 | TM_B0 | TM_ECHO)
+#undef COMPAS_FISP_MASK
+        default: return false;
+    }
+}
+
+// the smallest power of two of lanes per voxel that leaves <= spl states per lane
+int lanes_per_voxel(int max_state, int spl) {
+    int lpv = 1;
+    while (lpv < 64 && (max_state + lpv - 1) / lpv > spl) lpv *= 2;
+    return lpv;
 }
 
 }  // namespace
 
-Array<cfloat, 2> simulate_magnetization_tangent(
-    const CompasContext& ctx, int field, TissueParameters parameters, FISPSequence seq) {
+std::vector<Array<cfloat, 2>> simulate_magnetization_tangents(
+    const CompasContext& ctx,
+    const std::vector<int>& fields,
+    TissueParameters parameters,
+    FISPSequence seq,
+    Array<cfloat, 2>* echos) {
     const int nTR = int(seq.RF_train.size(0));
     const int nv = parameters.nvoxels;
     COMPAS_CHECK(seq.max_state >= 1);
@@ -336,24 +406,44 @@ Array<cfloat, 2> simulate_magnetization_tangent(
     COMPAS_CHECK(seq.sliceprofiles.size(1) == nTR);
     COMPAS_CHECK(seq.undersampling_factor >= 1);
     COMPAS_CHECK(seq.repetitions >= 1);
-    if (field != TissueParameterField::T1 && field != TissueParameterField::T2 &&
-        field != TissueParameterField::B1 && field != TissueParameterField::B0)
-        COMPAS_ERROR("tangent derivatives are available for T1, T2, B1 and B0");
+    int mask = 0;
+    std::vector<int> bits;
+    for (int f : fields) {
+        int bit = 0;
+        switch (f) {
+            case TissueParameterField::T1: bit = TM_T1; break;
+            case TissueParameterField::T2: bit = TM_T2; break;
+            case TissueParameterField::B1: bit = TM_B1; break;
+            case TissueParameterField::B0: bit = TM_B0; break;
+            default: COMPAS_ERROR("tangent derivatives are available for T1, T2, B1 and B0");
+        }
+        if (mask & bit) COMPAS_ERROR("a field is listed twice");
+        mask |= bit;
+        bits.push_back(bit);
+    }
     // (with no voxels every field reads as absent: the empty result is returned below)
-    if (nv > 0 && ((field == TissueParameterField::B1 && !parameters.has_b1) ||
-                   (field == TissueParameterField::B0 && !parameters.has_b0)))
+    if (nv > 0 && (((mask & TM_B1) && !parameters.has_b1) || ((mask & TM_B0) && !parameters.has_b0)))
         COMPAS_ERROR("cannot differentiate with respect to an absent parameter");
     const int nslices = int(seq.sliceprofiles.size(0));
     const int nr = nTR * seq.undersampling_factor;
     ctx.activate();
-    auto d = ctx.empty<cfloat, 2>({nr, nv});
-    if (nv == 0 || nr == 0) return d;
-    if (nslices == 0) {
-        ctx.fill_zero(d.data(), d.nbytes());
+    std::vector<Array<cfloat, 2>> d;
+    for (size_t i = 0; i < fields.size(); i++) d.push_back(ctx.empty<cfloat, 2>({nr, nv}));
+    if (echos) *echos = ctx.empty<cfloat, 2>({nr, nv});
+    if (nv == 0 || nr == 0 || fields.empty()) {
+        if (echos && nv > 0 && nr > 0) *echos = simulate_magnetization(ctx, parameters, seq);
         return d;
     }
-    auto table = fisp_rf_table(ctx, seq);
+    if (nslices == 0) {
+        for (auto& x : d) ctx.fill_zero(x.data(), x.nbytes());
+        if (echos) ctx.fill_zero(echos->data(), echos->nbytes());
+        return d;
+    }
+    cfloat* by_bit[32] = {};
+    for (size_t i = 0; i < bits.size(); i++) by_bit[bits[i]] = d[i].data();
+    if (echos) by_bit[TM_ECHO] = echos->data();
 
+    auto table = fisp_rf_table(ctx, seq);
     kernels::FispArgs a;
     a.tissue = dev::tissue_view(parameters);
     a.table = table.data();
@@ -370,39 +460,47 @@ Array<cfloat, 2> simulate_magnetization_tangent(
     a.wait_spoiling = seq.wait_spoiling;
     a.echos = nullptr;
 
-    // lanes per voxel: the smallest power of two that leaves <= 4 states per lane
-    int lpv = 1;
-    while (lpv < 32 && (seq.max_state + lpv - 1) / lpv > 4) lpv *= 2;
-    TangentLauncher fn;
-    switch (field) {
-        case TissueParameterField::T1: fn = tangent_launcher<kernels::TAN_T1>(lpv); break;
-        case TissueParameterField::T2: fn = tangent_launcher<kernels::TAN_T2>(lpv); break;
-        case TissueParameterField::B1: fn = tangent_launcher<kernels::TAN_B1>(lpv); break;
-        default: fn = tangent_launcher<kernels::TAN_B0>(lpv); break;
+    // passes with a kernel of their own: all four, or T1 + T2, then single fields
+    const int fields_mask = mask;
+    std::vector<int> passes;
+    const int with_echo = echos ? TM_ECHO : 0;
+    if (fields_mask == (TM_T1 | TM_T2 | TM_B1 | TM_B0)) {
+        passes.push_back(fields_mask | with_echo);
+    } else {
+        int rest = fields_mask;
+        if ((rest & (TM_T1 | TM_T2)) == (TM_T1 | TM_T2)) {
+            passes.push_back(TM_T1 | TM_T2 | with_echo);
+            rest &= ~(TM_T1 | TM_T2);
+        } else if (with_echo) {
+            passes.push_back(TM_ECHO);  // the plain simulation
+        }
+        for (int bit : {TM_T1, TM_T2, TM_B1, TM_B0})
+            if (rest & bit) passes.push_back(bit);
     }
-    KernelScope ks(ctx, "fisp_epg_tangent");
-    fn(a, d.data(), dev::div_ceil(int64_t(nv) * lpv, 64), ctx.stream());
-    check_launch("fisp_epg_tangent");
+    for (int pass : passes) {
+        if (pass == TM_ECHO) {
+            *echos = simulate_magnetization(ctx, parameters, seq);
+            continue;
+        }
+        kernels::FispPlanes o {};
+        int np = 0;
+        for (int bit : {TM_ECHO, TM_T1, TM_T2, TM_B1, TM_B0})
+            if (pass & bit) o.p[np++] = by_bit[bit];
+        TangentLauncher fn = nullptr;
+        int spl = 4;
+        launcher_for_mask(pass, 1, &fn, &spl);
+        const int lpv = lanes_per_voxel(seq.max_state, spl);
+        launcher_for_mask(pass, lpv, &fn, &spl);
+        KernelScope ks(ctx, "fisp_epg_tangents");
+        fn(a, o, unsigned(dev::div_ceil(int64_t(nv) * lpv, 64)), ctx.stream());
+        check_launch("fisp_epg_tangents");
+    }
     return d;
 }
 
-}  // namespace compas
-
-namespace compas {
-
-std::vector<Array<cfloat, 2>> simulate_magnetization_tangents(
-    const CompasContext& ctx,
-    const std::vector<int>& fields,
-    TissueParameters parameters,
-    FISPSequence seq,
-    Array<cfloat, 2>* echos) {
-    for (size_t i = 0; i < fields.size(); i++)
-        for (size_t j = 0; j < i; j++)
-            if (fields[i] == fields[j]) COMPAS_ERROR("a field is listed twice");
-    std::vector<Array<cfloat, 2>> d;
-    for (int f : fields) d.push_back(simulate_magnetization_tangent(ctx, f, parameters, seq));
-    if (echos) *echos = simulate_magnetization(ctx, parameters, seq);
-    return d;
+Array<cfloat, 2> simulate_magnetization_tangent(
+    const CompasContext& ctx, int field, TissueParameters parameters, FISPSequence seq) {
+    return simulate_magnetization_tangents(ctx, std::vector<int> {field}, parameters, seq, nullptr)[0];
 }
 
 }  // namespace compas

@@ -33,24 +33,11 @@
 
 #include "compas/device/common.h"
 #include "compas/device/pssfp_common.h"
+#include "compas/device/tangent_set.h"
 #include "compas/internal.h"
 
 namespace compas {
 namespace kernels {
-
-// Planes one pass produces: a bit per field, plus the echoes themselves.
-enum : int { TM_T1 = 1, TM_T2 = 2, TM_B1 = 4, TM_B0 = 8, TM_ECHO = 16 };
-
-template<int MASK>
-struct TangentSet {
-    static constexpr bool t1 = (MASK & TM_T1) != 0, t2 = (MASK & TM_T2) != 0;
-    static constexpr bool b1 = (MASK & TM_B1) != 0, b0 = (MASK & TM_B0) != 0;
-    static constexpr bool echo = (MASK & TM_ECHO) != 0;
-    static constexpr int nt = int(t1) + int(t2) + int(b1) + int(b0);  // tangents carried
-    static constexpr int planes = nt + int(echo);  // echoes (when asked) first, then T1, T2, B1, B0
-    static constexpr int it1 = 0, it2 = int(t1), ib1 = int(t1) + int(t2), ib0 = int(t1) + int(t2) + int(b1);
-    static constexpr bool axis = b1 || b0;  // tangents that move the rotation axis
-};
 
 // Output planes of one launch: group blockIdx.y == 0 writes first[k]; group
 // j > 0 writes rest + ((j - 1) * planes + k) * nTR * nvoxels.

@@ -75,7 +75,10 @@ def test_gauss_newton_spiral_with_b1():
 
     start = estimate_of(t, T1=(t.T1 * 0.85).astype(np.float32), T2=(t.T2 * 1.15).astype(np.float32),
                         B1=(t.B1 + 0.05).astype(np.float32), rho=(t.rho * 1.1).astype(np.complex64))
-    res = R.gauss_newton(y, start, seq, traj, coils, iterations=10, cg_iterations=200, cg_tol=1e-7, fit_b1=True,
+    # with B1 free the normal equations are ill-conditioned and CG stops at its
+    # cap, so the objective falls linearly (about 6x per iteration at the end):
+    # 12 iterations bring every voxel's T1, T2 and B1 within the bounds below
+    res = R.gauss_newton(y, start, seq, traj, coils, iterations=12, cg_iterations=200, cg_tol=1e-7, fit_b1=True,
                          method="direct", signal_method="direct")
     obj = res.objective
     print("objective", obj, "cg", res.cg_iterations, "steps", res.steps, flush=True)
