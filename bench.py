@@ -589,7 +589,17 @@ class ServeCatalog:
                 self.trajs[key] = ca.CartesianTrajectory.from_trajectory(t)
         return self.trajs[key]
 
-    def request(self, spec):
+    def host_sequence(self, kind, nTR):
+        return W.make_pssfp(nTR, 70.0) if kind == "pssfp" else W.make_fisp(nTR, 70.0, 25)
+
+    def host_trajectory(self, spiral, nTR, n):
+        if spiral:
+            return W.make_spiral(nTR, SERVE_NS, self.fov, n, 5e-6)
+        t = W.make_cartesian(n, nTR // n + 1, SERVE_NS, self.fov, 5e-6)
+        return W.CartesianTraj(nTR, SERVE_NS, t.delta_t, t.k_start[:nTR], t.delta_k)
+
+    def host_request(self, spec):
+        """Host data of a request: tissue slice, sequence, trajectory, coils, y."""
         nv, nc = spec["nv"], spec["nc"]
         total = self.master.nvoxels
         off = int(spec["offset"] * (total - nv))
@@ -598,21 +608,149 @@ class ServeCatalog:
         rng = W.SplitMix64(0xC0A50005 + spec["id"] + 0x9E3779B9)
         coils = W.make_coils(n, self.fov, nc)[:, :nv]
         phase = np.exp(1j * rng.uniform(nc) * 2 * np.pi)[:, None]
-        r = {"spec": spec, "params": ca.TissueParameters.from_tissue(t),
-             "seq": self.sequence(spec["kind"], spec["nTR"]),
-             "traj": self.trajectory(spec["spiral"], spec["nTR"], n),
-             "coils": ca.CompasArray.from_numpy((coils * phase).astype(np.complex64))}
+        y = None
         if spec["kind"] == "jhy":
             k = nc * spec["nTR"] * SERVE_NS
             y = ((rng.normal(k) + 1j * rng.normal(k)) / np.sqrt(2)).reshape(nc, spec["nTR"], SERVE_NS)
-            r["y"] = ca.CompasArray.from_numpy(y.astype(np.complex64))
+            y = y.astype(np.complex64)
+        return t, n, (coils * phase).astype(np.complex64), y
+
+    def request(self, spec):
+        t, n, coils, y = self.host_request(spec)
+        r = {"spec": spec, "params": ca.TissueParameters.from_tissue(t),
+             "seq": self.sequence(spec["kind"], spec["nTR"]),
+             "traj": self.trajectory(spec["spiral"], spec["nTR"], n),
+             "coils": ca.CompasArray.from_numpy(coils)}
+        if y is not None:
+            r["y"] = ca.CompasArray.from_numpy(y)
         return r
+
+    def frame(self, spec):
+        """The request as a wire frame (compas_amd.wire)."""
+        from compas_amd import wire
+
+        t, n, coils, y = self.host_request(spec)
+        seq = self.host_sequence(spec["kind"], spec["nTR"])
+        traj = self.host_trajectory(spec["spiral"], spec["nTR"], n)
+        if y is not None:
+            return wire.encode_request(wire.KIND_JACOBIAN_HERMITIAN, t, seq, traj, coils, y, jacobian_method="gemm_fast")
+        return wire.encode_request(wire.KIND_SIMULATE_SIGNAL, t, seq, traj, coils, signal_method="matmul_fast")
 
 
 def serve_submit(server, r):
     if r["spec"]["kind"] == "jhy":
         return server.submit_jacobian_hermitian(r["params"], r["seq"], r["traj"], r["coils"], r["y"], "gemm_fast")
     return server.submit_simulate_signal(r["params"], r["seq"], r["traj"], r["coils"], "matmul_fast")
+
+
+def serve_wire_bench(args):
+    """The serving loop through the wire format: every request is a frame
+    (pre-encoded on the client), sent over the TCP front end on --connections
+    connections in arrival order (round robin) at Poisson arrivals of --load x
+    the capacity measured the same way; latency = scheduled arrival -> result
+    frame received by the client (so it includes parsing, uploads, result
+    read-back, encoding and the socket).  A sample of results is checked
+    bitwise against the same requests submitted as library objects."""
+    import threading
+
+    from compas_amd import wire
+
+    ctx = ca.init_context(0)
+    n_req = args.requests
+    specs = [serve_request_spec(i) for i in range(n_req)]
+    cat = ServeCatalog()
+    t0 = time.perf_counter()
+    frames = [cat.frame(sp) for sp in specs]
+    t_encode = time.perf_counter() - t0
+    frame_bytes = sum(len(f) for f in frames)
+    budget = int(args.budget_gib * (1 << 30))
+    server = ca.Server(slots=args.slots, memory_budget=budget)
+    fe = wire.TcpFrontEnd(server)
+    nconn = max(1, args.connections)
+
+    def run(ids, arrivals):
+        """Send frames ids[j] at arrivals[j] (seconds from now, None = at once);
+        returns {id: (t_send_scheduled, t_received, result)} on the client clock."""
+        clients = [wire.TcpClient(fe.address) for _ in range(nconn)]
+        per_conn = [[j for j in range(len(ids)) if j % nconn == k] for k in range(nconn)]
+        received = [[] for _ in range(nconn)]
+
+        def reader(k):
+            for _ in per_conn[k]:
+                r = clients[k].recv()
+                received[k].append((time.perf_counter(), r))
+
+        readers = [threading.Thread(target=reader, args=(k,)) for k in range(nconn)]
+        for th in readers:
+            th.start()
+        base = time.perf_counter() + 0.05
+        sched = {}
+        for j, i in enumerate(ids):
+            when = base + (arrivals[j] if arrivals is not None else 0.0)
+            delay = when - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            sched[i] = when if arrivals is not None else time.perf_counter()
+            clients[j % nconn].send(frames[i])
+        for th in readers:
+            th.join()
+        for c in clients:
+            c.close()
+        got = {}
+        for k in range(nconn):  # results come back per connection in sending order
+            for j, (t, r) in zip(per_conn[k], received[k]):
+                got[ids[j]] = (sched[ids[j]], t, r)
+        return got
+
+    # capacity: a closed-loop burst of a sample, through the same path
+    cal_ids = np.random.default_rng(6).choice(n_req, size=min(args.calibrate, n_req), replace=False).tolist()
+    cal = run(cal_ids, None)
+    cal_span = max(t for _, t, _ in cal.values()) - min(s for s, _, _ in cal.values())
+    capacity = len(cal_ids) / cal_span
+    lam = args.load * capacity
+    arrivals = np.cumsum(np.random.default_rng(7).exponential(1.0 / lam, size=n_req))
+    got = run(list(range(n_req)), arrivals)
+    fe.close()
+    errors = [f"request {i}: {r['error']}" for i, (_, _, r) in got.items() if not r["ok"]]
+    lat = np.array([t - s for s, t, _ in got.values()])
+    span = max(t for _, t, _ in got.values()) - min(s for s, _, _ in got.values())
+
+    # a sample submitted as library objects must give the same bits
+    check_ids = sorted(np.random.default_rng(5).choice(n_req, size=min(args.check, n_req), replace=False).tolist())
+    worst = 0.0
+    for i in check_ids:
+        r = cat.request(specs[i])
+        t = serve_submit(server, r)
+        server.wait(t)
+        want = server.result(t).collect()
+        server.release(t)
+        res = got[i][2]
+        key = "jhy" if specs[i]["kind"] == "jhy" else "signal"
+        worst = max(worst, 0.0 if np.array_equal(res[key], want) else rel_l2(res[key], want))
+    server.close()
+    vox_tr = sum(sp["nv"] * sp["nTR"] for sp in specs)
+    counts = {k: sum(1 for sp in specs if sp["kind"] == k) for k in ("fisp", "pssfp", "jhy")}
+    print(json.dumps({
+        "metric": "serving over the wire format: requests/s and p50/p99 latency (arrival -> result frame at the client)",
+        "value": len(got) / span,
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32 (signal and J^H.y contractions BF16 hi+lo on MFMA)",
+        "data": "synthetic (the seeded request mix of --serve, every request a self-contained frame)",
+        "config": {"workload": f"{n_req} requests as frames over TCP ({nconn} connections), Poisson arrivals at "
+                               f"{args.load:.0%} of measured capacity, {args.slots} slots", "mix": counts},
+        "voxel_tr_per_s": vox_tr / span,
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99)),
+                       "mean": 1e3 * float(lat.mean())},
+        "arrival_rate_per_s": lam,
+        "capacity_per_s": capacity,
+        "frame_bytes": {"total": frame_bytes, "mean": frame_bytes / n_req},
+        "encode_s": t_encode,
+        "wire_vs_objects_max_rel_l2": worst,
+        "wire_vs_objects_checked": len(check_ids),
+        "errors": errors[:5],
+    }))
 
 
 def serve_bench(args):
@@ -860,6 +998,8 @@ def main():
     ap.add_argument("--calibrate", type=int, default=60)
     ap.add_argument("--check", type=int, default=100)
     ap.add_argument("--budget-gib", type=float, default=96.0)
+    ap.add_argument("--wire", action="store_true", help="with --serve: requests as wire frames over TCP")
+    ap.add_argument("--connections", type=int, default=8, help="TCP connections of --serve --wire")
     ap.add_argument("--products", action="store_true",
                     help="time signal + J.v + J^H.y at the 4-coil Cartesian shapes (--method picks the precision)")
     ap.add_argument("--precision-study", action="store_true",
@@ -877,7 +1017,7 @@ def main():
         products_bench(args)
         return
     if args.serve:
-        serve_bench(args)
+        (serve_wire_bench if args.wire else serve_bench)(args)
         return
     if args.workload == 3:
         spiral_bench(args)

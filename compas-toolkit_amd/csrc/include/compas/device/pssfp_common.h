@@ -4,8 +4,8 @@
 // Semantics: reference src/operators/isochromat.cuh:9-128.  Rodrigues'
 // rotation is evaluated without sqrt/rcp/sin/cos for the small per-step
 // angles: with u = |a|^2, s1(u) = sin(t)/t and s2(u) = (1 - cos t)/t^2 are
-// Taylor polynomials in u (|t| <= 1/4: 6 terms, error < 1e-17),
-// cos t = 1 - u s2(u), and
+// Taylor polynomials in u (|t| <= 1/4: 6 terms, error < 1e-17; |t| <= 1/2:
+// 7 terms, error < 5e-17), cos t = 1 - u s2(u), and
 //   m' = cos t m + s1 (a x m) + s2 (a . m) a,
 // which equals the reference's normalised form.  Larger angles use FP64 sincos.
 
@@ -29,8 +29,11 @@ struct Vec3d {
     double x, y, z;
 };
 
-// Angles up to sqrt(SERIES_LIMIT) = 1/4 rad use the Taylor series.
+// Angles up to sqrt(SERIES_LIMIT) = 1/4 rad use the six-term Taylor series,
+// up to sqrt(SERIES_LIMIT_WIDE) = 1/2 rad the seven-term one (first omitted
+// terms 1e-17 and 5e-17 relative); larger angles FP64 sincos.
 constexpr double SERIES_LIMIT = 0.0625;
+constexpr double SERIES_LIMIT_WIDE = 0.25;
 
 // s1 = sin(t)/t and s2 = (1 - cos t)/t^2 of t = sqrt(u), u <= SERIES_LIMIT:
 // Taylor series in u, six terms each (first omitted terms below 1e-17),
@@ -50,10 +53,27 @@ __device__ inline void rotation_series(double u, double& s1, double& s2) {
     s2 = fma(u5, -1.0 / 479001600.0, s2);
 }
 
-// the same for any u: the series up to SERIES_LIMIT, FP64 sincos above
+// the same with one more term each, for u <= SERIES_LIMIT_WIDE
+__device__ inline void rotation_series_wide(double u, double& s1, double& s2) {
+    const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u3 * u3;
+    s1 = fma(u, -1.0 / 6.0, 1.0);
+    s1 = fma(u2, 1.0 / 120.0, s1);
+    s1 = fma(u3, -1.0 / 5040.0, s1);
+    s1 = fma(u4, 1.0 / 362880.0, s1);
+    s1 = fma(u5, -1.0 / 39916800.0, s1);
+    s1 = fma(u6, 1.0 / 6227020800.0, s1);
+    s2 = fma(u, -1.0 / 24.0, 0.5);
+    s2 = fma(u2, 1.0 / 720.0, s2);
+    s2 = fma(u3, -1.0 / 40320.0, s2);
+    s2 = fma(u4, 1.0 / 3628800.0, s2);
+    s2 = fma(u5, -1.0 / 479001600.0, s2);
+    s2 = fma(u6, 1.0 / 87178291200.0, s2);
+}
+
+// the same for any u: the series up to SERIES_LIMIT_WIDE, FP64 sincos above
 __device__ inline void rotation_any(double u, double& s1, double& s2) {
-    if (u <= SERIES_LIMIT) {
-        rotation_series(u, s1, s2);
+    if (u <= SERIES_LIMIT_WIDE) {
+        rotation_series_wide(u, s1, s2);
     } else {
         const double t = sqrt(u);
         double s, c;

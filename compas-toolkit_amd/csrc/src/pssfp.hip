@@ -105,19 +105,27 @@ __global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat*
                 const double ax = zre * B1, ay = -zim * B1;
                 const double axy2 = fma(ax, ax, ay * ay);
                 double u[ZPL];
-                bool large = false;
+                double umax = 0.0;
 #pragma unroll
                 for (int i = 0; i < ZPL; i++) {
                     u[i] = fma(az[i], az[i], axy2);
-                    large = large || (u[i] > SERIES_LIMIT);
+                    umax = fmax(umax, u[i]);
                 }
-                // wave-uniform choice: the common all-small case is straight-line
-                // code over the ZPL independent isochromats
-                if (__any(large)) {
+                // wave-uniform choice of the series: the common small-angle
+                // cases are straight-line code over the ZPL isochromats
+                if (!__any(umax > SERIES_LIMIT)) {
 #pragma unroll
                     for (int i = 0; i < ZPL; i++) {
                         double s1, s2;
-                        rotation_any(u[i], s1, s2);
+                        rotation_series(u[i], s1, s2);
+                        apply_rotation(m[i], ax, ay, az[i], u[i], s1, s2);
+                        relax3(m[i], E1_ex, E2_ex, R1_ex);
+                    }
+                } else if (!__any(umax > SERIES_LIMIT_WIDE)) {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        double s1, s2;
+                        rotation_series_wide(u[i], s1, s2);
                         apply_rotation(m[i], ax, ay, az[i], u[i], s1, s2);
                         relax3(m[i], E1_ex, E2_ex, R1_ex);
                     }
@@ -125,7 +133,7 @@ __global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat*
 #pragma unroll
                     for (int i = 0; i < ZPL; i++) {
                         double s1, s2;
-                        rotation_series(u[i], s1, s2);
+                        rotation_any(u[i], s1, s2);
                         apply_rotation(m[i], ax, ay, az[i], u[i], s1, s2);
                         relax3(m[i], E1_ex, E2_ex, R1_ex);
                     }

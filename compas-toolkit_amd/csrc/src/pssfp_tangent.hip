@@ -15,7 +15,7 @@
 //   inversion   m.z = cos(pi B1) (B1 only)
 // With u = |a|^2, Rot(a) m = c m + s1 (a x m) + s2 (a . m) a and
 // c = 1 - u s2, so the axis tangents need s1'(u) and s2'(u) besides s1, s2:
-// their Taylor series (|a| <= 1/4) or, above, the closed forms
+// their Taylor series (|a| <= 1/2) or, above, the closed forms
 // s1' = (c - s1) / (2u), s2' = (s1 - 2 s2) / (2u).
 // The result is the exact derivative of the discrete recursion (no step).
 //
@@ -62,10 +62,31 @@ __device__ inline void rotation_series_derivative(double u, double& s1p, double&
     s2p = fma(u5, 6.0 / 87178291200.0, s2p);
 }
 
+// the same with one more term each, for u <= SERIES_LIMIT_WIDE (first omitted
+// terms below 2e-18)
+__device__ inline void rotation_series_derivative_wide(double u, double& s1p, double& s2p) {
+    const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u3 * u3;
+    s1p = fma(u, 2.0 / 120.0, -1.0 / 6.0);
+    s1p = fma(u2, -3.0 / 5040.0, s1p);
+    s1p = fma(u3, 4.0 / 362880.0, s1p);
+    s1p = fma(u4, -5.0 / 39916800.0, s1p);
+    s1p = fma(u5, 6.0 / 6227020800.0, s1p);
+    s1p = fma(u6, -7.0 / 1307674368000.0, s1p);
+    s2p = fma(u, 2.0 / 720.0, -1.0 / 24.0);
+    s2p = fma(u2, -3.0 / 40320.0, s2p);
+    s2p = fma(u3, 4.0 / 3628800.0, s2p);
+    s2p = fma(u4, -5.0 / 479001600.0, s2p);
+    s2p = fma(u5, 6.0 / 87178291200.0, s2p);
+    s2p = fma(u6, -7.0 / 20922789888000.0, s2p);
+}
+
+enum { TIER_SERIES = 0, TIER_WIDE = 1, TIER_ANY = 2 };
+
 // Rot(a) m and the tangents dm[k]: Rot(a) dm[k], plus (dRot/da . da/dp) m
 // for B1 (da = (zre, -zim, 0)) and B0 (da = (0, 0, daz)).
-// ANY: u may exceed SERIES_LIMIT (closed forms there); else the series only.
-template<int MASK, bool ANY>
+// TIER: the six-term series (u <= SERIES_LIMIT), the seven-term one (u <=
+// SERIES_LIMIT_WIDE), or any u (the seven-term series, closed forms above).
+template<int MASK, int TIER>
 __device__ inline void rotate_all(
     Vec3d& m,
     Vec3d* dm,
@@ -78,9 +99,12 @@ __device__ inline void rotate_all(
     double daz) {
     using S = TangentSet<MASK>;
     double s1, s2, s1p = 0.0, s2p = 0.0;
-    if (!ANY || u <= SERIES_LIMIT) {
+    if (TIER == TIER_SERIES) {
         rotation_series(u, s1, s2);
         if constexpr (S::axis) rotation_series_derivative(u, s1p, s2p);
+    } else if (TIER == TIER_WIDE || u <= SERIES_LIMIT_WIDE) {
+        rotation_series_wide(u, s1, s2);
+        if constexpr (S::axis) rotation_series_derivative_wide(u, s1p, s2p);
     } else {
         rotation_any(u, s1, s2);
         if constexpr (S::axis) {
@@ -265,22 +289,29 @@ __global__ void __launch_bounds__(64) pssfp_iso_tangents(PssfpArgs a, int group0
                 const double ax = zre * B1, ay = -zim * B1;
                 const double axy2 = fma(ax, ax, ay * ay);
                 double u[ZPL];
-                bool large = false;
+                double umax = 0.0;
 #pragma unroll
                 for (int i = 0; i < ZPL; i++) {
                     u[i] = fma(az[i], az[i], axy2);
-                    large = large || (u[i] > SERIES_LIMIT);
+                    umax = fmax(umax, u[i]);
                 }
-                if (__any(large)) {  // wave-uniform: the all-small case is branch-free
+                // wave-uniform choice of the series, as in the primal kernel
+                if (!__any(umax > SERIES_LIMIT)) {
 #pragma unroll
                     for (int i = 0; i < ZPL; i++) {
-                        rotate_all<MASK, true>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
+                        rotate_all<MASK, TIER_SERIES>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
+                        relax_all<MASK>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE1_ex, dE2_ex);
+                    }
+                } else if (!__any(umax > SERIES_LIMIT_WIDE)) {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        rotate_all<MASK, TIER_WIDE>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
                         relax_all<MASK>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE1_ex, dE2_ex);
                     }
                 } else {
 #pragma unroll
                     for (int i = 0; i < ZPL; i++) {
-                        rotate_all<MASK, false>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
+                        rotate_all<MASK, TIER_ANY>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
                         relax_all<MASK>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE1_ex, dE2_ex);
                     }
                 }
