@@ -96,11 +96,13 @@ def parse_frame(frame: bytes, magic: bytes) -> Tuple[int, Dict[str, np.ndarray]]
 
 
 def encode_request(kind: int, tissue, sequence, trajectory, coils, residual=None, signal_method="matmul_fast",
-                   jacobian_method="gemm_fast", fields=("T1", "T2", "B1", "B0", "rho", "x", "y", "z")) -> bytes:
+                   jacobian_method="gemm_fast", fields=("T1", "T2", "B1", "B0", "rho", "x", "y", "z"),
+                   return_echos: bool = False) -> bytes:
     """Request frame from host data: tissue (workloads.Tissue), sequence
     (FispSeq or PssfpSeq), trajectory (CartesianTraj or SpiralTraj), coils
     [nc][nv] complex, residual [nc][nr][ns] (J^H.y).  `fields` lists the
-    tissue fields to send (B1, B0 and z may be left out)."""
+    tissue fields to send (B1, B0 and z may be left out); with return_echos
+    the result frame also carries the echoes [nr][nv]."""
     from . import JACOBIAN_METHODS, SIGNAL_METHODS
     from . import workloads as W
 
@@ -114,6 +116,7 @@ def encode_request(kind: int, tissue, sequence, trajectory, coils, residual=None
     jm = JACOBIAN_METHODS[jacobian_method] if isinstance(jacobian_method, str) else int(jacobian_method)
     e["signal_method"] = np.array(sm, i32)
     e["jacobian_method"] = np.array(jm, i32)
+    e["return_echos"] = np.array(int(bool(return_echos)), i32)
     for name in ("T1", "T2", "B1", "B0"):
         if name in fields:
             e[name] = np.asarray(getattr(tissue, name), f32)

@@ -42,6 +42,7 @@ struct ServeRequest {
     Array<cfloat, 3> residual;  // J^H.y input y [nc][nr][ns]
     SimulateSignalMethod signal_method = SimulateSignalMethod::MatmulFast;
     JacobianComputeMethod jacobian_method = JacobianComputeMethod::GemmFast;
+    bool return_echos = true;  // the result frame carries the echoes (wire submissions ask explicitly)
 };
 
 struct ServeResult {
@@ -50,6 +51,7 @@ struct ServeResult {
     Array<cfloat, 2> jhy;  // JacobianHermitian: [4 or 6][nv]
     double t_arrival = 0, t_start = 0, t_done = 0;  // seconds since the server started
     std::string error;
+    bool return_echos = true;  // copied from the request
 };
 
 class Server {

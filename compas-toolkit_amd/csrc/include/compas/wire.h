@@ -17,7 +17,8 @@
 // Request kinds: 0 simulate + signal, 1 J^H.y.  Entries:
 //   sequence_type i32 (0 FISP, 1 pSSFP), trajectory_type i32 (0 Cartesian,
 //   1 spiral), signal_method i32 and jacobian_method i32 (optional, the
-//   enumerators of SimulateSignalMethod / JacobianComputeMethod);
+//   enumerators of SimulateSignalMethod / JacobianComputeMethod),
+//   return_echos i32 (optional, 0: the result carries no echoes);
 //   tissue  T1 T2 rho_x rho_y x y f32[nv]; B1 B0 z f32[nv] optional (absent:
 //           the field is absent, as with a NULL pointer in the C-ABI);
 //   FISP    RF c64[nTR], sliceprofiles c64[nslices][nTR], TR TE TW TI f32,
@@ -29,8 +30,9 @@
 //           c64 (Cartesian) / c64[nr] (spiral);
 //   coils c64[nc][nv]; J^H.y: residual c64[nc][nr][ns].
 // Result kinds: 0 done, 1 failed.  Entries: ticket i64, t_arrival t_start
-// t_done f64 (seconds since the server started); done: echos c64[nr][nv] and
-// signal c64[nc][nr][ns] (simulate + signal) or jhy c64[rows][nv] (J^H.y);
+// t_done f64 (seconds since the server started); done: signal
+// c64[nc][nr][ns] (simulate + signal) or jhy c64[rows][nv] (J^H.y), and
+// echos c64[nr][nv] when the request set return_echos;
 // failed: error u8[len] (UTF-8).
 //
 // Parsing checks every size against the frame before reading it; a

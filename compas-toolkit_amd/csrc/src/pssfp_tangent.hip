@@ -82,6 +82,22 @@ __device__ inline void rotation_series_derivative_wide(double u, double& s1p, do
 
 enum { TIER_SERIES = 0, TIER_WIDE = 1, TIER_ANY = 2 };
 
+// Rot(a) as a matrix: c I + s1 [a]x + s2 a a^T
+struct RotationMatrix {
+    double r00, r01, r02, r10, r11, r12, r20, r21, r22;
+    __device__ Vec3d apply(const Vec3d& v) const {
+        return {fma(r00, v.x, fma(r01, v.y, r02 * v.z)), fma(r10, v.x, fma(r11, v.y, r12 * v.z)),
+                fma(r20, v.x, fma(r21, v.y, r22 * v.z))};
+    }
+};
+
+__device__ inline RotationMatrix rotation_matrix(double ax, double ay, double az, double c, double s1, double s2) {
+    const double bx = s2 * ax, by = s2 * ay, bz = s2 * az;
+    const double pxy = bx * ay, pxz = bx * az, pyz = by * az;
+    const double qx = s1 * ax, qy = s1 * ay, qz = s1 * az;
+    return {fma(bx, ax, c), pxy - qz, pxz + qy, pxy + qz, fma(by, ay, c), pyz - qx, pxz - qy, pyz + qx, fma(bz, az, c)};
+}
+
 // Rot(a) m and the tangents dm[k]: Rot(a) dm[k], plus (dRot/da . da/dp) m
 // for B1 (da = (zre, -zim, 0)) and B0 (da = (0, 0, daz)).
 // TIER: the six-term series (u <= SERIES_LIMIT), the seven-term one (u <=
@@ -114,6 +130,16 @@ __device__ inline void rotate_all(
         }
     }
     const double c = fma(-u, s2, 1.0);
+    if constexpr (S::nt >= 2 && !S::axis) {
+        // several vectors through one rotation: the matrix c I + s1 [a]x +
+        // s2 a a^T (18 operations) and 9 per vector, against about 19 per
+        // vector for the Rodrigues form
+        const RotationMatrix R = rotation_matrix(ax, ay, az, c, s1, s2);
+#pragma unroll
+        for (int k = 0; k < S::nt; k++) dm[k] = R.apply(dm[k]);
+        m = R.apply(m);
+        return;
+    }
     const double cx = ay * m.z - az * m.y;
     const double cy = az * m.x - ax * m.z;
     const double cz = ax * m.y - ay * m.x;
@@ -141,6 +167,26 @@ __device__ inline void rotate_all(
         kb0.z = dc * m.z + ds1 * cz + g * az + h * daz;
     }
 
+    if constexpr (S::nt >= 2) {  // with axis tangents: the matrix, plus their terms
+        const RotationMatrix R = rotation_matrix(ax, ay, az, c, s1, s2);
+#pragma unroll
+        for (int k = 0; k < S::nt; k++) {
+            Vec3d t = R.apply(dm[k]);
+            if (S::b1 && k == S::ib1) {
+                t.x += kb1.x;
+                t.y += kb1.y;
+                t.z += kb1.z;
+            }
+            if (S::b0 && k == S::ib0) {
+                t.x += kb0.x;
+                t.y += kb0.y;
+                t.z += kb0.z;
+            }
+            dm[k] = t;
+        }
+        m = R.apply(m);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < S::nt; k++) {
         const Vec3d d = dm[k];

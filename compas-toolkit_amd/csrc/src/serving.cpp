@@ -85,6 +85,7 @@ int64_t Server::submit(ServeRequest request) {
     COMPAS_CHECK(request.fisp || request.pssfp);
     auto e = std::make_unique<Entry>();
     e->bytes = estimate_bytes(request);
+    e->result.return_echos = request.return_echos;
     e->request = std::move(request);
     std::lock_guard<std::mutex> lock(mutex_);
     COMPAS_CHECK(!stop_);

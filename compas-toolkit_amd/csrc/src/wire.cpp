@@ -3,6 +3,7 @@
 #include "compas/wire.h"
 
 #include <climits>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <set>
@@ -171,7 +172,7 @@ const char* const TISSUE_FIELDS[] = {"T1", "T2", "B1", "B0", "rho_x", "rho_y", "
 
 // The request's entries, checked for presence, type and consistent sizes.
 struct HostRequest {
-    int kind = 0, sequence = 0, trajectory = 0, signal_method = 0, jacobian_method = 0;
+    int kind = 0, sequence = 0, trajectory = 0, signal_method = 0, jacobian_method = 0, return_echos = 0;
     int nv = 0;
     const Entry* tissue[9] = {};
     const Entry *RF = nullptr, *profiles = nullptr, *gamma_dt_RF = nullptr, *slice_z = nullptr;
@@ -199,6 +200,7 @@ HostRequest read_request(const Frame& f) {
     r.jacobian_method = f.i32("jacobian_method", int(JacobianComputeMethod::GemmFast));
     if (r.signal_method < 0 || r.signal_method > 5) throw CompasException("invalid signal_method");
     if (r.jacobian_method < 0 || r.jacobian_method > 4) throw CompasException("invalid jacobian_method");
+    r.return_echos = f.i32("return_echos", 0);
 
     const Entry& T1 = f.get("T1", F32, 1);
     if (T1.dims[0] > int64_t(INT32_MAX)) throw CompasException("too many voxels");
@@ -287,6 +289,7 @@ ServeRequest upload_request(const CompasContext& ctx, const Frame& f) {
     r.kind = h.kind == 0 ? RequestKind::SimulateSignal : RequestKind::JacobianHermitian;
     r.signal_method = SimulateSignalMethod(h.signal_method);
     r.jacobian_method = JacobianComputeMethod(h.jacobian_method);
+    r.return_echos = h.return_echos != 0;
 
     // tissue fields are copied out of the frame (alignment), then uploaded as one map
     std::vector<std::vector<float>> fields(9);
@@ -328,7 +331,8 @@ std::string describe_request(const Frame& f) {
        << (h.sequence == 0 ? "fisp" : "pssfp") << "\", \"trajectory\": \"" << (h.trajectory == 0 ? "cartesian" : "spiral")
        << "\", \"nvoxels\": " << h.nv << ", \"nTR\": " << h.RF->dims[0] << ", \"nreadouts\": " << h.nr
        << ", \"samples\": " << h.ns << ", \"ncoils\": " << h.coils->dims[0] << ", \"signal_method\": " << h.signal_method
-       << ", \"jacobian_method\": " << h.jacobian_method << ", \"fields\": [";
+       << ", \"jacobian_method\": " << h.jacobian_method << ", \"return_echos\": " << h.return_echos
+       << ", \"fields\": [";
     bool first = true;
     for (int i = 0; i < 9; i++)
         if (h.tissue[i]) {
@@ -348,9 +352,17 @@ std::string describe_request(const Frame& f) {
         for (size_t i = 0; i < n; i++) s += double(load<float>(e.data + 4 * i));
         return s;
     };
-    ss.precision(17);
-    ss << ", \"sum_T1\": " << sum(*h.tissue[0]) << ", \"sum_RF\": " << sum(*h.RF) << ", \"sum_coils\": " << sum(*h.coils);
-    if (h.residual) ss << ", \"sum_residual\": " << sum(*h.residual);
+    // (JSON has no NaN or infinity: non-finite sums print as null)
+    auto num = [](double v) {
+        if (!std::isfinite(v)) return std::string("null");
+        std::ostringstream o;
+        o.precision(17);
+        o << v;
+        return o.str();
+    };
+    ss << ", \"sum_T1\": " << num(sum(*h.tissue[0])) << ", \"sum_RF\": " << num(sum(*h.RF))
+       << ", \"sum_coils\": " << num(sum(*h.coils));
+    if (h.residual) ss << ", \"sum_residual\": " << num(sum(*h.residual));
     ss << "}";
     return ss.str();
 }
@@ -367,7 +379,7 @@ std::vector<uint8_t> encode_result(int64_t ticket, const ServeResult& result) {
         w.add("error", U8, 1, &n, result.error.data());
         return w.finish();
     }
-    put_array(w, "echos", result.echos);
+    if (result.return_echos) put_array(w, "echos", result.echos);
     put_array(w, "signal", result.signal);
     put_array(w, "jhy", result.jhy);
     return w.finish();

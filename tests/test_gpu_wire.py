@@ -16,11 +16,13 @@ from test_gpu_serving import device_objects, make_request, submit
 pytestmark = pytest.mark.gpu
 
 
-def frame_of(req):
+def frame_of(req, echos=False):
     kind, t, seq, traj, coils, y = req
     if kind == "jhy":
-        return wire.encode_request(wire.KIND_JACOBIAN_HERMITIAN, t, seq, traj, coils, y, jacobian_method="gemm_fast")
-    return wire.encode_request(wire.KIND_SIMULATE_SIGNAL, t, seq, traj, coils, signal_method="matmul_fast")
+        return wire.encode_request(wire.KIND_JACOBIAN_HERMITIAN, t, seq, traj, coils, y, jacobian_method="gemm_fast",
+                                   return_echos=echos)
+    return wire.encode_request(wire.KIND_SIMULATE_SIGNAL, t, seq, traj, coils, signal_method="matmul_fast",
+                               return_echos=echos)
 
 
 def object_results(server, reqs):
@@ -40,7 +42,7 @@ def test_wire_requests_equal_object_requests():
     reqs = [make_request(i) for i in range(12)]
     server = ca.Server(slots=4, memory_budget=8 << 30)
     want = object_results(server, reqs)
-    tickets = [server.submit_wire(frame_of(r)) for r in reqs]
+    tickets = [server.submit_wire(frame_of(r, echos=True)) for r in reqs]
     for i, t in enumerate(tickets):
         assert server.wait_done(t)
         res = wire.decode_result(server.result_wire(t))
@@ -102,9 +104,9 @@ def test_tcp_front_end_pipelined_clients():
     server.close()
     for i in range(12):
         r = results[i]
-        assert r["ok"]
+        assert r["ok"] and "echos" not in r  # not asked for
         key = "jhy" if reqs[i][0] == "jhy" else "signal"
-        assert np.array_equal(r["echos"], want[i][0]) and np.array_equal(r[key], want[i][1]), f"request {i}"
+        assert np.array_equal(r[key], want[i][1]), f"request {i}"
     for k in (0, 1):
         bad = results[("bad", k)]
         assert not bad["ok"] and "malformed frame" in bad["error"]

@@ -38,8 +38,10 @@ def small(kind=0, spiral=False, pssfp=False, nv=50, seed=4):
                                                (0, True, True)])
 def test_library_reads_python_frames(kind, spiral, pssfp):
     t, seq, traj, coils, res = small(kind, spiral, pssfp)
-    frame = wire.encode_request(kind, t, seq, traj, coils, res, signal_method="direct", jacobian_method="gemm")
+    frame = wire.encode_request(kind, t, seq, traj, coils, res, signal_method="direct", jacobian_method="gemm",
+                                return_echos=bool(kind))
     d = wire.describe(frame)
+    assert d["return_echos"] == kind
     assert d["kind"] == ("simulate_signal" if kind == 0 else "jacobian_hermitian")
     assert d["sequence"] == ("pssfp" if pssfp else "fisp")
     assert d["trajectory"] == ("spiral" if spiral else "cartesian")
