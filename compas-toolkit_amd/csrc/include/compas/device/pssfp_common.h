@@ -5,7 +5,7 @@
 // rotation is evaluated without sqrt/rcp/sin/cos for the small per-step
 // angles: with u = |a|^2, s1(u) = sin(t)/t and s2(u) = (1 - cos t)/t^2 are
 // Taylor polynomials in u (|t| <= 1/4: 6 terms, error < 1e-17; |t| <= 1/2:
-// 7 terms, error < 5e-17), cos t = 1 - u s2(u), and
+// 7 terms, error < 5e-17; Horner form), cos t = 1 - u s2(u), and
 //   m' = cos t m + s1 (a x m) + s2 (a . m) a,
 // which equals the reference's normalised form.  Larger angles use FP64 sincos.
 
@@ -36,10 +36,43 @@ constexpr double SERIES_LIMIT = 0.0625;
 constexpr double SERIES_LIMIT_WIDE = 0.25;
 
 // s1 = sin(t)/t and s2 = (1 - cos t)/t^2 of t = sqrt(u), u <= SERIES_LIMIT:
-// Taylor series in u, six terms each (first omitted terms below 1e-17),
-// summed over the powers of u with the running sum as the FMA addend, so
-// every step is one v_fmac and both series share the powers.
+// Taylor series in u, six terms each (first omitted terms below 1e-17), in
+// Horner form: five FMAs per series and no powers of u; the ZPL isochromats
+// of a lane give independent chains that hide the FMA latency.
 __device__ inline void rotation_series(double u, double& s1, double& s2) {
+    s1 = fma(u, -1.0 / 39916800.0, 1.0 / 362880.0);
+    s1 = fma(u, s1, -1.0 / 5040.0);
+    s1 = fma(u, s1, 1.0 / 120.0);
+    s1 = fma(u, s1, -1.0 / 6.0);
+    s1 = fma(u, s1, 1.0);
+    s2 = fma(u, -1.0 / 479001600.0, 1.0 / 3628800.0);
+    s2 = fma(u, s2, -1.0 / 40320.0);
+    s2 = fma(u, s2, 1.0 / 720.0);
+    s2 = fma(u, s2, -1.0 / 24.0);
+    s2 = fma(u, s2, 0.5);
+}
+
+// the same with one more term each, for u <= SERIES_LIMIT_WIDE
+__device__ inline void rotation_series_wide(double u, double& s1, double& s2) {
+    s1 = fma(u, 1.0 / 6227020800.0, -1.0 / 39916800.0);
+    s1 = fma(u, s1, 1.0 / 362880.0);
+    s1 = fma(u, s1, -1.0 / 5040.0);
+    s1 = fma(u, s1, 1.0 / 120.0);
+    s1 = fma(u, s1, -1.0 / 6.0);
+    s1 = fma(u, s1, 1.0);
+    s2 = fma(u, 1.0 / 87178291200.0, -1.0 / 479001600.0);
+    s2 = fma(u, s2, 1.0 / 3628800.0);
+    s2 = fma(u, s2, -1.0 / 40320.0);
+    s2 = fma(u, s2, 1.0 / 720.0);
+    s2 = fma(u, s2, -1.0 / 24.0);
+    s2 = fma(u, s2, 0.5);
+}
+
+// The same two series summed over the shared powers of u (4 extra multiplies,
+// a shorter dependency chain and fewer live values): the form of the tangent
+// passes, whose state already fills the registers and where the Horner
+// chains raise the register count past 256.
+__device__ inline void rotation_series_powers(double u, double& s1, double& s2) {
     const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
     s1 = fma(u, -1.0 / 6.0, 1.0);
     s1 = fma(u2, 1.0 / 120.0, s1);
@@ -53,8 +86,7 @@ __device__ inline void rotation_series(double u, double& s1, double& s2) {
     s2 = fma(u5, -1.0 / 479001600.0, s2);
 }
 
-// the same with one more term each, for u <= SERIES_LIMIT_WIDE
-__device__ inline void rotation_series_wide(double u, double& s1, double& s2) {
+__device__ inline void rotation_series_wide_powers(double u, double& s1, double& s2) {
     const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u3 * u3;
     s1 = fma(u, -1.0 / 6.0, 1.0);
     s1 = fma(u2, 1.0 / 120.0, s1);

@@ -47,7 +47,8 @@ struct TangentPlanes {
 };
 
 // s1'(u), s2'(u) from the Taylor series (term-wise derivatives, one more
-// term each; first omitted terms below 1e-18), u <= SERIES_LIMIT
+// term each; first omitted terms below 1e-18), u <= SERIES_LIMIT; summed over
+// the powers of u like rotation_series_powers
 __device__ inline void rotation_series_derivative(double u, double& s1p, double& s2p) {
     const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u;
     s1p = fma(u, 2.0 / 120.0, -1.0 / 6.0);
@@ -116,10 +117,10 @@ __device__ inline void rotate_all(
     using S = TangentSet<MASK>;
     double s1, s2, s1p = 0.0, s2p = 0.0;
     if (TIER == TIER_SERIES) {
-        rotation_series(u, s1, s2);
+        rotation_series_powers(u, s1, s2);
         if constexpr (S::axis) rotation_series_derivative(u, s1p, s2p);
     } else if (TIER == TIER_WIDE || u <= SERIES_LIMIT_WIDE) {
-        rotation_series_wide(u, s1, s2);
+        rotation_series_wide_powers(u, s1, s2);
         if constexpr (S::axis) rotation_series_derivative_wide(u, s1p, s2p);
     } else {
         rotation_any(u, s1, s2);
