@@ -278,6 +278,114 @@ __device__ inline void mfma_accumulate(
     }
 }
 
+// The same product with two LDS buffers: the split and store of the next K
+// step go to the other buffer while this step's MFMAs read the current one,
+// so each step needs one barrier instead of two and the staging overlaps the
+// matrix pipe.  Twice the LDS of mfma_accumulate.
+template<int P, int WM, int WN, int BKC>
+__device__ inline void mfma_accumulate_db(
+    f32x16 (&acc)[2][2], MfmaSmem<P, WM, WN, BKC> (&sm)[2],
+    const cfloat* __restrict__ a_src, float sa, bool vec_a,
+    const cfloat* __restrict__ b_src, float sb, bool vec_b,
+    int kb, int ke) {
+    using Tl = MfmaTile<P, WM, WN, BKC>;
+    using V = typename Half16<P>::V;
+    constexpr bool LO = Tl::LO;
+    constexpr int LD = Tl::LD;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int a_row = tid / Tl::A_TPR, a_k = (tid % Tl::A_TPR) * Tl::A_PER;
+    const int b_row = tid / Tl::B_TPR, b_k = (tid % Tl::B_TPR) * Tl::B_PER;
+    const uint32_t sel = (lane & 1) ? 0x01000302u : 0x03020100u;
+    const uint32_t flip = (lane & 1) ? 0u : 0x80000000u;
+
+    cfloat ra[Tl::A_PER], rb[Tl::B_PER];
+    auto gload = [&](int k0) {
+        const int ka = k0 + a_k;
+        if (vec_a && a_src && ka + Tl::A_PER <= ke) {
+#pragma unroll
+            for (int i = 0; i < Tl::A_PER / 2; i++) {
+                const float4 v = *reinterpret_cast<const float4*>(a_src + ka + 2 * i);
+                ra[2 * i] = {v.x, v.y};
+                ra[2 * i + 1] = {v.z, v.w};
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < Tl::A_PER; i++) ra[i] = (a_src && ka + i < ke) ? a_src[ka + i] : cfloat {0, 0};
+        }
+        const int kb_ = k0 + b_k;
+        if (vec_b && b_src && kb_ + Tl::B_PER <= ke) {
+#pragma unroll
+            for (int i = 0; i < Tl::B_PER / 2; i++) {
+                const float4 v = *reinterpret_cast<const float4*>(b_src + kb_ + 2 * i);
+                rb[2 * i] = {v.x, v.y};
+                rb[2 * i + 1] = {v.z, v.w};
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < Tl::B_PER; i++) rb[i] = (b_src && kb_ + i < ke) ? b_src[kb_ + i] : cfloat {0, 0};
+        }
+    };
+    auto stage = [&](MfmaSmem<P, WM, WN, BKC>& buf) {
+        uint32_t h[Tl::A_PER], l[Tl::A_PER];
+#pragma unroll
+        for (int i = 0; i < Tl::A_PER; i++) split_pair<P>(ra[i].re * sa, ra[i].im * sa, h[i], l[i]);
+        store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(buf.a_hi + a_row * LD + 2 * a_k), h);
+        if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(buf.a_lo + a_row * LD + 2 * a_k), l);
+        uint32_t hb[Tl::B_PER], lb[Tl::B_PER];
+#pragma unroll
+        for (int i = 0; i < Tl::B_PER; i++) split_pair<P>(rb[i].re * sb, rb[i].im * sb, hb[i], lb[i]);
+        store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_hi + b_row * LD + 2 * b_k), hb);
+        if constexpr (LO) store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_lo + b_row * LD + 2 * b_k), lb);
+    };
+    auto compute = [&](const MfmaSmem<P, WM, WN, BKC>& buf) {
+#pragma unroll
+        for (int kk = 0; kk < BKC / 8; kk++) {
+            const int koff = kk * 16 + (lane >> 5) * 8;
+            V ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int r = wm * 64 + i * 32 + (lane & 31);
+                ah[i] = *reinterpret_cast<const V*>(buf.a_hi + r * LD + koff);
+                if constexpr (LO) al[i] = *reinterpret_cast<const V*>(buf.a_lo + r * LD + koff);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int c = wn * 32 + j * 16 + (lane & 31) / 2;
+                bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(buf.b_hi + c * LD + koff), sel, flip);
+                if constexpr (LO)
+                    bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(buf.b_lo + c * LD + koff), sel, flip);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
+                    if constexpr (LO) {
+                        acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
+                        acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
+                    }
+                }
+        }
+    };
+
+    if (kb >= ke) return;  // uniform over the workgroup
+    gload(kb);
+    stage(sm[0]);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = kb; k0 < ke; k0 += BKC) {
+        const bool more = k0 + BKC < ke;  // uniform over the workgroup
+        if (more) gload(k0 + BKC);  // in flight during the MFMAs below
+        compute(sm[cur]);
+        if (more) stage(sm[cur ^ 1]);  // the other buffer: last read before the previous barrier
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
 __device__ inline void zero(f32x16 (&acc)[2][2]) {
 #pragma unroll
     for (int i = 0; i < 2; i++)

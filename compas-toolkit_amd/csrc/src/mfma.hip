@@ -25,7 +25,7 @@ namespace kernels {
 
 using namespace mfma;
 
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, bool DB>
 __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const cfloat* __restrict__ A,
     const cfloat* __restrict__ B,
@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     float2* __restrict__ partial,
     cfloat* __restrict__ direct, int nsub, int64_t out_outer) {
     using Tl = MfmaTile<P, WM, WN, BKC>;
-    __shared__ MfmaSmem<P, WM, WN, BKC> sm;
+    __shared__ MfmaSmem<P, WM, WN, BKC> sm[DB ? 2 : 1];
 
     const int total = tiles_m * tiles_n * splits;
     const int lid = xcd_logical_id(blockIdx.x, total);
@@ -62,8 +62,13 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     // 16-byte vector loads need 16-byte aligned rows
     const bool vec_a = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 2 == 0);
     const bool vec_b = ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % 2 == 0);
-    mfma_accumulate<P, WM, WN, BKC>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
-                                    bn < N ? B + int64_t(bn) * ldb : nullptr, sb, vec_b, kb, ke);
+    if constexpr (DB) {
+        mfma_accumulate_db<P, WM, WN, BKC>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
+                                           bn < N ? B + int64_t(bn) * ldb : nullptr, sb, vec_b, kb, ke);
+    } else {
+        mfma_accumulate<P, WM, WN, BKC>(acc, sm[0], am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
+                                        bn < N ? B + int64_t(bn) * ldb : nullptr, sb, vec_b, kb, ke);
+    }
 
     // ---- epilogue: D[row][realcol], realcol = 2n + {re, im}
 #pragma unroll
@@ -151,12 +156,12 @@ struct Plan {
 };
 
 // Workgroups of one variant that fit on a CU at once (LDS and registers).
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, bool DB>
 int resident_per_cu() {
     static const int cached = [] {
         int n = 0;
         COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, contraction_mfma<P, WM, WN, BKC>, MfmaTile<P, WM, WN, BKC>::THREADS, 0));
+            &n, contraction_mfma<P, WM, WN, BKC, DB>, MfmaTile<P, WM, WN, BKC>::THREADS, 0));
         return std::max(1, n);
     }();
     return cached;
@@ -167,7 +172,7 @@ int resident_per_cu() {
 // above a multiple of the slots leaves a nearly empty last round.  The split
 // count minimises rounds x K-chunk (the critical path), with a small charge
 // per split for the partial-sum traffic.
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, bool DB>
 Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
     using Tl = MfmaTile<P, WM, WN, BKC>;
     Plan p;
@@ -175,7 +180,7 @@ Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
     p.tiles_m = int(dev::div_ceil(s.M, Tl::BM));
     p.tiles_n = int(dev::div_ceil(s.N, Tl::BNC));
     const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
-    const int64_t slots = int64_t(resident_per_cu<P, WM, WN, BKC>()) * cu_count(ctx.device());
+    const int64_t slots = int64_t(resident_per_cu<P, WM, WN, BKC, DB>()) * cu_count(ctx.device());
     const int max_splits = std::max(1, std::min(64, s.K / (8 * BKC)));
     double best_cost = 1e300;
     p.splits = 1;
@@ -194,17 +199,17 @@ Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
     return p;
 }
 
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, bool DB>
 void launch_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, cfloat* out,
                     const ContractionShape& s, const float* sa, const float* sb) {
-    const Plan p = make_plan<P, WM, WN, BKC>(ctx, s);
+    const Plan p = make_plan<P, WM, WN, BKC, DB>(ctx, s);
     const int total = p.tiles_m * p.tiles_n * p.splits;
     const bool direct = (p.splits == 1 && !s.accumulate);
     Array<float2, 1> partial;
     if (!direct) partial = ctx.empty<float2, 1>({int64_t(p.splits) * s.M * s.N});
     {
         KernelScope ks(ctx, "contraction_mfma");
-        contraction_mfma<P, WM, WN, BKC><<<total, p.threads, 0, ctx.stream()>>>(
+        contraction_mfma<P, WM, WN, BKC, DB><<<total, p.threads, 0, ctx.stream()>>>(
             A, B, s.M, s.N, s.K, s.lda, s.ldb, p.k_chunk, p.tiles_m, p.tiles_n, p.splits, sa, sb,
             direct ? nullptr : partial.data(), direct ? out : nullptr, s.nsub, s.out_outer);
         check_launch("contraction_mfma");
@@ -226,18 +231,23 @@ struct Variant {
     VariantFn fn[NUM_P];  // indexed by P
 };
 
-#define COMPAS_MFMA_VARIANT(WM, WN, BKC)                                                          \
-    {#WM "x" #WN "x" #BKC,                                                                        \
-     {&launch_variant<P_BF16X3, WM, WN, BKC>, &launch_variant<P_FP16X3, WM, WN, BKC>,             \
-      &launch_variant<P_BF16X1, WM, WN, BKC>, &launch_variant<P_FP16X1, WM, WN, BKC>}}
+#define COMPAS_MFMA_VARIANT(WM, WN, BKC, DB, NAME)                                                        \
+    {NAME,                                                                                                \
+     {&launch_variant<P_BF16X3, WM, WN, BKC, DB>, &launch_variant<P_FP16X3, WM, WN, BKC, DB>,             \
+      &launch_variant<P_BF16X1, WM, WN, BKC, DB>, &launch_variant<P_FP16X1, WM, WN, BKC, DB>}}
 
+// WM x WN waves of 64 x 64 outputs, BKC complex K per step; "db": two LDS
+// buffers, one barrier per step
 const Variant VARIANTS[] = {
-    COMPAS_MFMA_VARIANT(2, 2, 32),
-    COMPAS_MFMA_VARIANT(4, 2, 32),
-    COMPAS_MFMA_VARIANT(2, 4, 32),
-    COMPAS_MFMA_VARIANT(4, 2, 16),
-    COMPAS_MFMA_VARIANT(2, 4, 16),
-    COMPAS_MFMA_VARIANT(4, 4, 16),
+    COMPAS_MFMA_VARIANT(2, 2, 32, false, "2x2x32"),
+    COMPAS_MFMA_VARIANT(4, 2, 32, false, "4x2x32"),
+    COMPAS_MFMA_VARIANT(2, 4, 32, false, "2x4x32"),
+    COMPAS_MFMA_VARIANT(4, 2, 16, false, "4x2x16"),
+    COMPAS_MFMA_VARIANT(2, 4, 16, false, "2x4x16"),
+    COMPAS_MFMA_VARIANT(4, 4, 16, false, "4x4x16"),
+    COMPAS_MFMA_VARIANT(2, 4, 16, true, "2x4x16db"),
+    COMPAS_MFMA_VARIANT(4, 2, 16, true, "4x2x16db"),
+    COMPAS_MFMA_VARIANT(2, 2, 32, true, "2x2x32db"),
 };
 constexpr int NUM_VARIANTS = int(sizeof(VARIANTS) / sizeof(VARIANTS[0]));
 constexpr int DEFAULT_VARIANT = 4;  // 2x4x16: fastest at 1120 x 896 x 50176 on MI355X
