@@ -263,16 +263,23 @@ __device__ inline void mfma_accumulate(
                 bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_hi + c * LD + koff), sel, flip);
                 if constexpr (LO) bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_lo + c * LD + koff), sel, flip);
             }
+            // hi*hi, then hi*lo, then lo*hi over the four accumulators: each
+            // accumulator's products stay in that order, and four independent
+            // MFMAs separate two that chain on one accumulator
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
-                    if constexpr (LO) {
-                        acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
-                        acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
-                    }
-                }
+                for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
+            if constexpr (LO) {
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
+            }
         }
         __syncthreads();
     }
@@ -358,16 +365,23 @@ __device__ inline void mfma_accumulate_db(
                 if constexpr (LO)
                     bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(buf.b_lo + c * LD + koff), sel, flip);
             }
+            // hi*hi, then hi*lo, then lo*hi over the four accumulators: each
+            // accumulator's products stay in that order, and four independent
+            // MFMAs separate two that chain on one accumulator
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
-                    if constexpr (LO) {
-                        acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
-                        acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
-                    }
-                }
+                for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
+            if constexpr (LO) {
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
+            }
         }
     };
 

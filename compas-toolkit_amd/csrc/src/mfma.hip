@@ -15,6 +15,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 #include "compas/device/common.h"
 #include "compas/device/mfma_tile.h"
@@ -254,6 +255,7 @@ constexpr int DEFAULT_VARIANT = 4;  // 2x4x16: fastest at 1120 x 896 x 50176 on 
 
 std::mutex g_tune_mutex;
 std::map<std::tuple<int, int, int, int64_t, int64_t, int, int, bool>, int> g_tuned;
+std::map<std::tuple<int, int, int, int64_t, int64_t, int, int, bool>, std::vector<float>> g_trial_ms;  // per variant
 int g_forced_variant = -1;
 
 bool autotune_enabled() {
@@ -292,6 +294,7 @@ int pick_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, con
     COMPAS_HIP_CHECK(hipEventCreate(&e1));
     int best = DEFAULT_VARIANT;
     float best_ms = 1e30f;
+    std::vector<float> trial(NUM_VARIANTS, 0.0f);
     for (int v = 0; v < NUM_VARIANTS; v++) {
         VARIANTS[v].fn[P](ctx, A, B, scratch.data(), t, sa, sb);  // warm-up
         COMPAS_HIP_CHECK(hipEventRecord(e0, ctx.stream()));
@@ -300,6 +303,7 @@ int pick_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, con
         COMPAS_HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
         COMPAS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        trial[v] = ms / 2;
         if (ms < best_ms) {
             best_ms = ms;
             best = v;
@@ -309,6 +313,7 @@ int pick_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, con
     (void)hipEventDestroy(e1);
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_tuned[key] = best;
+    g_trial_ms[key] = trial;
     return best;
 }
 
@@ -329,10 +334,20 @@ std::string contraction_tuning_report() {
     for (auto& kv : g_tuned) {
         char line[256];
         static const char* const names[NUM_P] = {"bf16x3", "fp16x3", "bf16x1", "fp16x1"};
-        std::snprintf(line, sizeof(line), "M=%d N=%d K=%d prec=%s -> %s\n", std::get<1>(kv.first),
+        std::snprintf(line, sizeof(line), "M=%d N=%d K=%d prec=%s -> %s", std::get<1>(kv.first),
                       std::get<2>(kv.first), std::get<5>(kv.first), names[std::get<6>(kv.first)],
                       VARIANTS[kv.second].name);
         out += line;
+        auto t = g_trial_ms.find(kv.first);
+        if (t != g_trial_ms.end()) {  // the trial time of every variant, ms
+            out += " (";
+            for (int v = 0; v < NUM_VARIANTS; v++) {
+                std::snprintf(line, sizeof(line), "%s%s %.3f", v ? ", " : "", VARIANTS[v].name, t->second[v]);
+                out += line;
+            }
+            out += ")";
+        }
+        out += "\n";
     }
     return out;
 }
