@@ -167,3 +167,81 @@ def test_random_corruption_never_reads_outside_the_frame():
         except ca.CompasError:
             rejected += 1
     assert rejected > 0
+
+
+class _StubServer:
+    """Host-only stand-in for the library server (the front end's contract:
+    submit_wire, wait_done, result_wire, release): 'runs' a request by
+    doubling its `x` entry, finishing after a seeded random delay so that
+    requests complete out of order."""
+
+    def __init__(self):
+        import threading
+
+        self.lock = threading.Lock()
+        self.next = 0
+        self.results = {}
+        self.released = set()
+        self.rng = np.random.default_rng(3)
+
+    def submit_wire(self, frame):
+        kind, e = wire.parse_frame(frame, wire.REQUEST_MAGIC)
+        if "x" not in e:
+            raise ValueError("request has no entry `x`")
+        with self.lock:
+            t = self.next
+            self.next += 1
+            self.results[t] = (2 * e["x"], float(self.rng.uniform(0, 0.02)))
+        return t
+
+    def wait_done(self, t):
+        import time
+
+        time.sleep(self.results[t][1])
+        return True
+
+    def result_wire(self, t):
+        return wire.build_frame(wire.RESULT_MAGIC, 0, {
+            "ticket": np.array(t, np.int64), "t_arrival": np.array(0.0), "t_start": np.array(0.0),
+            "t_done": np.array(0.0), "signal": self.results[t][0]})
+
+    def release(self, t):
+        with self.lock:
+            self.released.add(t)
+
+
+def test_tcp_front_end_order_and_errors_on_the_host():
+    import threading
+
+    server = _StubServer()
+    fe = wire.TcpFrontEnd(server)
+    got = {}
+
+    def client(k):
+        c = wire.TcpClient(fe.address)
+        frames = [wire.build_frame(wire.REQUEST_MAGIC, 0, {"x": np.full(3, 10 * k + i, np.complex64)})
+                  for i in range(15)]
+        for f in frames[:7]:
+            c.send(f)
+        c.send(b"garbage")  # rejected by the parser
+        c.send(wire.build_frame(wire.REQUEST_MAGIC, 0, {"y": np.zeros(1, np.float32)}))  # rejected by the server
+        for f in frames[7:]:
+            c.send(f)
+        got[k] = [c.recv() for _ in range(17)]
+        c.close()
+
+    threads = [threading.Thread(target=client, args=(k,)) for k in range(3)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=60)
+    fe.close()
+    for k in range(3):
+        res = got[k]
+        assert len(res) == 17
+        ok = [r for j, r in enumerate(res) if j not in (7, 8)]
+        # results in sending order on each connection, whatever order they finished in
+        assert [complex(r["signal"][0]) for r in ok] == [2 * (10 * k + i) for i in range(15)]
+        assert not res[7]["ok"] and "shorter than its header" in res[7]["error"]
+        assert not res[8]["ok"] and "no entry `x`" in res[8]["error"]
+    assert len(server.released) == 45
