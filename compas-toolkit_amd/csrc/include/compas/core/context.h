@@ -98,6 +98,9 @@ class Runtime {
     void abort_capture();
     void release_graph(int id);  // the graph is gone: its held blocks return to the pool
     bool capturing() const { return capture_id_ >= 0; }
+    // whether the contraction autotuner may time trials on this runtime
+    bool autotune() const { return autotune_; }
+    void set_autotune(bool on) { autotune_ = on; }
 
   private:
     int device_;
@@ -106,6 +109,7 @@ class Runtime {
     std::multimap<size_t, void*> free_blocks_;
     size_t in_use_ = 0, cached_ = 0;
     bool timing_ = false;
+    bool autotune_ = true;
     bool timing_before_capture_ = false;
     int capture_id_ = -1, next_graph_id_ = 0;
     std::unordered_map<void*, int> graph_blocks_;  // block allocated during a capture -> its graph

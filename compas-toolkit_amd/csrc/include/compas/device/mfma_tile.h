@@ -110,26 +110,29 @@ __device__ inline int xcd_logical_id(int b, int total) {
     return base + slot;
 }
 
-template<int P, int WM, int WN, int BKC>
+// WM x WN waves; each wave owns TM x TN 32 x 32 output tiles (TM row tiles
+// of A, TN tiles of 16 complex B columns); BKC complex K per step.
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
 struct MfmaTile {
     static constexpr int THREADS = 64 * WM * WN;
-    static constexpr int BM = 64 * WM;  // A rows
-    static constexpr int BNC = 32 * WN;  // complex B columns
+    static constexpr int BM = 32 * TM * WM;  // A rows
+    static constexpr int BNC = 16 * TN * WN;  // complex B columns
     static constexpr int LD = 2 * BKC + 8;  // LDS row stride, 16-bit elements (+16 bytes of padding)
     static constexpr bool LO = has_lo(P);
-    static constexpr int A_PER = BKC / WN;  // complex per thread per step (contiguous in K)
-    static constexpr int A_TPR = WN;  // threads per A row
-    static constexpr int B_PER = BKC / (2 * WM);
-    static constexpr int B_TPR = 2 * WM;
+    static constexpr int A_TPR = THREADS / BM;  // threads per A row
+    static constexpr int A_PER = BKC / A_TPR;  // complex per thread per step (contiguous in K)
+    static constexpr int B_TPR = THREADS / BNC;
+    static constexpr int B_PER = BKC / B_TPR;
     static constexpr int LDS_BYTES = (BM + BNC) * LD * 2 * (LO ? 2 : 1);
+    static_assert(THREADS % BM == 0 && THREADS % BNC == 0, "staging rows");
     static_assert(A_PER >= 4 && A_PER % 4 == 0, "A staging granularity");
     static_assert(B_PER >= 2 && B_PER % 2 == 0, "B staging granularity");
 };
 
 // LDS of one workgroup: packed hi (and lo) parts of the A and B tiles
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
 struct MfmaSmem {
-    using Tl = MfmaTile<P, WM, WN, BKC>;
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
     using T = typename Half16<P>::T;
     __attribute__((aligned(16))) T a_hi[Tl::BM * Tl::LD];
     __attribute__((aligned(16))) T b_hi[Tl::BNC * Tl::LD];  // one row per complex column
@@ -162,13 +165,13 @@ __device__ inline V b_row_form_t(uint4 w, uint32_t sel, uint32_t flip) {
 
 // Row of the A tile that thread `tid` stages, and row of the B tile (complex
 // column).  Each thread stages one A row and one B row for the whole K range.
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
 __device__ inline int staged_a_row(int tid) {
-    return tid / MfmaTile<P, WM, WN, BKC>::A_TPR;
+    return tid / MfmaTile<P, WM, WN, BKC, TM, TN>::A_TPR;
 }
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
 __device__ inline int staged_b_row(int tid) {
-    return tid / MfmaTile<P, WM, WN, BKC>::B_TPR;
+    return tid / MfmaTile<P, WM, WN, BKC, TM, TN>::B_TPR;
 }
 
 // acc += A_tile[.., kb:ke] * B_tile[.., kb:ke]^T for the workgroup's tile.
@@ -176,13 +179,13 @@ __device__ inline int staged_b_row(int tid) {
 // K-contiguous), nullptr for a row outside the operand (staged as zeros);
 // sa / sb scale the row's values before the split; vec_*: the row may be read
 // with 16-byte loads.  All threads of the workgroup must call this together.
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
 __device__ inline void mfma_accumulate(
-    f32x16 (&acc)[2][2], MfmaSmem<P, WM, WN, BKC>& sm,
+    f32x16 (&acc)[TM][TN], MfmaSmem<P, WM, WN, BKC, TM, TN>& sm,
     const cfloat* __restrict__ a_src, float sa, bool vec_a,
     const cfloat* __restrict__ b_src, float sb, bool vec_b,
     int kb, int ke) {
-    using Tl = MfmaTile<P, WM, WN, BKC>;
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
     using T = typename Half16<P>::T;
     using V = typename Half16<P>::V;
     constexpr bool LO = Tl::LO;
@@ -250,35 +253,35 @@ __device__ inline void mfma_accumulate(
 #pragma unroll
         for (int kk = 0; kk < BKC / 8; kk++) {
             const int koff = kk * 16 + (lane >> 5) * 8;
-            V ah[2], al[2], bh[2], bl[2];
+            V ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const int r = wm * 64 + i * 32 + (lane & 31);
+            for (int i = 0; i < TM; i++) {
+                const int r = wm * (32 * TM) + i * 32 + (lane & 31);
                 ah[i] = *reinterpret_cast<const V*>(sm.a_hi + r * LD + koff);
                 if constexpr (LO) al[i] = *reinterpret_cast<const V*>(sm.a_lo + r * LD + koff);
             }
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int c = wn * 32 + j * 16 + (lane & 31) / 2;  // complex column of this lane
+            for (int j = 0; j < TN; j++) {
+                const int c = wn * (16 * TN) + j * 16 + (lane & 31) / 2;  // complex column of this lane
                 bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_hi + c * LD + koff), sel, flip);
                 if constexpr (LO) bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_lo + c * LD + koff), sel, flip);
             }
-            // hi*hi, then hi*lo, then lo*hi over the four accumulators: each
-            // accumulator's products stay in that order, and four independent
+            // hi*hi, then hi*lo, then lo*hi over the accumulators: each
+            // accumulator's products stay in that order, and TM x TN independent
             // MFMAs separate two that chain on one accumulator
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < TM; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
+                for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
             if constexpr (LO) {
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int i = 0; i < TM; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
+                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int i = 0; i < TM; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
+                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
             }
         }
         __syncthreads();
@@ -289,13 +292,13 @@ __device__ inline void mfma_accumulate(
 // step go to the other buffer while this step's MFMAs read the current one,
 // so each step needs one barrier instead of two and the staging overlaps the
 // matrix pipe.  Twice the LDS of mfma_accumulate.
-template<int P, int WM, int WN, int BKC>
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
 __device__ inline void mfma_accumulate_db(
-    f32x16 (&acc)[2][2], MfmaSmem<P, WM, WN, BKC> (&sm)[2],
+    f32x16 (&acc)[TM][TN], MfmaSmem<P, WM, WN, BKC, TM, TN> (&sm)[2],
     const cfloat* __restrict__ a_src, float sa, bool vec_a,
     const cfloat* __restrict__ b_src, float sb, bool vec_b,
     int kb, int ke) {
-    using Tl = MfmaTile<P, WM, WN, BKC>;
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
     using V = typename Half16<P>::V;
     constexpr bool LO = Tl::LO;
     constexpr int LD = Tl::LD;
@@ -335,7 +338,7 @@ __device__ inline void mfma_accumulate_db(
             for (int i = 0; i < Tl::B_PER; i++) rb[i] = (b_src && kb_ + i < ke) ? b_src[kb_ + i] : cfloat {0, 0};
         }
     };
-    auto stage = [&](MfmaSmem<P, WM, WN, BKC>& buf) {
+    auto stage = [&](MfmaSmem<P, WM, WN, BKC, TM, TN>& buf) {
         uint32_t h[Tl::A_PER], l[Tl::A_PER];
 #pragma unroll
         for (int i = 0; i < Tl::A_PER; i++) split_pair<P>(ra[i].re * sa, ra[i].im * sa, h[i], l[i]);
@@ -347,40 +350,40 @@ __device__ inline void mfma_accumulate_db(
         store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_hi + b_row * LD + 2 * b_k), hb);
         if constexpr (LO) store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_lo + b_row * LD + 2 * b_k), lb);
     };
-    auto compute = [&](const MfmaSmem<P, WM, WN, BKC>& buf) {
+    auto compute = [&](const MfmaSmem<P, WM, WN, BKC, TM, TN>& buf) {
 #pragma unroll
         for (int kk = 0; kk < BKC / 8; kk++) {
             const int koff = kk * 16 + (lane >> 5) * 8;
-            V ah[2], al[2], bh[2], bl[2];
+            V ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const int r = wm * 64 + i * 32 + (lane & 31);
+            for (int i = 0; i < TM; i++) {
+                const int r = wm * (32 * TM) + i * 32 + (lane & 31);
                 ah[i] = *reinterpret_cast<const V*>(buf.a_hi + r * LD + koff);
                 if constexpr (LO) al[i] = *reinterpret_cast<const V*>(buf.a_lo + r * LD + koff);
             }
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int c = wn * 32 + j * 16 + (lane & 31) / 2;
+            for (int j = 0; j < TN; j++) {
+                const int c = wn * (16 * TN) + j * 16 + (lane & 31) / 2;
                 bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(buf.b_hi + c * LD + koff), sel, flip);
                 if constexpr (LO)
                     bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(buf.b_lo + c * LD + koff), sel, flip);
             }
-            // hi*hi, then hi*lo, then lo*hi over the four accumulators: each
-            // accumulator's products stay in that order, and four independent
+            // hi*hi, then hi*lo, then lo*hi over the accumulators: each
+            // accumulator's products stay in that order, and TM x TN independent
             // MFMAs separate two that chain on one accumulator
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < TM; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
+                for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
             if constexpr (LO) {
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int i = 0; i < TM; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
+                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int i = 0; i < TM; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
+                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
             }
         }
     };
@@ -400,11 +403,12 @@ __device__ inline void mfma_accumulate_db(
     }
 }
 
-__device__ inline void zero(f32x16 (&acc)[2][2]) {
+template<int TM, int TN>
+__device__ inline void zero(f32x16 (&acc)[TM][TN]) {
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < TN; j++)
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
 }

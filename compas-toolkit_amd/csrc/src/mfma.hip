@@ -4,7 +4,8 @@
 // staging, MFMA main loop) lives in compas/device/mfma_tile.h.  This file
 // holds the plain contraction kernel, its split-K plan and reduction, and a
 // small autotuner: the tile variants (WM, WN, BKC) are timed on the live shape
-// on first use and the fastest is cached per shape.  Workgroups are numbered
+// on first use and the fastest is cached per shape (runtimes that opt out,
+// such as the request server's slots, run the default tile on new shapes).  Workgroups are numbered
 // so that those sharing A rows run on the same XCD (consecutive hardware ids
 // are spread over the 8 XCDs), so the XCD's L2 serves the re-reads of A
 // across column tiles.  K slices are reduced in FP64.
@@ -26,7 +27,7 @@ namespace kernels {
 
 using namespace mfma;
 
-template<int P, int WM, int WN, int BKC, bool DB>
+template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
 __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const cfloat* __restrict__ A,
     const cfloat* __restrict__ B,
@@ -36,8 +37,8 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const float* __restrict__ row_scale_b,
     float2* __restrict__ partial,
     cfloat* __restrict__ direct, int nsub, int64_t out_outer) {
-    using Tl = MfmaTile<P, WM, WN, BKC>;
-    __shared__ MfmaSmem<P, WM, WN, BKC> sm[DB ? 2 : 1];
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
+    __shared__ MfmaSmem<P, WM, WN, BKC, TM, TN> sm[DB ? 2 : 1];
 
     const int total = tiles_m * tiles_n * splits;
     const int lid = xcd_logical_id(blockIdx.x, total);
@@ -53,33 +54,33 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
 
-    f32x16 acc[2][2];
+    f32x16 acc[TM][TN];
     zero(acc);
 
-    const int am = m0 + staged_a_row<P, WM, WN, BKC>(tid);
-    const int bn = n0 + staged_b_row<P, WM, WN, BKC>(tid);
+    const int am = m0 + staged_a_row<P, WM, WN, BKC, TM, TN>(tid);
+    const int bn = n0 + staged_b_row<P, WM, WN, BKC, TM, TN>(tid);
     const float sa = (row_scale_a && am < M) ? row_scale_a[am] : 1.0f;
     const float sb = (row_scale_b && bn < N) ? row_scale_b[bn] : 1.0f;
     // 16-byte vector loads need 16-byte aligned rows
     const bool vec_a = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 2 == 0);
     const bool vec_b = ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % 2 == 0);
     if constexpr (DB) {
-        mfma_accumulate_db<P, WM, WN, BKC>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
+        mfma_accumulate_db<P, WM, WN, BKC, TM, TN>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
                                            bn < N ? B + int64_t(bn) * ldb : nullptr, sb, vec_b, kb, ke);
     } else {
-        mfma_accumulate<P, WM, WN, BKC>(acc, sm[0], am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
+        mfma_accumulate<P, WM, WN, BKC, TM, TN>(acc, sm[0], am < M ? A + int64_t(am) * lda : nullptr, sa, vec_a,
                                         bn < N ? B + int64_t(bn) * ldb : nullptr, sb, vec_b, kb, ke);
     }
 
     // ---- epilogue: D[row][realcol], realcol = 2n + {re, im}
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < TN; j++)
 #pragma unroll
             for (int e = 0; e < 16; e++) {
-                const int row = wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                const int rc = wn * 64 + j * 32 + (lane & 31);
+                const int row = wm * (32 * TM) + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                const int rc = wn * (32 * TN) + j * 32 + (lane & 31);
                 const int m = m0 + row, n = n0 + (rc >> 1), part = rc & 1;
                 if (m >= M || n >= N) continue;
                 float v = acc[i][j][e];
@@ -157,12 +158,12 @@ struct Plan {
 };
 
 // Workgroups of one variant that fit on a CU at once (LDS and registers).
-template<int P, int WM, int WN, int BKC, bool DB>
+template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
 int resident_per_cu() {
     static const int cached = [] {
         int n = 0;
         COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, contraction_mfma<P, WM, WN, BKC, DB>, MfmaTile<P, WM, WN, BKC>::THREADS, 0));
+            &n, contraction_mfma<P, WM, WN, BKC, DB, TM, TN>, MfmaTile<P, WM, WN, BKC, TM, TN>::THREADS, 0));
         return std::max(1, n);
     }();
     return cached;
@@ -173,15 +174,15 @@ int resident_per_cu() {
 // above a multiple of the slots leaves a nearly empty last round.  The split
 // count minimises rounds x K-chunk (the critical path), with a small charge
 // per split for the partial-sum traffic.
-template<int P, int WM, int WN, int BKC, bool DB>
+template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
 Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
-    using Tl = MfmaTile<P, WM, WN, BKC>;
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
     Plan p;
     p.threads = Tl::THREADS;
     p.tiles_m = int(dev::div_ceil(s.M, Tl::BM));
     p.tiles_n = int(dev::div_ceil(s.N, Tl::BNC));
     const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
-    const int64_t slots = int64_t(resident_per_cu<P, WM, WN, BKC, DB>()) * cu_count(ctx.device());
+    const int64_t slots = int64_t(resident_per_cu<P, WM, WN, BKC, DB, TM, TN>()) * cu_count(ctx.device());
     const int max_splits = std::max(1, std::min(64, s.K / (8 * BKC)));
     double best_cost = 1e300;
     p.splits = 1;
@@ -200,17 +201,17 @@ Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
     return p;
 }
 
-template<int P, int WM, int WN, int BKC, bool DB>
+template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
 void launch_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, cfloat* out,
                     const ContractionShape& s, const float* sa, const float* sb) {
-    const Plan p = make_plan<P, WM, WN, BKC, DB>(ctx, s);
+    const Plan p = make_plan<P, WM, WN, BKC, DB, TM, TN>(ctx, s);
     const int total = p.tiles_m * p.tiles_n * p.splits;
     const bool direct = (p.splits == 1 && !s.accumulate);
     Array<float2, 1> partial;
     if (!direct) partial = ctx.empty<float2, 1>({int64_t(p.splits) * s.M * s.N});
     {
         KernelScope ks(ctx, "contraction_mfma");
-        contraction_mfma<P, WM, WN, BKC, DB><<<total, p.threads, 0, ctx.stream()>>>(
+        contraction_mfma<P, WM, WN, BKC, DB, TM, TN><<<total, p.threads, 0, ctx.stream()>>>(
             A, B, s.M, s.N, s.K, s.lda, s.ldb, p.k_chunk, p.tiles_m, p.tiles_n, p.splits, sa, sb,
             direct ? nullptr : partial.data(), direct ? out : nullptr, s.nsub, s.out_outer);
         check_launch("contraction_mfma");
@@ -232,23 +233,23 @@ struct Variant {
     VariantFn fn[NUM_P];  // indexed by P
 };
 
-#define COMPAS_MFMA_VARIANT(WM, WN, BKC, DB, NAME)                                                        \
-    {NAME,                                                                                                \
-     {&launch_variant<P_BF16X3, WM, WN, BKC, DB>, &launch_variant<P_FP16X3, WM, WN, BKC, DB>,             \
-      &launch_variant<P_BF16X1, WM, WN, BKC, DB>, &launch_variant<P_FP16X1, WM, WN, BKC, DB>}}
+#define COMPAS_MFMA_VARIANT(WM, WN, BKC, DB, TM, TN, NAME)                                                 \
+    {NAME,                                                                                                    \
+     {&launch_variant<P_BF16X3, WM, WN, BKC, DB, TM, TN>, &launch_variant<P_FP16X3, WM, WN, BKC, DB, TM, TN>, \
+      &launch_variant<P_BF16X1, WM, WN, BKC, DB, TM, TN>, &launch_variant<P_FP16X1, WM, WN, BKC, DB, TM, TN>}}
 
 // WM x WN waves of 64 x 64 outputs, BKC complex K per step; "db": two LDS
-// buffers, one barrier per step
+// buffers, one barrier per step.  (Tiles of 4 x 2 and 2 x 4 sub-tiles per
+// wave, MfmaTile's TM / TN, measured 1.25-2.10 ms against 1.13 ms for 2x4x16
+// at 1120 x 896 x 50176, and are not offered.)
 const Variant VARIANTS[] = {
-    COMPAS_MFMA_VARIANT(2, 2, 32, false, "2x2x32"),
-    COMPAS_MFMA_VARIANT(4, 2, 32, false, "4x2x32"),
-    COMPAS_MFMA_VARIANT(2, 4, 32, false, "2x4x32"),
-    COMPAS_MFMA_VARIANT(4, 2, 16, false, "4x2x16"),
-    COMPAS_MFMA_VARIANT(2, 4, 16, false, "2x4x16"),
-    COMPAS_MFMA_VARIANT(4, 4, 16, false, "4x4x16"),
-    COMPAS_MFMA_VARIANT(2, 4, 16, true, "2x4x16db"),
-    COMPAS_MFMA_VARIANT(4, 2, 16, true, "4x2x16db"),
-    COMPAS_MFMA_VARIANT(2, 2, 32, true, "2x2x32db"),
+    COMPAS_MFMA_VARIANT(2, 2, 32, false, 2, 2, "2x2x32"),
+    COMPAS_MFMA_VARIANT(4, 2, 32, false, 2, 2, "4x2x32"),
+    COMPAS_MFMA_VARIANT(2, 4, 32, false, 2, 2, "2x4x32"),
+    COMPAS_MFMA_VARIANT(4, 2, 16, false, 2, 2, "4x2x16"),
+    COMPAS_MFMA_VARIANT(2, 4, 16, false, 2, 2, "2x4x16"),
+    COMPAS_MFMA_VARIANT(4, 4, 16, false, 2, 2, "4x4x16"),
+    COMPAS_MFMA_VARIANT(2, 4, 16, true, 2, 2, "2x4x16db"),
 };
 constexpr int NUM_VARIANTS = int(sizeof(VARIANTS) / sizeof(VARIANTS[0]));
 constexpr int DEFAULT_VARIANT = 4;  // 2x4x16: fastest at 1120 x 896 x 50176 on MI355X
@@ -280,6 +281,9 @@ int pick_variant(const CompasContext& ctx, const cfloat* A, const cfloat* B, con
         if (v >= 0 && v < NUM_VARIANTS) return v;
     }
     if (!autotune_enabled() || int64_t(s.M) * s.N * s.K < (int64_t(1) << 24)) return DEFAULT_VARIANT;
+    // runtimes that do not time trials (the request server's slots, whose
+    // request sizes vary per request) run the default tile for a new shape
+    if (!ctx.runtime()->autotune()) return DEFAULT_VARIANT;
     // timing trials synchronise the host: not inside a graph capture (the
     // default tile is captured; a shape run once eagerly first uses its tuned one)
     if (ctx.runtime()->capturing()) return DEFAULT_VARIANT;

@@ -41,7 +41,14 @@ Server::Server(const CompasContext& base, int slots, size_t memory_budget) :
     base_(base), budget_(memory_budget), t0_(std::chrono::steady_clock::now()) {
     COMPAS_CHECK(slots >= 1 && slots <= 32);
     base.activate();
-    for (int i = 0; i < slots; i++) slots_.emplace_back(std::make_shared<Runtime>(base.device()));
+    for (int i = 0; i < slots; i++) {
+        auto rt = std::make_shared<Runtime>(base.device());
+        // no autotuning trials on the request path: request sizes vary per
+        // request, so every request would time every tile variant; shapes
+        // tuned elsewhere keep their tile, new ones run the default
+        rt->set_autotune(false);
+        slots_.emplace_back(rt);
+    }
     slot_busy_.assign(size_t(slots), false);
     thread_ = std::thread([this] { dispatch_loop(); });
 }
