@@ -132,7 +132,10 @@ std::shared_ptr<Runtime> runtime_for_device(int device);
 class Buffer {
   public:
     Buffer(std::shared_ptr<Runtime> rt, size_t nbytes);
-    Buffer(std::shared_ptr<Runtime> rt, void* external, size_t nbytes, std::function<void()> release);
+    // sync_first: wait for the runtime's queued work before the hook (memory a
+    // caller owns); false when the hook orders the release itself
+    Buffer(std::shared_ptr<Runtime> rt, void* external, size_t nbytes, std::function<void()> release,
+           bool sync_first = true);
     ~Buffer();
     Buffer(const Buffer&) = delete;
     Buffer& operator=(const Buffer&) = delete;
@@ -147,6 +150,7 @@ class Buffer {
     void* ptr_ = nullptr;
     size_t nbytes_ = 0;
     bool external_ = false;
+    bool sync_first_ = true;
     std::function<void()> release_;
 };
 

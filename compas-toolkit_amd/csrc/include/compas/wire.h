@@ -102,8 +102,9 @@ ServeRequest upload_request(const CompasContext& ctx, const Frame& request);
 // request's entries as upload_request does, without a GPU.
 std::string describe_request(const Frame& request);
 
-// Result frame of a finished request (reads its arrays back to the host).
-std::vector<uint8_t> encode_result(int64_t ticket, const ServeResult& result);
+// Result frame of a finished request; its arrays are read back on the
+// stream of `readback` (not behind later work of the producing slot).
+std::vector<uint8_t> encode_result(int64_t ticket, const ServeResult& result, const CompasContext& readback);
 
 }  // namespace wire
 }  // namespace compas

@@ -939,7 +939,7 @@ int64_t compas_server_result_wire(CompasServerHandle* server, int64_t ticket, vo
         Server& s = server_of(server);
         if (cached_server != &s || cached_ticket != ticket) {
             if (!s.wait(ticket, 0.0)) throw CompasException("request has not finished");
-            cached = wire::encode_result(ticket, s.result(ticket));
+            cached = wire::encode_result(ticket, s.result(ticket), s.base());
             cached_server = &s;
             cached_ticket = ticket;
         }

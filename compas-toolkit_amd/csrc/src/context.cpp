@@ -328,11 +328,13 @@ Buffer::Buffer(std::shared_ptr<Runtime> rt, size_t nbytes) : rt_(std::move(rt)),
     ptr_ = rt_->allocate(nbytes);
 }
 
-Buffer::Buffer(std::shared_ptr<Runtime> rt, void* external, size_t nbytes, std::function<void()> release) :
+Buffer::Buffer(
+    std::shared_ptr<Runtime> rt, void* external, size_t nbytes, std::function<void()> release, bool sync_first) :
     rt_(std::move(rt)),
     ptr_(external),
     nbytes_(nbytes),
     external_(true),
+    sync_first_(sync_first),
     release_(std::move(release)) {}
 
 Buffer::~Buffer() {
@@ -345,7 +347,7 @@ Buffer::~Buffer() {
             // queued kernels may still read the memory: let them finish before
             // its owner may reuse it (arrays a captured graph reads must outlive
             // the graph, wrapped or not)
-            if (!rt_->capturing()) rt_->synchronize();
+            if (sync_first_ && !rt_->capturing()) rt_->synchronize();
             release_();
         } catch (...) {  // a destructor must not throw; the owner's hook failed
         }

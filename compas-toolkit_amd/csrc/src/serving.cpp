@@ -35,6 +35,29 @@ Array<cfloat, 2> derivative(const CompasContext& ctx, int field, const Array<cfl
     return simulate_magnetization_derivative(ctx, field, echos, r.parameters, *r.pssfp);
 }
 
+// A view of a finished result on another runtime: same memory, kept alive by
+// the view and read in that runtime's stream order.  When the view goes, the
+// producing runtime's stream waits (an event, no host wait) for the work
+// queued on the view's runtime before the memory returns to its pool.
+template<typename T, size_t N>
+Array<T, N> rehome(const Array<T, N>& a, const std::shared_ptr<Runtime>& rt) {
+    if (!a.buffer()) return a;
+    std::shared_ptr<Buffer> orig = a.buffer();
+    auto release = [orig, rt] {
+        rt->activate();
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            rt->synchronize();  // no event: order by waiting instead
+            return;
+        }
+        COMPAS_HIP_CHECK(hipEventRecord(ev, rt->stream()));
+        COMPAS_HIP_CHECK(hipStreamWaitEvent(orig->runtime()->stream(), ev, 0));
+        (void)hipEventDestroy(ev);
+    };
+    auto view = std::make_shared<Buffer>(rt, orig->data(), orig->nbytes(), release, false);
+    return Array<T, N>(std::move(view), a.sizes());
+}
+
 }  // namespace
 
 Server::Server(const CompasContext& base, int slots, size_t memory_budget) :
@@ -144,6 +167,11 @@ void Server::dispatch_loop() {
                 continue;
             }
             if (q != hipSuccess) e.result.error = hipGetErrorString(q);
+            // the results are final: hand them to the base runtime, so reading
+            // them does not queue behind the next request on this slot's stream
+            e.result.echos = rehome(e.result.echos, base_.runtime());
+            e.result.signal = rehome(e.result.signal, base_.runtime());
+            e.result.jhy = rehome(e.result.jhy, base_.runtime());
             e.result.t_done = now();
             (void)hipEventDestroy(e.done_event);
             e.done_event = nullptr;

@@ -272,10 +272,18 @@ Array<T, N> upload(const CompasContext& ctx, const Entry& e) {
     return ctx.allocate_from<T, N>(reinterpret_cast<const T*>(e.data), sizes);
 }
 
+// The array read back on `stream`: the request has finished (its completion
+// event has fired), so the copy need not queue behind the later work of the
+// slot stream that produced it.
 template<size_t N>
-void put_array(Writer& w, const char* name, const Array<cfloat, N>& a) {
+void put_array(Writer& w, const char* name, const Array<cfloat, N>& a, hipStream_t stream, int device) {
     if (!a.buffer()) return;  // not produced by this kind of request
-    const auto host = a.copy_to_vector();
+    std::vector<cfloat> host(static_cast<size_t>(a.numel()));
+    if (!host.empty()) {
+        COMPAS_HIP_CHECK(hipSetDevice(device));
+        COMPAS_HIP_CHECK(hipMemcpyAsync(host.data(), a.data(), a.nbytes(), hipMemcpyDeviceToHost, stream));
+        COMPAS_HIP_CHECK(hipStreamSynchronize(stream));
+    }
     int64_t dims[3] = {1, 1, 1};
     for (size_t i = 0; i < N; i++) dims[i] = a.size(i);
     w.add(name, C64, int(N), dims, host.data());
@@ -367,7 +375,7 @@ std::string describe_request(const Frame& f) {
     return ss.str();
 }
 
-std::vector<uint8_t> encode_result(int64_t ticket, const ServeResult& result) {
+std::vector<uint8_t> encode_result(int64_t ticket, const ServeResult& result, const CompasContext& readback) {
     const bool failed = !result.error.empty();
     Writer w(RESULT_MAGIC, failed ? 1 : 0);
     w.scalar_i64("ticket", ticket);
@@ -379,9 +387,10 @@ std::vector<uint8_t> encode_result(int64_t ticket, const ServeResult& result) {
         w.add("error", U8, 1, &n, result.error.data());
         return w.finish();
     }
-    if (result.return_echos) put_array(w, "echos", result.echos);
-    put_array(w, "signal", result.signal);
-    put_array(w, "jhy", result.jhy);
+    const hipStream_t st = readback.stream();
+    if (result.return_echos) put_array(w, "echos", result.echos, st, readback.device());
+    put_array(w, "signal", result.signal, st, readback.device());
+    put_array(w, "jhy", result.jhy, st, readback.device());
     return w.finish();
 }
 
