@@ -12,6 +12,7 @@
 // others run starts as soon as a slot and memory free up.  Completion is
 // detected with an event per request; results stay in HBM until taken.
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -75,6 +76,7 @@ class Server {
     size_t estimate_bytes(const ServeRequest& r) const;
     double now() const;
     const CompasContext& base() const { return base_; }  // where callers upload request data
+    uint64_t id() const { return id_; }  // unique per server in the process
     int slots() const { return int(slots_.size()); }
 
   private:
@@ -91,6 +93,7 @@ class Server {
     void launch(Entry& e, int slot);
 
     CompasContext base_;
+    uint64_t id_;
     std::vector<CompasContext> slots_;
     std::vector<bool> slot_busy_;
     size_t budget_, in_flight_bytes_ = 0;

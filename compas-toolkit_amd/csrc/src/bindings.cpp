@@ -841,7 +841,7 @@ struct ServerHandle {
 
 // The result frame of the last size query on this thread, so the usual
 // query-then-copy pair of compas_server_result_wire reads the result back once.
-thread_local Server* cached_server = nullptr;
+thread_local uint64_t cached_server = 0;  // Server::id(), never 0
 thread_local int64_t cached_ticket = -1;
 thread_local std::vector<uint8_t> cached;
 
@@ -887,8 +887,8 @@ CompasServerHandle* compas_server_create(const CompasContextHandle* ctx, int slo
 void compas_server_destroy(CompasServerHandle* server) {
     catch_exceptions([&] {
         auto* h = reinterpret_cast<ServerHandle*>(server);
-        if (h && cached_server == h->server.get()) {
-            cached_server = nullptr;
+        if (h && h->server && cached_server == h->server->id()) {
+            cached_server = 0;
             cached_ticket = -1;
             std::vector<uint8_t>().swap(cached);
         }
@@ -937,16 +937,16 @@ int64_t compas_server_submit_wire(CompasServerHandle* server, const void* frame,
 int64_t compas_server_result_wire(CompasServerHandle* server, int64_t ticket, void* buf, int64_t capacity) {
     const int64_t n = catch_exceptions([&]() -> int64_t {
         Server& s = server_of(server);
-        if (cached_server != &s || cached_ticket != ticket) {
+        if (cached_server != s.id() || cached_ticket != ticket) {
             if (!s.wait(ticket, 0.0)) throw CompasException("request has not finished");
             cached = wire::encode_result(ticket, s.result(ticket), s.base());
-            cached_server = &s;
+            cached_server = s.id();
             cached_ticket = ticket;
         }
         const int64_t n = int64_t(cached.size());
         if (buf != nullptr && capacity >= n) {
             std::memcpy(buf, cached.data(), cached.size());
-            cached_server = nullptr;
+            cached_server = 0;
             cached_ticket = -1;
             std::vector<uint8_t>().swap(cached);
         }

@@ -60,8 +60,12 @@ Array<T, N> rehome(const Array<T, N>& a, const std::shared_ptr<Runtime>& rt) {
 
 }  // namespace
 
+namespace {
+std::atomic<uint64_t> g_next_server_id {1};
+}
+
 Server::Server(const CompasContext& base, int slots, size_t memory_budget) :
-    base_(base), budget_(memory_budget), t0_(std::chrono::steady_clock::now()) {
+    base_(base), id_(g_next_server_id++), budget_(memory_budget), t0_(std::chrono::steady_clock::now()) {
     COMPAS_CHECK(slots >= 1 && slots <= 32);
     base.activate();
     for (int i = 0; i < slots; i++) {
