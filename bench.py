@@ -753,6 +753,11 @@ def serve_wire_bench(args):
     }))
 
 
+def progress(msg):
+    """A progress line on stderr (long serving runs would otherwise be silent)."""
+    print(f"[serve] {msg}", file=sys.stderr, flush=True)
+
+
 def serve_bench(args):
     """Serving loop: requests with Poisson arrivals at `--load` x the measured
     capacity, continuous batching on `--slots` slots, p50/p99 latency (arrival
@@ -765,9 +770,14 @@ def serve_bench(args):
     specs = [serve_request_spec(i) for i in range(n_req)]
     t0 = time.perf_counter()
     cat = ServeCatalog()
-    reqs = [cat.request(sp) for sp in specs]
+    reqs = []
+    for i, sp in enumerate(specs):
+        reqs.append(cat.request(sp))
+        if (i + 1) % 1000 == 0:
+            progress(f"set-up: {i + 1} of {n_req} requests resident")
     ctx.synchronize()
     t_setup = time.perf_counter() - t0
+    progress(f"set-up done in {t_setup:.1f} s")
     check_ids = set(np.random.default_rng(5).choice(n_req, size=min(args.check, n_req), replace=False).tolist())
     budget = int(args.budget_gib * (1 << 30))
 
@@ -783,6 +793,7 @@ def serve_bench(args):
     cal_span = max(d for _, _, d in times) - min(s for _, s, _ in times)
     capacity = len(cal_ids) / cal_span
     lam = args.load * capacity
+    progress(f"capacity {capacity:.1f} requests/s; arrivals at {lam:.1f}/s")
 
     # open loop: Poisson arrivals, a collector thread waits and releases in order
     gaps = np.random.default_rng(7).exponential(1.0 / lam, size=n_req)
@@ -813,7 +824,10 @@ def serve_bench(args):
             time.sleep(delay)
         tickets[i] = serve_submit(server, reqs[i])
         submitted.release()
+        if (i + 1) % 500 == 0:
+            progress(f"submitted {i + 1} of {n_req}")
     th.join()
+    progress("all requests done; re-running the checked ones alone")
     server.close()
 
     ok = [s for s in stamps if s is not None]

@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat*
 
     Vec3d m[ZPL];
     double az[ZPL], cpr[ZPL], spr[ZPL];
+    double az2max = 0.0;  // max over the lane's slice positions of a_z^2
     bool valid[ZPL];
 #pragma unroll
     for (int i = 0; i < ZPL; i++) {
@@ -74,6 +75,7 @@ __global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat*
         valid[i] = iz < a.nz;
         const double zz = valid[i] ? double(a.z[iz]) : 0.0;
         az[i] = -(double(a.g_ex) * zz) - off_ex;
+        az2max = fmax(az2max, az[i] * az[i]);
         sincos(-(double(a.g_pr) * zz) - off_pr, &spr[i], &cpr[i]);
         // m = (0,0,1); invert(p): m_z *= cos(pi B1); decay + regrowth over dt_inv
         m[i] = {0.0, 0.0, cos(M_PI * B1)};
@@ -98,19 +100,20 @@ __global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat*
                 __syncthreads();
             }
             const int kn = min(RF_CHUNK, a.nRF - k0);
+            double2 g_next = grf[0];
             for (int k = 0; k < kn; k++) {
-                const double2 g = grf[k];
+                // the next sub-step's waveform sample is read from LDS a step ahead
+                const double2 g = g_next;
+                if (k + 1 < kn) g_next = grf[k + 1];
                 const double zre = tre * g.x - tim * g.y;
                 const double zim = tre * g.y + tim * g.x;
                 const double ax = zre * B1, ay = -zim * B1;
                 const double axy2 = fma(ax, ax, ay * ay);
                 double u[ZPL];
-                double umax = 0.0;
 #pragma unroll
-                for (int i = 0; i < ZPL; i++) {
-                    u[i] = fma(az[i], az[i], axy2);
-                    umax = fmax(umax, u[i]);
-                }
+                for (int i = 0; i < ZPL; i++) u[i] = fma(az[i], az[i], axy2);
+                // the lane's largest u: a_z is fixed per slice position
+                const double umax = axy2 + az2max;
                 // wave-uniform choice of the series: the common small-angle
                 // cases are straight-line code over the ZPL isochromats
                 if (!__any(umax > SERIES_LIMIT)) {
