@@ -703,7 +703,8 @@ def serve_wire_bench(args):
         return got
 
     # capacity: a closed-loop burst of a sample, through the same path
-    cal_ids = np.random.default_rng(6).choice(n_req, size=min(args.calibrate, n_req), replace=False).tolist()
+    cal_ids = np.random.default_rng(6).choice(n_req, size=min(calibration_size(args, n_req), n_req),
+                                              replace=False).tolist()
     cal = run(cal_ids, None)
     cal_span = max(t for _, t, _ in cal.values()) - min(s for s, _, _ in cal.values())
     capacity = len(cal_ids) / cal_span
@@ -753,6 +754,12 @@ def serve_wire_bench(args):
     }))
 
 
+def calibration_size(args, n_req):
+    if args.calibrate is not None:
+        return args.calibrate
+    return max(60, min(600, n_req // 10))
+
+
 def progress(msg):
     """A progress line on stderr (long serving runs would otherwise be silent)."""
     print(f"[serve] {msg}", file=sys.stderr, flush=True)
@@ -782,8 +789,9 @@ def serve_bench(args):
     budget = int(args.budget_gib * (1 << 30))
 
     # capacity: a closed-loop burst of a seeded sample of the requests (also warms up the
-    # autotuner's shapes and the slot pools)
-    cal_ids = np.random.default_rng(6).choice(n_req, size=min(args.calibrate, n_req), replace=False).tolist()
+    # slot pools); request costs are heavy-tailed, so the sample is 10% of the run
+    cal_ids = np.random.default_rng(6).choice(n_req, size=min(calibration_size(args, n_req), n_req),
+                                              replace=False).tolist()
     server = ca.Server(slots=args.slots, memory_budget=budget)
     tickets = [serve_submit(server, reqs[i]) for i in cal_ids]
     times = []
@@ -1009,7 +1017,8 @@ def main():
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--load", type=float, default=0.7)
-    ap.add_argument("--calibrate", type=int, default=60)
+    ap.add_argument("--calibrate", type=int, default=None,
+                    help="requests in the capacity burst (default: 10%% of --requests, 60 to 600)")
     ap.add_argument("--check", type=int, default=100)
     ap.add_argument("--budget-gib", type=float, default=96.0)
     ap.add_argument("--wire", action="store_true", help="with --serve: requests as wire frames over TCP")
