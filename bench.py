@@ -757,8 +757,9 @@ def serve_wire_bench(args):
 def reconstruct_bench(args):
     """Gauss-Newton fits on the HIP path (compas_amd.reconstruct): noise-free
     k-space data of a seeded phantom, a perturbed start, a fixed number of
-    outer iterations; each run twice, with the normal operator J^H(J v)
-    replayed from a captured HIP graph and called eagerly.
+    outer iterations; each run three ways: the normal operator J^H(J v)
+    replayed from a captured HIP graph with CG on the device, the same with
+    CG on the host, and the operator called eagerly (CG on the host).
       cartesian: config 1 (4096 voxels, FISP 256 TRs, 16 states, Cartesian
                  256 x 64, 1 coil), T1, T2, rho; 12 iterations, CG <= 100
       spiral:    config 3 at 1/4 scale (3136 voxels, pSSFP 280 TRs x 35
@@ -785,12 +786,17 @@ def reconstruct_bench(args):
         if fit_b1:
             start.B1 = (t.B1 + 0.04).astype(np.float32)
         runs = {}
-        for use_graph in (True, False, True):  # the first graph run also warms up
+        # the first run warms up (autotuning, torch's allocator) and is not kept
+        for label, use_graph, cg_dev in (("warmup", True, True), ("graph_device_cg", True, True),
+                                         ("graph_host_cg", True, False), ("eager", False, False)):
+            progress(f"reconstruct {name}: {label}")
             t0 = time.perf_counter()
             res = R.gauss_newton(y, start, seq, traj, wl.coils, iterations=iters, cg_iterations=100, cg_tol=1e-9,
-                                 fit_b1=fit_b1, use_graph=use_graph)
+                                 fit_b1=fit_b1, use_graph=use_graph, cg_on_device=cg_dev)
             wall = time.perf_counter() - t0
-            runs["graph" if use_graph else "eager"] = {
+            if label == "warmup":
+                continue
+            runs[label] = {
                 "seconds": wall, "iterations": len(res.cg_iterations), "ms_per_iteration": 1e3 * wall / max(1, len(res.cg_iterations)),
                 "normal_calls": res.normal_calls, "normal_ms_per_call": 1e3 * res.normal_seconds / max(1, res.normal_calls),
                 "objective_first": res.objective[0], "objective_last": res.objective[-1],
@@ -798,9 +804,10 @@ def reconstruct_bench(args):
                 **{f"{f}_rel_err_p{q}": float(np.percentile(np.abs(getattr(res.estimate, f) / getattr(t, f) - 1), q))
                    for f in ("T1", "T2") for q in (50, 95, 100)}}
         out[name] = {"voxels": t.nvoxels, "fit": "T1 T2 B1 rho" if fit_b1 else "T1 T2 rho", **runs}
-    c = out["cartesian"]["graph"]
+    c = out["cartesian"]["graph_device_cg"]
     print(json.dumps({
-        "metric": "Gauss-Newton iterations/s (config-1 Cartesian FISP fit, normal operator from a captured graph)",
+        "metric": "Gauss-Newton iterations/s (config-1 Cartesian FISP fit, normal operator from a captured graph, "
+                  "CG on the device)",
         "value": c["iterations"] / c["seconds"],
         "unit": "iterations/s",
         "n_gpus": 1,

@@ -216,6 +216,25 @@ class Context:
         lib().compas_synchronize(self.ptr)
         _check()
 
+    @property
+    def stream(self) -> int:
+        """The HIP stream the library's calls on this context run on."""
+        s = lib().compas_context_stream(self.ptr)
+        _check()
+        return int(s or 0)
+
+    def wait_for_stream(self, stream: int) -> None:
+        """Library calls queued after this wait for the work queued so far on
+        `stream` (an event; the host does not block)."""
+        lib().compas_context_wait_stream(self.ptr, C.c_void_p(int(stream)))
+        _check()
+
+    def make_stream_wait(self, stream: int) -> None:
+        """Work queued on `stream` after this waits for the library calls
+        queued so far on this context."""
+        lib().compas_stream_wait_context(self.ptr, C.c_void_p(int(stream)))
+        _check()
+
     def set_kernel_timing(self, enabled: bool):
         lib().compas_set_kernel_timing(self.ptr, int(enabled))
         _check()

@@ -23,12 +23,16 @@ data the six-parameter ones (T1, T2, B1, B0, rho_x, rho_y), where B1 and B0
 may be held fixed.  With phase_encode (Cartesian), echoes and derivatives pass
 through phase_encoding before the signal, as the reference's examples do.
 
-Vectors of the normal equations ([rows][nvoxels] real) live on the host; every
-product runs on the GPU.  With use_graph (default) the normal operator
+Every product runs on the GPU.  With use_graph (default) the normal operator
 J^H(J v) is captured as a HIP graph after each linearisation (one eager
 product first, for autotuning) and each CG step replays it with v written
 into the graph's input array: the same kernels and results, without the
-per-launch host overhead that dominates small problems.
+per-launch host overhead.  The CG vectors ([rows][nvoxels] real) then live
+on the device as torch tensors that share memory with the graph's input and
+output arrays (DLPack, no copies); torch's stream and the library's are
+ordered by events, and the CG scalars stay on the device (convergence is
+checked every few steps), so a CG step issues no host round trip.  Without
+the graph, or without torch, the vectors live on the host (numpy).
 """
 
 from __future__ import annotations
@@ -73,8 +77,9 @@ class GaussNewtonResult:
     objective: List[float] = field(default_factory=list)  # before each iteration, then the final value
     cg_iterations: List[int] = field(default_factory=list)
     steps: List[float] = field(default_factory=list)
-    normal_seconds: float = 0.0  # wall time in the normal operator J^H(J v) (host view)
+    normal_seconds: float = 0.0  # wall time in the normal operator J^H(J v) (host view; the whole CG on the device)
     normal_calls: int = 0
+    cg_on_device: bool = False
 
 
 class _Model:
@@ -161,10 +166,38 @@ class _Model:
         self.normal_calls += 1
         return out
 
+    # ---- device-resident normal operator (torch views of the graph's arrays)
+
+    def device_views(self, torch):
+        """(input, output) torch views of the captured graph's arrays."""
+        if getattr(self, "_views_of", None) is not self.graph:
+            self.v_t = self.v_in.to_torch()
+            self.out_t = self.normal_out.to_torch()
+            self._views_of = self.graph
+            mask = np.ones((self.rows, 1))
+            if self.spiral:
+                mask[2] = 0.0 if not self.fit_b1 else 1.0
+                mask[3] = 0.0 if not self.fit_b0 else 1.0
+            self.mask_t = torch.tensor(mask, dtype=torch.float64, device=self.v_t.device)
+        return self.v_t, self.out_t
+
+    def normal_device(self, v, torch):
+        """Re J^H J v (masked) for a device vector v: [rows][nv] float64."""
+        v_t, out_t = self.device_views(torch)
+        stream = torch.cuda.current_stream(v.device).cuda_stream
+        v_t.copy_(v)  # real -> complex, on torch's stream
+        self.v_in.context.wait_for_stream(stream)
+        self.graph.launch()
+        self.v_in.context.make_stream_wait(stream)
+        self.normal_calls += 1
+        return out_t.real.to(torch.float64) * self.mask_t
+
     def close(self):
         if self.graph is not None:
             self.graph.close()
             self.graph = None
+        self.v_t = self.out_t = None
+        self._views_of = None
 
 
 def _cg(apply, b, iterations, tol, shift_rel=0.0):
@@ -198,6 +231,47 @@ def _cg(apply, b, iterations, tol, shift_rel=0.0):
     return x, k
 
 
+def _cg_device(apply, b, iterations, tol, shift_rel, torch, check_every=8):
+    """_cg with the vectors and scalars on the device: no host round trip per
+    step; the residual test (and a non-positive curvature stop) every
+    `check_every` steps."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    p = r.clone()
+    rr = torch.sum(r * r)
+    b_norm2 = float(rr)
+    if b_norm2 == 0.0:
+        return x, 0
+    shift = None
+    k = 0
+    for k in range(1, iterations + 1):
+        Ap = apply(p)
+        if shift is None:
+            shift = shift_rel * torch.sum(p * Ap) / rr
+        Ap = Ap + shift * p
+        pAp = torch.sum(p * Ap)
+        ok = pAp > 0
+        alpha = torch.where(ok, rr / torch.where(ok, pAp, torch.ones_like(pAp)), torch.zeros_like(pAp))
+        x += alpha * p
+        r -= alpha * Ap
+        rr_new = torch.sum(r * r)
+        if k % check_every == 0 or k == iterations:
+            if not bool(ok) or float(rr_new) <= tol * tol * b_norm2:
+                break
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+    return x, k
+
+
+def _torch_gpu():
+    """torch with a visible GPU, or None."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    return torch if torch.cuda.is_available() else None
+
+
 def _apply_step(est: Estimate, delta: np.ndarray, step: float, spiral: bool, max_log_step: float) -> Estimate:
     out = est.copy()
     d = step * delta
@@ -217,11 +291,16 @@ def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, itera
                  cg_iterations: int = 100, cg_tol: float = 1e-6, damping: float = 1e-6,
                  method: str = "gemm_fast", signal_method: str = "matmul_fast", phase_encode: bool = True,
                  fit_b1: bool = False, fit_b0: bool = False, max_log_step: float = 0.7,
-                 min_step: float = 1.0 / 64, use_graph: bool = True) -> GaussNewtonResult:
+                 min_step: float = 1.0 / 64, use_graph: bool = True,
+                 cg_on_device: Optional[bool] = None) -> GaussNewtonResult:
     """Minimise ||S(p) - measured||^2 over the tissue parameters, starting from
     `initial`.  `measured` is [ncoils][nreadouts][samples] complex64 (host or
     device array).  `damping` is the Levenberg shift relative to the normal
-    operator's Rayleigh quotient along the gradient."""
+    operator's Rayleigh quotient along the gradient.  cg_on_device (default:
+    with use_graph, when torch sees the GPU) keeps the CG on the device."""
+    torch = _torch_gpu() if use_graph and cg_on_device is not False else None
+    if cg_on_device and torch is None:
+        raise RuntimeError("cg_on_device needs use_graph and torch with a GPU")
     model = _Model(sequence, trajectory, coils, method, signal_method, phase_encode, fit_b1, fit_b0, use_graph)
     y = as_array(measured, np.complex64)
     est = initial.copy()
@@ -232,7 +311,14 @@ def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, itera
         result.objective.append(obj)
         model.linearise(est, params, m)
         g = model.jhy(r)
-        delta, k = _cg(model.normal, -g, cg_iterations, cg_tol, damping)
+        t0 = time.perf_counter()
+        if torch is not None:
+            b = torch.from_numpy(-g).to(device="cuda")
+            delta_t, k = _cg_device(lambda v: model.normal_device(v, torch), b, cg_iterations, cg_tol, damping, torch)
+            delta = delta_t.cpu().numpy()
+            model.normal_seconds += time.perf_counter() - t0
+        else:
+            delta, k = _cg(model.normal, -g, cg_iterations, cg_tol, damping)
         result.cg_iterations.append(k)
         step = 1.0
         while True:
@@ -252,4 +338,5 @@ def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, itera
     result.objective.append(obj)
     result.normal_seconds = model.normal_seconds
     result.normal_calls = model.normal_calls
+    result.cg_on_device = torch is not None
     return result

@@ -121,7 +121,7 @@ def test_gauss_newton_graph_replay_is_identical():
     m = ca.phase_encoding(ca.simulate_magnetization(params, seq), params, traj)
     y = ca.magnetization_to_signal(m, params, traj, coils).collect()
     start = estimate_of(t, T1=(t.T1 * 1.2).astype(np.float32), T2=(t.T2 * 0.85).astype(np.float32))
-    a = R.gauss_newton(y, start, seq, traj, coils, iterations=4, use_graph=True)
+    a = R.gauss_newton(y, start, seq, traj, coils, iterations=4, use_graph=True, cg_on_device=False)
     b = R.gauss_newton(y, start, seq, traj, coils, iterations=4, use_graph=False)
     print("normal operator ms per call: graph %.3f, eager %.3f" % (
         1e3 * a.normal_seconds / a.normal_calls, 1e3 * b.normal_seconds / b.normal_calls), flush=True)
@@ -130,3 +130,38 @@ def test_gauss_newton_graph_replay_is_identical():
     assert np.array_equal(a.estimate.T2, b.estimate.T2)
     assert np.array_equal(a.estimate.rho, b.estimate.rho)
 
+
+
+def test_gauss_newton_device_cg_matches_host_cg():
+    # CG with its vectors on the device (torch views of the graph's arrays,
+    # event-ordered streams) takes the same steps as CG on the host up to
+    # rounding and its every-8-steps convergence test: the same fixed point
+    banner("gauss-newton device CG vs host CG")
+    context()
+    torch = pytest.importorskip("torch")
+    assert torch.cuda.is_available()
+    n, fov = 8, 8.0
+    t = phantom(n, fov, seed=15)
+    seq = ca.FispSequence.from_sequence(W.make_fisp(96, 60.0, 12))
+    traj = ca.CartesianTrajectory.from_trajectory(W.make_cartesian(n, 12, n, fov, 1e-5))
+    coils = W.make_coils(n, fov, 2)
+    params = estimate_of(t).parameters()
+    m = ca.phase_encoding(ca.simulate_magnetization(params, seq), params, traj)
+    y = ca.magnetization_to_signal(m, params, traj, coils, method="direct").collect()
+    start = estimate_of(t, T1=(t.T1 * 1.2).astype(np.float32), T2=(t.T2 * 0.85).astype(np.float32),
+                        rho=(t.rho * 0.9).astype(np.complex64))
+    kw = dict(iterations=8, cg_iterations=200, cg_tol=1e-7, method="direct", signal_method="direct")
+    host = R.gauss_newton(y, start, seq, traj, coils, cg_on_device=False, **kw)
+    dev = R.gauss_newton(y, start, seq, traj, coils, cg_on_device=True, **kw)
+    print("objective host", host.objective, "device", dev.objective, "cg", host.cg_iterations, dev.cg_iterations,
+          flush=True)
+    assert dev.cg_on_device and not host.cg_on_device
+    assert dev.objective[-1] < 1e-8 * dev.objective[0], dev.objective
+    # the first CG solve: same system, same start; the device run may take up
+    # to 7 more steps before its convergence test, so its step is as close to
+    # the solution as the host's or closer
+    assert abs(dev.objective[1] - host.objective[1]) <= 1e-3 * host.objective[0], (host.objective, dev.objective)
+    for f in ("T1", "T2"):
+        a, b = getattr(dev.estimate, f), getattr(host.estimate, f)
+        assert np.max(np.abs(a / b - 1)) < 1e-4, f
+    assert np.max(np.abs(dev.estimate.rho - host.estimate.rho)) < 1e-4 * np.max(np.abs(t.rho))
