@@ -760,12 +760,13 @@ def reconstruct_bench(args):
     outer iterations; each run three ways: the normal operator J^H(J v)
     replayed from a captured HIP graph with CG on the device, the same with
     CG on the host, and the operator called eagerly (CG on the host).
-      cartesian: config 1 (4096 voxels, FISP 256 TRs, 16 states, Cartesian
-                 256 x 64, 1 coil), T1, T2, rho; 12 iterations, CG <= 100
-      spiral:    config 3 at 1/4 scale (3136 voxels, pSSFP 280 TRs x 35
+      cartesian: 4096 voxels, FISP 256 TRs, 16 states, Cartesian 256 x 64,
+                 1 coil; T1, T2, rho; 12 iterations, CG <= 100
+      spiral:    the pSSFP spiral workload at 1/4 scale (3136 voxels, 280 TRs x 35
                  isochromats, spiral 280 x 128, 4 coils), T1, T2, B1, rho;
                  8 iterations, CG <= 100
-    value = Gauss-Newton iterations/s of the cartesian fit with graphs."""
+    value = Gauss-Newton iterations/s of the cartesian fit with graphs and
+    CG on the device."""
     from compas_amd import reconstruct as R
 
     ca.init_context(0)
@@ -806,7 +807,7 @@ def reconstruct_bench(args):
         out[name] = {"voxels": t.nvoxels, "fit": "T1 T2 B1 rho" if fit_b1 else "T1 T2 rho", **runs}
     c = out["cartesian"]["graph_device_cg"]
     print(json.dumps({
-        "metric": "Gauss-Newton iterations/s (config-1 Cartesian FISP fit, normal operator from a captured graph, "
+        "metric": "Gauss-Newton iterations/s (Cartesian FISP fit, 4096 voxels x 256 TRs, normal operator from a captured graph, "
                   "CG on the device)",
         "value": c["iterations"] / c["seconds"],
         "unit": "iterations/s",
@@ -814,8 +815,8 @@ def reconstruct_bench(args):
         "higher_is_better": True,
         "dtype": "f32 (products BF16 hi+lo on MFMA; derivatives by the fused tangent passes)",
         "data": "synthetic (seeded phantoms, noise-free k-space data, perturbed starts)",
-        "config": {"workload": "Gauss-Newton: config 1 Cartesian FISP (12 iterations, CG <= 100) and config 3 at "
-                               "1/4 scale spiral pSSFP with B1 (8 iterations, CG <= 100)"},
+        "config": {"workload": "Gauss-Newton: Cartesian FISP 4096 voxels x 256 TRs (12 iterations, CG <= 100) and spiral pSSFP "
+                               "3136 voxels x 280 TRs x 35 isochromats with B1 (8 iterations, CG <= 100)"},
         "fits": out,
     }))
 
