@@ -243,6 +243,7 @@ def _cg_device(apply, b, iterations, tol, shift_rel, torch, check_every=8):
     if b_norm2 == 0.0:
         return x, 0
     shift = None
+    live = torch.ones((), dtype=torch.bool, device=b.device)  # false from the first pAp <= 0 on
     k = 0
     for k in range(1, iterations + 1):
         Ap = apply(p)
@@ -250,13 +251,13 @@ def _cg_device(apply, b, iterations, tol, shift_rel, torch, check_every=8):
             shift = shift_rel * torch.sum(p * Ap) / rr
         Ap = Ap + shift * p
         pAp = torch.sum(p * Ap)
-        ok = pAp > 0
-        alpha = torch.where(ok, rr / torch.where(ok, pAp, torch.ones_like(pAp)), torch.zeros_like(pAp))
+        live = live & (pAp > 0)
+        alpha = torch.where(live, rr / torch.where(live, pAp, torch.ones_like(pAp)), torch.zeros_like(pAp))
         x += alpha * p
         r -= alpha * Ap
         rr_new = torch.sum(r * r)
         if k % check_every == 0 or k == iterations:
-            if not bool(ok) or float(rr_new) <= tol * tol * b_norm2:
+            if not bool(live) or float(rr_new) <= tol * tol * b_norm2:
                 break
         p = r + (rr_new / rr) * p
         rr = rr_new
