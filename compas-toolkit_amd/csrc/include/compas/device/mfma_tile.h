@@ -91,6 +91,18 @@ __device__ inline void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) 
     }
 }
 
+// split_pair of a complex value scaled by s; the per-row power-of-two scales
+// exist only for the FP16 precisions (BF16 has FP32's exponent range), so
+// the BF16 forms skip the two multiplies
+template<int P>
+__device__ inline void split_scaled(cfloat v, float s, uint32_t& hi, uint32_t& lo) {
+    if constexpr (is_fp16(P)) {
+        split_pair<P>(v.re * s, v.im * s, hi, lo);
+    } else {
+        split_pair<P>(v.re, v.im, hi, lo);
+    }
+}
+
 template<int P>
 __device__ inline f32x16 mfma(typename Half16<P>::V a, typename Half16<P>::V b, f32x16 c) {
     if constexpr (is_fp16(P)) {
@@ -235,7 +247,7 @@ __device__ inline void mfma_accumulate(
         {
             uint32_t h[Tl::A_PER], l[Tl::A_PER];
 #pragma unroll
-            for (int i = 0; i < Tl::A_PER; i++) split_pair<P>(ra[i].re * sa, ra[i].im * sa, h[i], l[i]);
+            for (int i = 0; i < Tl::A_PER; i++) split_scaled<P>(ra[i], sa, h[i], l[i]);
             store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_hi + a_row * LD + 2 * a_k), h);
             if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_lo + a_row * LD + 2 * a_k), l);
         }
@@ -243,7 +255,7 @@ __device__ inline void mfma_accumulate(
         {
             uint32_t h[Tl::B_PER], l[Tl::B_PER];
 #pragma unroll
-            for (int i = 0; i < Tl::B_PER; i++) split_pair<P>(rb[i].re * sb, rb[i].im * sb, h[i], l[i]);
+            for (int i = 0; i < Tl::B_PER; i++) split_scaled<P>(rb[i], sb, h[i], l[i]);
             store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_hi + b_row * LD + 2 * b_k), h);
             if constexpr (LO) store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_lo + b_row * LD + 2 * b_k), l);
         }
@@ -341,12 +353,12 @@ __device__ inline void mfma_accumulate_db(
     auto stage = [&](MfmaSmem<P, WM, WN, BKC, TM, TN>& buf) {
         uint32_t h[Tl::A_PER], l[Tl::A_PER];
 #pragma unroll
-        for (int i = 0; i < Tl::A_PER; i++) split_pair<P>(ra[i].re * sa, ra[i].im * sa, h[i], l[i]);
+        for (int i = 0; i < Tl::A_PER; i++) split_scaled<P>(ra[i], sa, h[i], l[i]);
         store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(buf.a_hi + a_row * LD + 2 * a_k), h);
         if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(buf.a_lo + a_row * LD + 2 * a_k), l);
         uint32_t hb[Tl::B_PER], lb[Tl::B_PER];
 #pragma unroll
-        for (int i = 0; i < Tl::B_PER; i++) split_pair<P>(rb[i].re * sb, rb[i].im * sb, hb[i], lb[i]);
+        for (int i = 0; i < Tl::B_PER; i++) split_scaled<P>(rb[i], sb, hb[i], lb[i]);
         store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_hi + b_row * LD + 2 * b_k), hb);
         if constexpr (LO) store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_lo + b_row * LD + 2 * b_k), lb);
     };
