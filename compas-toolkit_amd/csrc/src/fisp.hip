@@ -75,6 +75,16 @@ __global__ void __launch_bounds__(64) fisp_epg(FispArgs a) {
     const bool owns_zero = (sub == 0);  // lane holding state k = 0
     const int k0 = sub * SPL;           // global index of local state 0
 
+    // F+_0 at TE from the updated state: over TR - TE the transverse state
+    // only decays and precesses (F+ times d_TRmTE), so F+_0(TE) = F+_0(TR) /
+    // d_TRmTE, one complex multiply per TR instead of a second 2x3 operator.
+    // A wave with a lane whose d_TRmTE is too small to invert (T2 of a few
+    // microseconds) applies the operator D(TE) R to state 0 instead.
+    const float dd = d_TRmTE.re * d_TRmTE.re + d_TRmTE.im * d_TRmTE.im;
+    const bool direct_echo = __any(!(dd > 1e-30f));  // wave-uniform
+    const cfloat inv_TRmTE = direct_echo ? cfloat {0.0f, 0.0f}
+                                         : cfloat {d_TRmTE.re / dd, -d_TRmTE.im / dd};
+
     for (int slice = 0; slice < a.nslices; slice++) {
         const float4* tab = a.table + size_t(slice) * size_t(a.nTR);
         cfloat P[SPL], Q[SPL], Z[SPL];
@@ -122,7 +132,7 @@ __global__ void __launch_bounds__(64) fisp_epg(FispArgs a) {
                 const float h20 = 0.5f * M20, h21 = 0.5f * M21;
 
                 // sample F+_0 at TE:  rows 0/1 of D(TE) R applied to state 0
-                if (sample && owns_zero) {
+                if (direct_echo && sample && owns_zero) {
                     const float er = d_TE.re, ei = d_TE.im;
                     const float T00 = er * R00 - ei * R10, T01 = er * R01 - ei * R11, T02 = er * R02 - ei * R12;
                     const float T10 = ei * R00 + er * R10, T11 = ei * R01 + er * R11, T12 = ei * R02 + er * R12;
@@ -155,6 +165,13 @@ __global__ void __launch_bounds__(64) fisp_epg(FispArgs a) {
                     Z[i] = {zr, zi};
                 }
                 if (owns_zero) Z[0].re += G;
+                if (!direct_echo && sample && owns_zero) {
+                    const cfloat echo = dev::cmul(Pn[0], inv_TRmTE);
+                    for (int u = 0; u < a.usf; u++) {
+                        cfloat* out = a.echos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
+                        *out = (slice == 0) ? echo : (*out + echo);
+                    }
+                }
 
                 // dephasing: F+_k <- F+_{k-1}, F-_k <- F-_{k+1}, F-_{N-1} <- 0, F+_0 <- conj(F-_0)
                 cfloat p_in = {0, 0}, q_in = {0, 0};

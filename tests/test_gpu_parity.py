@@ -84,6 +84,25 @@ def test_fisp_b0_zero_branch(ctx, oracle):
     assert rel(run_fisp(t, seq), oracle.fisp(t, seq, "f64")) < MAG_TOL
 
 
+def test_fisp_echo_paths(ctx, oracle):
+    # the kernel samples F+_0 at TE by undoing the decay over TR - TE on the
+    # updated state; a wave holding a voxel whose decay over TR - TE is too
+    # small to invert applies D(TE) R to state 0 instead.  TE = 0.2 ms and
+    # T2 = 0.15 ms: the echo keeps e^(-TE/T2) = 0.26 of the transverse state
+    # while e^(-(TR-TE)/T2) underflows FP32
+    banner("fisp echo paths (short T2)")
+    t, seq = small_fisp(nv=300, nTR=60, N=16)
+    seq.TE = 0.0002
+    t.T2[:64] = 1.5e-4  # the first wave: every lane short
+    t.T2[100] = 1.5e-4  # the second wave: one lane short
+    got = run_fisp(t, seq)
+    want = oracle.fisp(t, seq, "f64")
+    assert rel(got, want) < MAG_TOL
+    for cols in (slice(0, 64), slice(64, 128), slice(128, 300)):
+        assert rel(got[:, cols], want[:, cols]) < MAG_TOL, cols
+    assert np.linalg.norm(want[:, :64]) > 0.05 * np.linalg.norm(want[:, 128:])  # the short-T2 echoes are not ~0
+
+
 @pytest.mark.parametrize("index,scale", [(1, 1), (2, 4)])
 def test_fisp_workloads(ctx, oracle, index, scale):
     banner(f"fisp workload {index} scale {scale}")
