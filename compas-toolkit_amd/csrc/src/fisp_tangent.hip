@@ -134,6 +134,17 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
     const bool owns_zero = (sub == 0);
     const int k0 = sub * SPL;
 
+    // F+_0 at TE from the updated state, as in the primal kernel: F+_0(TE) =
+    // F+_0(TR) / d_TRmTE, and for the tangents d/dp [F+_0(TR) / d_TRmTE] with
+    // d_TRmTE depending on T2 (factor (TR - TE)/T2^2) and B0 (i 2 pi (TR - TE)).
+    // A wave with a lane whose d_TRmTE cannot be inverted samples with the
+    // operator D(TE) R as before.
+    const float dd = d_TRmTE.re * d_TRmTE.re + d_TRmTE.im * d_TRmTE.im;
+    const bool direct_echo = __any(!(dd > 1e-30f));  // wave-uniform
+    const cfloat inv_TRmTE = direct_echo ? cfloat {0.0f, 0.0f}
+                                         : cfloat {d_TRmTE.re / dd, -d_TRmTE.im / dd};
+    const float k2TRmTE = (a.TR - a.TE) * iT2s, k0TRmTE = dev::TWO_PI_F * (a.TR - a.TE);
+
     for (int slice = 0; slice < a.nslices; slice++) {
         const float4* tab = a.table + size_t(slice) * size_t(a.nTR);
         cfloat P[SPL], Q[SPL], Z[SPL], dP[NT][SPL], dQ[NT][SPL], dZ[NT][SPL];
@@ -185,10 +196,10 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
                     const float dR10 = dR01, dR11 = -sa * cp * cp * t.x, dR12 = -ca * cp * t.x;
                     const float dR20 = -ca * sp * t.x, dR21 = ca * cp * t.x, dR22 = -sa * t.x;
                     N = decay_rows(d_TR, E1_TR, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
-                    Sb = decay_rows(d_TE, 0.0f, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
+                    if (direct_echo) Sb = decay_rows(d_TE, 0.0f, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
                 }
 
-                if (sample && owns_zero) {
+                if (direct_echo && sample && owns_zero) {
                     const Rows T = decay_rows(d_TE, 0.0f, R00, R01, R02, R10, R11, R12, R20, R21, R22);
                     const cfloat S0 = P[0] + Q[0], D0 = P[0] - Q[0];
                     const cfloat echo = sample_rows(T, S0, D0, Z[0]);
@@ -244,6 +255,23 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
                 if (owns_zero) {
                     Z[0].re += G;
                     if constexpr (TS::t1) dZ[TS::it1][0].re += dG;
+                }
+                if (!direct_echo && sample && owns_zero) {
+                    const cfloat echo = dev::cmul(Pn[0], inv_TRmTE);
+                    cfloat v[NP];
+                    if constexpr (TS::echo) v[0] = echo;
+#pragma unroll
+                    for (int k = 0; k < NT; k++) {
+                        cfloat dv = dev::cmul(dPn[k][0], inv_TRmTE);
+                        if (TS::t2 && k == TS::it2) dv = dv - echo * k2TRmTE;
+                        if (TS::b0 && k == TS::ib0) dv = dv - times_i(echo) * k0TRmTE;
+                        v[OFS + k] = dv;
+                    }
+                    for (int u = 0; u < a.usf; u++) {
+                        const size_t at = size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
+#pragma unroll
+                        for (int k = 0; k < NP; k++) out.p[k][at] = (slice == 0) ? v[k] : (out.p[k][at] + v[k]);
+                    }
                 }
 
                 // dephasing: F+ moves up one state, F- down one, primal and tangents alike

@@ -543,6 +543,27 @@ def test_fisp_tangent_sequence_options(ctx, oracle, field):
         assert rel(got, central_difference(oracle.fisp, t, seq, name)) < TANGENT_TOL
 
 
+@pytest.mark.parametrize("field", [ca.T1, ca.T2, ca.B1, ca.B0])
+def test_fisp_tangent_echo_paths(ctx, oracle, field):
+    # as test_fisp_echo_paths for the tangents: TE = 0.2 ms, T2 = 0.15 ms in
+    # voxels 0..63 (whole waves, whatever the lanes per voxel) and in voxel
+    # 100 (one lane of a wave whose other voxels sample from the updated state)
+    name = ["T1", "T2", "B1", "B0"][field]
+    banner(f"fisp tangent {name} echo paths (short T2)")
+    t, seq = small_fisp(nv=150, nTR=40, N=12)
+    seq.TE = 0.0002
+    t.T2[:64] = 1.5e-4
+    t.T2[100] = 1.5e-4
+    context()
+    p = ca.TissueParameters.from_tissue(t)
+    s = ca.FispSequence.from_sequence(seq)
+    got = ca.simulate_magnetization_tangent(field, p, s).collect()
+    want = central_difference(oracle.fisp, t, seq, name)
+    assert rel(got, want) < TANGENT_TOL
+    for cols in (slice(0, 64), slice(64, 150)):
+        assert rel(got[:, cols], want[:, cols]) < TANGENT_TOL, cols
+
+
 def test_fisp_tangent_b0_zero_and_errors(ctx, oracle):
     t, seq = small_fisp(nv=50, nTR=30, N=10, b0=None)
     context()
