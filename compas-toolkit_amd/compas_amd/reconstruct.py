@@ -102,6 +102,8 @@ class _Model:
         self.rows = 6 if self.spiral else 4
         self.use_graph = use_graph
         self.graph = None
+        self.v_t = self.out_t = self.mask_t = None  # torch views of the graph's arrays (device CG)
+        self._views_of = None
         self.normal_seconds = 0.0
         self.normal_calls = 0
 
@@ -170,7 +172,7 @@ class _Model:
 
     def device_views(self, torch):
         """(input, output) torch views of the captured graph's arrays."""
-        if getattr(self, "_views_of", None) is not self.graph:
+        if self._views_of is not self.graph:
             self.v_t = self.v_in.to_torch()
             self.out_t = self.normal_out.to_torch()
             self._views_of = self.graph
@@ -314,7 +316,7 @@ def gauss_newton(measured, initial: Estimate, sequence, trajectory, coils, itera
         g = model.jhy(r)
         t0 = time.perf_counter()
         if torch is not None:
-            b = torch.from_numpy(-g).to(device="cuda")
+            b = torch.from_numpy(-g).to(device=model.device_views(torch)[0].device)  # the library's GPU
             delta_t, k = _cg_device(lambda v: model.normal_device(v, torch), b, cg_iterations, cg_tol, damping, torch)
             delta = delta_t.cpu().numpy()
             model.normal_seconds += time.perf_counter() - t0
