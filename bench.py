@@ -79,75 +79,99 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
 
-def cpu_baseline(wl, gpu_echos, gpu_signal, seconds_budget=20.0):
-    """FP32 port of the reference algorithm (oracle/, test infrastructure) timed on
-    the host cores on a bounded sample, plus the accuracy of the GPU outputs
-    against the FP64 oracle on the same sample."""
+def median_time(fn, reps=5):
+    """(last result, median seconds, all seconds) of `reps` timed calls."""
+    ts = []
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return out, float(np.median(ts)), ts
+
+
+def oracle_for_timing():
+    """(pyoracle, native, threads): the FP32 port is timed with the oracle built
+    for this host (-march=native) when that builds, else the portable build."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     pyoracle.build()
     threads = int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    return pyoracle, pyoracle.native_lib() is not None, threads
+
+
+def cpu_baseline(wl, gpu_echos, gpu_signal, seconds_budget=20.0, other_signals=None):
+    """FP32 port of the reference algorithm (oracle/, test infrastructure) timed on
+    the host cores on a bounded sample (one calibration run as warm-up, then the
+    median of 5 runs), plus the accuracy of the GPU outputs against the FP64
+    oracle on the same sample."""
+    pyoracle, native, threads = oracle_for_timing()
     nv = wl.tissue.nvoxels
     nTR = int(wl.sequence.RF.shape[0])
     nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
     nc = wl.coils.shape[0]
 
     # simulation: contiguous voxel slice (voxels are independent), sized from a short
-    # calibration run to take about a third of the time budget
+    # calibration run so that 5 runs take about a third of the time budget
     ncal = max(64, nv // 64)
     t0 = time.perf_counter()
-    pyoracle.fisp(wl.tissue.slice(0, ncal), wl.sequence, "f32", threads)
+    pyoracle.fisp(wl.tissue.slice(0, ncal), wl.sequence, "f32", threads, native=native)
     t_cal = max(time.perf_counter() - t0, 1e-6)
-    nsim = int(min(nv, max(ncal, ncal * (seconds_budget / 3) / t_cal)))
+    nsim = int(min(nv, max(ncal, ncal * (seconds_budget / 15) / t_cal)))
     sub = wl.tissue.slice(0, nsim)
-    t0 = time.perf_counter()
-    e32 = pyoracle.fisp(sub, wl.sequence, "f32", threads)
-    t_sim = time.perf_counter() - t0
+    e32, t_sim, sim_runs = median_time(lambda: pyoracle.fisp(sub, wl.sequence, "f32", threads, native=native))
+    e32p = pyoracle.fisp(sub, wl.sequence, "f32", threads)  # portable build: the parity proxy
     e64 = pyoracle.fisp(sub, wl.sequence, "f64", threads)
     mag_err64 = pyoracle.rel_l2(gpu_echos[:, :nsim], e64)
-    mag_err32 = pyoracle.rel_l2(gpu_echos[:, :nsim], e32)
-    port_err = pyoracle.rel_l2(e32, e64)
+    mag_err32 = pyoracle.rel_l2(gpu_echos[:, :nsim], e32p)
+    port_err = pyoracle.rel_l2(e32p, e64)
 
     # signal: a subset of readouts (Cartesian rows are independent), full voxel sum,
-    # sized from a 2-readout calibration to take about two thirds of the budget
+    # sized from a 2-readout calibration so that 5 runs take about two thirds of the budget
     def rows_traj(ridx):
         return W.CartesianTraj(len(ridx), ns, wl.trajectory.delta_t, wl.trajectory.k_start[ridx],
                                wl.trajectory.delta_k)
 
     cal = np.array([0, nr - 1])
     t0 = time.perf_counter()
-    pyoracle.signal_cartesian(wl.tissue, rows_traj(cal), wl.coils, gpu_echos[cal].astype(np.complex64), "f32", threads)
+    pyoracle.signal_cartesian(wl.tissue, rows_traj(cal), wl.coils, gpu_echos[cal].astype(np.complex64), "f32",
+                              threads, native=native)
     t_cal = max(time.perf_counter() - t0, 1e-6)
-    rows = int(min(nr, max(2, 2 * (2 * seconds_budget / 3) / t_cal)))
+    rows = int(min(nr, max(2, 2 * (2 * seconds_budget / 15) / t_cal)))
     ridx = np.linspace(0, nr - 1, rows).astype(int)
     traj = rows_traj(ridx)
-    echos_rows = gpu_echos[ridx]
-    t0 = time.perf_counter()
-    s32 = pyoracle.signal_cartesian(wl.tissue, traj, wl.coils, echos_rows.astype(np.complex64), "f32", threads)
-    t_sig = time.perf_counter() - t0
+    echos_rows = gpu_echos[ridx].astype(np.complex64)
+    s32, t_sig, sig_runs = median_time(
+        lambda: pyoracle.signal_cartesian(wl.tissue, traj, wl.coils, echos_rows, "f32", threads, native=native))
     s64 = pyoracle.signal_cartesian(wl.tissue, traj, wl.coils, echos_rows.astype(np.complex128), "f64", threads)
     sig_err64 = pyoracle.rel_l2(gpu_signal[:, ridx, :], s64)
+    other_err = {f"signal_{k}_rel_l2_vs_fp64": pyoracle.rel_l2(v[:, ridx, :], s64)
+                 for k, v in (other_signals or {}).items()}
 
     sim_rate = nsim * nTR / t_sim  # voxel.TR/s
     sig_rate = nc * rows * ns / t_sig  # k-space samples/s
     t_step = nv * nTR / sim_rate + nc * nr * ns / sig_rate
+    build = "-O3 -march=native (built on this host)" if native else "-O3 -march=x86-64-v3 (native build failed)"
     base = {
         "value": nv * nTR / t_step,
         "unit": "voxel*TR/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"FP32 port of the reference algorithm: FISP on voxels [0, {nsim}) of {nv} "
-                   f"({t_sim:.2f} s), Cartesian signal on {rows} of {nr} readouts x all {nv} voxels "
-                   f"({t_sig:.2f} s); full-step time extrapolated linearly"),
+        "sample": (f"FP32 port of the reference algorithm, gcc {build}, OpenMP: FISP on voxels [0, {nsim}) of "
+                   f"{nv} (median {t_sim:.2f} s of 5 runs), Cartesian signal on {rows} of {nr} readouts x all "
+                   f"{nv} voxels (median {t_sig:.2f} s of 5 runs); step time = whole workload at these "
+                   f"size-independent rates (voxel-TR/s, samples/s)"),
         "sim_voxel_tr_per_s": sim_rate,
         "signal_samples_per_s": sig_rate,
+        "runs_s": {"sim": sim_runs, "signal": sig_runs},
     }
     accuracy = {
         "magnetization_rel_l2_vs_fp64": mag_err64,
         "magnetization_rel_l2_vs_fp32_port": mag_err32,
         "fp32_port_rel_l2_vs_fp64": port_err,
         "signal_rel_l2_vs_fp64": sig_err64,
+        **other_err,
         "sample": f"voxels [0, {nsim}) for magnetization; readouts {ridx.tolist()} for signal",
     }
     return base, accuracy
@@ -431,16 +455,15 @@ def spiral_bench(args):
 
 def spiral_cpu_baseline(wl, e, s, jv, jh, ds, seconds_budget=16.0):
     """FP32 port (oracle/) of the pSSFP + spiral step on bounded subsets, and the
-    error of the GPU outputs against the FP64 oracle on the same subsets."""
-    import pyoracle
-
-    pyoracle.build()
-    threads = int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    error of the GPU outputs against the FP64 oracle on the same subsets.  Each
+    leg is sized by a calibration run (the warm-up) and timed as the median of
+    5 runs; the FP32 port is the oracle built for this host when that builds."""
+    pyoracle, native, threads = oracle_for_timing()
     nv = wl.tissue.nvoxels
     nTR = int(wl.sequence.RF.shape[0])
     nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
     nc = wl.coils.shape[0]
-    budget = seconds_budget / 4
+    budget = seconds_budget / 4 / 5  # per run: four legs of five runs
 
     def calibrated(run, unit_max, first):
         t0 = time.perf_counter()
@@ -457,40 +480,38 @@ def spiral_cpu_baseline(wl, e, s, jv, jh, ds, seconds_budget=16.0):
         return W.SpiralTraj(len(ridx), ns, t.delta_t, t.k_start[ridx], t.delta_k[ridx])
 
     # simulation: contiguous voxel slice
-    nsim = calibrated(lambda n: pyoracle.pssfp(wl.tissue.slice(0, n), wl.sequence, "f32", threads), nv, 16)
-    t0 = time.perf_counter()
-    pyoracle.pssfp(wl.tissue.slice(0, nsim), wl.sequence, "f32", threads)
-    t_sim = time.perf_counter() - t0
+    nsim = calibrated(lambda n: pyoracle.pssfp(wl.tissue.slice(0, n), wl.sequence, "f32", threads, native=native),
+                      nv, 16)
+    _, t_sim, _ = median_time(lambda: pyoracle.pssfp(wl.tissue.slice(0, nsim), wl.sequence, "f32", threads,
+                                                     native=native))
     e64 = pyoracle.pssfp(wl.tissue.slice(0, nsim), wl.sequence, "f64", threads)
     # signal and J.v: readout subsets (rows are independent)
     def rows(n):
         return np.linspace(0, nr - 1, n).astype(int)
 
     nsig = calibrated(lambda n: pyoracle.signal_spiral(wl.tissue, traj_sub(rows(n)), wl.coils, e[rows(n)], "f32",
-                                                       threads), nr, 2)
+                                                       threads, native=native), nr, 2)
     ri = rows(nsig)
-    t0 = time.perf_counter()
-    pyoracle.signal_spiral(wl.tissue, traj_sub(ri), wl.coils, e[ri], "f32", threads)
-    t_sig = time.perf_counter() - t0
+    _, t_sig, _ = median_time(lambda: pyoracle.signal_spiral(wl.tissue, traj_sub(ri), wl.coils, e[ri], "f32",
+                                                             threads, native=native))
     s64 = pyoracle.signal_spiral(wl.tissue, traj_sub(ri), wl.coils, e[ri].astype(np.complex128), "f64", threads)
     v = wl.extra["v"]
     njv = max(2, nsig // 2)
     rj = rows(njv)
-    t0 = time.perf_counter()
-    pyoracle.jacobian_spiral(wl.tissue, traj_sub(rj), wl.coils, e[rj], *[d[rj] for d in ds], v, "f32", threads)
-    t_jv = time.perf_counter() - t0
+    _, t_jv, _ = median_time(lambda: pyoracle.jacobian_spiral(wl.tissue, traj_sub(rj), wl.coils, e[rj],
+                                                              *[d[rj] for d in ds], v, "f32", threads,
+                                                              native=native))
     jv64 = pyoracle.jacobian_spiral(wl.tissue, traj_sub(rj), wl.coils, e[rj], *[d[rj] for d in ds], v, "f64",
                                     threads)
     # J^H.y: voxel subset (outputs per voxel are independent)
     nvh = calibrated(lambda n: pyoracle.jacobian_hermitian_spiral(
         tissue_sub(np.arange(n)), wl.trajectory, wl.coils[:, :n], e[:, :n], *[d[:, :n] for d in ds], wl.extra["y"],
-        "f32", threads), nv, 8)
+        "f32", threads, native=native), nv, 8)
     vi = np.linspace(0, nv - 1, nvh).astype(int)
     ts = tissue_sub(vi)
-    t0 = time.perf_counter()
-    pyoracle.jacobian_hermitian_spiral(ts, wl.trajectory, wl.coils[:, vi], e[:, vi], *[d[:, vi] for d in ds],
-                                       wl.extra["y"], "f32", threads)
-    t_jh = time.perf_counter() - t0
+    _, t_jh, _ = median_time(lambda: pyoracle.jacobian_hermitian_spiral(
+        ts, wl.trajectory, wl.coils[:, vi], e[:, vi], *[d[:, vi] for d in ds], wl.extra["y"], "f32", threads,
+        native=native))
     jh64 = pyoracle.jacobian_hermitian_spiral(ts, wl.trajectory, wl.coils[:, vi], e[:, vi], *[d[:, vi] for d in ds],
                                               wl.extra["y"], "f64", threads)
     t_step = (nv / nsim) * t_sim + (nr / nsig) * t_sig + (nr / njv) * t_jv + (nv / nvh) * t_jh
@@ -499,9 +520,10 @@ def spiral_cpu_baseline(wl, e, s, jv, jh, ds, seconds_budget=16.0):
         "unit": "voxel*TR/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"FP32 port: pSSFP on voxels [0, {nsim}) ({t_sim:.2f} s), spiral signal on {nsig} of {nr} readouts "
+        "sample": (f"FP32 port, gcc -O3 {'-march=native' if native else '-march=x86-64-v3'}, OpenMP, medians of 5 "
+                   f"runs: pSSFP on voxels [0, {nsim}) ({t_sim:.2f} s), spiral signal on {nsig} of {nr} readouts "
                    f"({t_sig:.2f} s), J.v on {njv} readouts ({t_jv:.2f} s), J^H.y on {nvh} voxels ({t_jh:.2f} s); "
-                   f"full-step time extrapolated linearly ({t_step:.1f} s)"),
+                   f"step time = whole workload at these size-independent rates ({t_step:.1f} s)"),
     }
     acc = {
         "magnetization_rel_l2_vs_fp64": pyoracle.rel_l2(e[:, :nsim], e64),
@@ -1079,6 +1101,91 @@ def precision_study(args):
     print(json.dumps(line))
 
 
+SIGNAL_DTYPE = {
+    "matmul_fast": "f32 sim; bf16x3 (hi+lo split) MFMA, fp32-acc contraction",
+    "matmul_pedantic": "f32 sim; fp16x3 (hi+lo split, per-row scaling) MFMA, fp32-acc contraction",
+    "matmul_low": "f32 sim; bf16x1 MFMA, fp32-acc contraction",
+    "matmul": "f32 sim; bf16x3 (hi+lo split) MFMA, fp32-acc contraction",
+    "direct": "f32 sim; f32 VALU contraction",
+    "naive": "f32 sim; f32 VALU contraction",
+}
+
+
+def mixed_vs_plain(ctx, wl, params, seq, traj, coils, echos, reps=5, threads=None):
+    """signal + J.v + J^H.y at this workload's shapes, the default mixed-precision
+    methods (BF16 hi+lo on MFMA) against the plain FP32 VALU methods (Direct):
+    wall time per step (median of `reps` synchronised steps after one warm-up)
+    and each output's rel-L2 against the FP64 oracle on independent subsets
+    (6 readouts for signal and J.v, 96 voxels for J^H.y).  Inputs: the step's
+    echoes, their finite-difference T1/T2 derivatives, a seeded [4][nv] direction
+    and a seeded [nc][nr][ns] residual."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    pyoracle.build()
+    threads = threads or int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS",
+                                                                               os.cpu_count() or 1)))
+    nv = wl.tissue.nvoxels
+    nr, ns = wl.trajectory.nreadouts, wl.trajectory.samples_per_readout
+    nc = wl.coils.shape[0]
+    d = ca.simulate_magnetization_derivatives(echos, params, seq)
+    rng = W.SplitMix64(0xC0A50004)
+    v = (rng.normal(4 * nv) + 1j * rng.normal(4 * nv)).reshape(4, nv).astype(np.complex64)
+    y = ((rng.normal(nc * nr * ns) + 1j * rng.normal(nc * nr * ns)) / np.sqrt(2)).reshape(nc, nr, ns).astype(
+        np.complex64)
+    va, ya = ca.CompasArray.from_numpy(v), ca.CompasArray.from_numpy(y)
+
+    def run(sig_m, jac_m):
+        def step():
+            return (ca.magnetization_to_signal(echos, params, traj, coils, method=sig_m),
+                    ca.compute_jacobian(echos, d, params, traj, coils, va, method=jac_m),
+                    ca.compute_jacobian_hermitian(echos, d, params, traj, coils, ya, method=jac_m))
+
+        step()
+        ctx.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = step()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return out, 1e3 * float(np.median(ts))
+
+    mixed, t_mixed = run("matmul_fast", "gemm_fast")
+    plain, t_plain = run("direct", "direct")
+    e = echos.collect()
+    d1, d2 = d["T1"].collect(), d["T2"].collect()
+    ridx = np.linspace(0, nr - 1, 6).astype(int)
+    sub = W.CartesianTraj(len(ridx), ns, wl.trajectory.delta_t, wl.trajectory.k_start[ridx], wl.trajectory.delta_k)
+    vsub = np.linspace(0, nv - 1, 96).astype(int)
+    tsub = wl.tissue.replace(**{f: getattr(wl.tissue, f)[vsub] for f in ("T1", "T2", "B1", "B0", "rho", "x", "y",
+                                                                          "z")})
+    ref = {
+        "signal": pyoracle.signal_cartesian(wl.tissue, sub, wl.coils, e[ridx].astype(np.complex128), "f64", threads),
+        "jv": pyoracle.jacobian(wl.tissue, sub, wl.coils, e[ridx], d1[ridx], d2[ridx], v, "f64", threads),
+        "jhy": pyoracle.jacobian_hermitian(tsub, wl.trajectory, wl.coils[:, vsub], e[:, vsub], d1[:, vsub],
+                                           d2[:, vsub], y, "f64", threads),
+    }
+
+    def errors(outs):
+        s_, jv_, jh_ = (o.collect() for o in outs)
+        return {"signal": pyoracle.rel_l2(s_[:, ridx, :], ref["signal"]),
+                "jv": pyoracle.rel_l2(jv_[:, ridx, :], ref["jv"]),
+                "jhy": pyoracle.rel_l2(jh_[:, vsub], ref["jhy"])}
+
+    return {
+        "mixed_ms": t_mixed, "plain_fp32_ms": t_plain, "speedup": t_plain / t_mixed,
+        "mixed_rel_l2_vs_fp64": errors(mixed), "plain_fp32_rel_l2_vs_fp64": errors(plain),
+        "methods": {"mixed": "matmul_fast / gemm_fast (bf16x3 MFMA, fp32 accumulate)",
+                    "plain_fp32": "direct (fp32 VALU, no MFMA)"},
+        "kspace_samples_per_s": {"mixed": 3 * nc * nr * ns / (t_mixed * 1e-3),
+                                 "plain_fp32": 3 * nc * nr * ns / (t_plain * 1e-3)},
+        "note": f"one step = signal + J.v + J^H.y ({nc} coils x {nr} readouts x {ns} samples x {nv} voxels), "
+                f"median of {reps} synchronised steps; errors on readouts {ridx.tolist()} (signal, J.v) and "
+                f"96 strided voxels (J^H.y)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1087,6 +1194,8 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3])
     ap.add_argument("--method", default="matmul_fast", choices=list(ca.SIGNAL_METHODS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-products", action="store_true",
+                    help="skip the signal + J.v + J^H.y mixed-vs-plain comparison of the default run")
     ap.add_argument("--pipeline", type=int, default=3,
                     help="request-server slots the timed steps are spread over (1 = one stream, no overlap)")
     ap.add_argument("--serve", action="store_true", help="serving loop with Poisson arrivals")
@@ -1220,6 +1329,24 @@ def main():
     ms_per_step = 1e3 * t_total / args.steps
     value = nv * nTR * args.steps / t_total
 
+    # the same sequential step with the plain FP32 contraction (no MFMA)
+    plain_step = None
+    if args.method != "direct":
+        def plain():
+            e_ = ca.simulate_magnetization(params, seq)
+            return ca.magnetization_to_signal(e_, params, traj, coils, method="direct")
+
+        plain()
+        ctx.synchronize()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            plain_out = plain()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        plain_step = {"ms_per_step": 1e3 * float(np.median(ts)), "method": "direct (fp32 VALU contraction)",
+                      "value": nv * nTR / float(np.median(ts)), "note": "median of 5 sequential steps"}
+
     # plain streaming copy of known size: achievable HBM bandwidth, and the
     # reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters to bytes
     probe_bytes = 1 << 30
@@ -1237,7 +1364,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": SIGNAL_DTYPE.get(args.method, "f32"),
         "data": "synthetic (seeded SplitMix64 tissue maps, sinusoidal flip train, Gaussian coil maps)",
         "config": {
             "workload": f"FISP {nv} voxels x {nTR} TRs x {wl.sequence.max_state} states + Cartesian signal "
@@ -1253,11 +1380,15 @@ def main():
         "hbm_copy_GBs": 2 * probe_bytes / (probe_ms * 1e-3) / 1e9,
         "contraction_tuning": ca.contraction_tuning_report().strip().splitlines(),
         "roofline": roofline,
+        "plain_fp32_step": plain_step,
     }
+    if args.method != "direct" and not args.no_products:
+        line["products_mixed_vs_plain"] = mixed_vs_plain(ctx, wl, params, seq, traj, coils, echos)
     if not args.no_cpu_baseline:
         e = echos.collect()
         s = signal.collect()
-        base, acc = cpu_baseline(wl, e, s)
+        others = {"plain_fp32": plain_out.collect()} if plain_step is not None else None
+        base, acc = cpu_baseline(wl, e, s, other_signals=others)
         line["cpu_baseline"] = base
         line["accuracy"] = acc
     else:

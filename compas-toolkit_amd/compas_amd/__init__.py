@@ -534,12 +534,15 @@ class FispSequence(_Object):
 
 
 class pSSFPSequence(_Object):
-    """CompasToolkit.jl:594-631 (z is passed as float32)."""
+    """CompasToolkit.jl:594-631.  Like the Julia wrapper (:614), z is converted
+    to complex64 before the call; the library takes its real parts (a complex z
+    with a nonzero imaginary part is an error).  z_dtype=np.float32 passes a
+    float array instead, as C++ callers of make_pssfp_sequence do."""
 
-    def __init__(self, RF_train, TR, gamma_dt_RF, dt, gamma_dt_GRz, z):
+    def __init__(self, RF_train, TR, gamma_dt_RF, dt, gamma_dt_GRz, z, z_dtype=np.complex64):
         rf = as_array(np.asarray(RF_train, dtype=np.complex64))
         g = as_array(np.asarray(gamma_dt_RF, dtype=np.complex64))
-        zz = as_array(np.asarray(z, dtype=np.float32))
+        zz = as_array(np.asarray(z, dtype=z_dtype))
         ptr = _check(lib().compas_make_pssfp_sequence(rf.ptr, float(TR), g.ptr, *[float(t) for t in dt],
                                                       *[float(t) for t in gamma_dt_GRz], zz.ptr))
         super().__init__(ptr, keep=(rf, g, zz))

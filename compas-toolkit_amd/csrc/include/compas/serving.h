@@ -86,6 +86,7 @@ class Server {
         size_t bytes = 0;
         int slot = -1;
         hipEvent_t done_event = nullptr;
+        hipEvent_t inputs_ready = nullptr;  // recorded on the base stream at submission
         bool done = false;
     };
 

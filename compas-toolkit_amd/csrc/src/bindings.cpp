@@ -98,6 +98,26 @@ const ArrayHandle& array(const void* p, const char* what) {
     return unwrap<ArrayHandle>(p, what);
 }
 
+// Slice positions of a pSSFP sequence.  CompasToolkit.jl:614 converts z to
+// ComplexF32 before the call, and the reference binding reads that complex
+// array as Array<float> (julia-bindings/src/bindings.cpp:186,194), i.e. it
+// simulates the first nz floats [z0, 0, z1, 0, ...] of the complex buffer.
+// Here a complex z stands for its real parts (its imaginary parts must be
+// zero, as Julia's conversion of real positions leaves them); a float z (C++
+// and other callers) is taken as is.
+Array<float, 1> slice_positions(const ArrayHandle& z) {
+    if (z.dtype == DT_FLOAT) return z.as<float, 1>("z");
+    auto zc = z.as<cfloat, 1>("z");
+    std::vector<cfloat> host = zc.copy_to_vector();
+    std::vector<float> re(host.size());
+    for (size_t i = 0; i < host.size(); i++) {
+        if (host[i].imag() != 0.0f)
+            throw CompasException("pSSFP slice positions `z` must be real (a complex z needs zero imaginary parts)");
+        re[i] = host[i].real();
+    }
+    return CompasContext(zc.buffer()->runtime()).allocate(re);
+}
+
 const CompasContext& context(const void* p) {
     if (p == nullptr) throw CompasException("null context");
     return static_cast<const ContextHandle*>(p)->ctx;
@@ -542,7 +562,7 @@ CompasObjectHandle* compas_make_pssfp_sequence(
     return catch_exceptions([&] {
         auto rf = array(RF_train, "RF_train").as<cfloat, 1>("RF_train");
         auto g = array(gamma_dt_RF, "gamma_dt_RF").as<cfloat, 1>("gamma_dt_RF");
-        auto zz = array(z, "z").as<float, 1>("z");
+        auto zz = slice_positions(array(z, "z"));
         COMPAS_CHECK(rf.size(0) > 0);
         return new_object(pSSFPSequence(rf, TR, g, RepetitionData {dt_ex, dt_inv, dt_pr},
                                         RepetitionData {g_ex, g_inv, g_pr}, zz));

@@ -140,12 +140,12 @@ static Array<cfloat, 2> derivative_impl(
             p.data.data(), data.data(), nv, field, delta);
         check_launch("add_difference_to_parameters");
     }
-    // the perturbed field is present even where the original map defaulted it
+    // the perturbed parameters keep the original presence flags
+    // (derivative.cu:43-45): a field the map does not hold (B1, B0 or z absent)
+    // keeps its default in the simulation, so its perturbation has no effect and
+    // the difference is (sim(p) - echos) / delta, zero for echoes of the same map
     TissueParameters q = p;
     q.data = data;
-    if (field == TissueParameterField::B1 && !q.has_b1) COMPAS_ERROR("B1 is not part of these tissue parameters");
-    if (field == TissueParameterField::B0 && !q.has_b0) COMPAS_ERROR("B0 is not part of these tissue parameters");
-    if (field == TissueParameterField::Z && !q.has_z) COMPAS_ERROR("z is not part of these tissue parameters");
     auto next = simulate_magnetization(ctx, q, seq);
     COMPAS_CHECK(next.size(0) == echos.size(0));
     auto out = ctx.empty<cfloat, 2>({echos.size(0), echos.size(1)});

@@ -49,7 +49,7 @@ TissueParameters make_tissue_parameters(
     };
     put(TissueParameterField::T1, T1, 0.0f);
     put(TissueParameterField::T2, T2, 0.0f);
-    put(TissueParameterField::B1, B1, 1.0f);  // tissue.cpp:225-228
+    put(TissueParameterField::B1, B1, 1.0f);  // tissue.cpp:77-80
     put(TissueParameterField::B0, B0, 0.0f);
     put(TissueParameterField::RHO_X, rho_x, 0.0f);
     put(TissueParameterField::RHO_Y, rho_y, 0.0f);

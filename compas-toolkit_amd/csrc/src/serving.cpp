@@ -119,6 +119,12 @@ int64_t Server::submit(ServeRequest request) {
     COMPAS_CHECK(request.fisp || request.pssfp);
     auto e = std::make_unique<Entry>();
     e->bytes = estimate_bytes(request);
+    // the request's inputs may still be in flight on the base stream (uploads,
+    // a residual from compute_residual, arrays shared with another framework
+    // and ordered onto it): the slot waits for this point before its first kernel
+    base_.activate();
+    COMPAS_HIP_CHECK(hipEventCreateWithFlags(&e->inputs_ready, hipEventDisableTiming));
+    COMPAS_HIP_CHECK(hipEventRecord(e->inputs_ready, base_.stream()));
     e->result.return_echos = request.return_echos;
     e->request = std::move(request);
     std::lock_guard<std::mutex> lock(mutex_);
@@ -134,6 +140,7 @@ int64_t Server::submit(ServeRequest request) {
 void Server::launch(Entry& e, int slot) {
     const CompasContext& ctx = slots_[size_t(slot)];
     ctx.activate();
+    COMPAS_HIP_CHECK(hipStreamWaitEvent(ctx.stream(), e.inputs_ready, 0));
     const ServeRequest& r = e.request;
     if (r.kind == RequestKind::SimulateSignal) {
         e.result.echos = simulate(ctx, r);
@@ -179,6 +186,8 @@ void Server::dispatch_loop() {
             e.result.t_done = now();
             (void)hipEventDestroy(e.done_event);
             e.done_event = nullptr;
+            (void)hipEventDestroy(e.inputs_ready);
+            e.inputs_ready = nullptr;
             slot_busy_[size_t(e.slot)] = false;
             in_flight_bytes_ -= e.bytes;
             e.request = ServeRequest {};  // drop the references to the inputs
@@ -212,6 +221,8 @@ void Server::dispatch_loop() {
             } else {
                 // the slot's stream may hold part of the request: let it drain
                 (void)hipStreamSynchronize(slots_[size_t(slot)].stream());
+                (void)hipEventDestroy(e.inputs_ready);
+                e.inputs_ready = nullptr;
                 e.result.error = error;
                 e.result.t_done = now();
                 slot_busy_[size_t(slot)] = false;

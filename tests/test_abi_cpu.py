@@ -105,3 +105,34 @@ def test_no_gpu_context_is_an_error_here():
 
     with pytest.raises(compas_amd.CompasError, match="no HIP device"):
         compas_amd.init_context()
+
+
+def test_error_channel_is_per_thread(lib):
+    """Julia tasks call the library from several threads at once (CompasToolkit.jl:7-14,
+    104-136): each thread reads back the error of its own last call.  ctypes
+    releases the GIL inside the calls, so the calls overlap in the library."""
+    import threading
+
+    calls = [
+        (lambda: lib.compas_make_fisp_sequence(None, None, 0.01, 0.005, 0.0, 8, 0.1, 1, 1), b"`RF_train`"),
+        (lambda: lib.compas_compute_residual(None, None, None, None), b"null context"),
+        (lambda: lib.compas_simulate_magnetization_fisp(None, None, None), b"null context"),
+    ]
+    failures = []
+    barrier = threading.Barrier(len(calls) * 2)
+
+    def worker(call, want):
+        barrier.wait()
+        for _ in range(2000):
+            call()
+            err = lib.compas_last_error()
+            if not err or want not in err:
+                failures.append((want, err))
+                return
+
+    threads = [threading.Thread(target=worker, args=c) for c in calls * 2]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not failures, failures[:3]

@@ -129,3 +129,26 @@ def test_request_errors_are_reported():
     t = server.submit_simulate_signal(p, s, tr, coils)
     assert server.wait(t, timeout=60) is not None
     server.close()
+
+
+@pytest.mark.parametrize("i", [2, 5])  # Cartesian and spiral J^H.y
+def test_inputs_still_queued_on_the_base_stream(i):
+    """A residual produced by library calls on the server's base context and
+    submitted at once (no host wait): the slot waits for the base stream's
+    work queued before the submission, so the result equals the alone path."""
+    banner(f"serving: J^H.y on a residual still in flight (request {i})")
+    ctx = context()
+    kind, p, s, tr, coils, y = device_objects(make_request(i))
+    assert kind == "jhy"
+    server = ca.Server(slots=2, memory_budget=8 << 30)
+    echos = ca.simulate_magnetization(p, s)
+    sig = ca.magnetization_to_signal(echos, p, tr, coils, method="matmul_fast")
+    # compas_compute_residual without the objective pointer: nothing waits on the host
+    res = ca._wrap(ca.lib().compas_compute_residual(ctx.ptr, sig.ptr, y.ptr, None))
+    t = server.submit_jacobian_hermitian(p, s, tr, coils, res, method="gemm_fast")
+    assert server.wait(t, timeout=120) is not None
+    got = server.result(t).collect()
+    server.release(t)
+    server.close()
+    want = run_alone((kind, p, s, tr, coils, res))
+    assert np.array_equal(got, want)

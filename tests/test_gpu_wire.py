@@ -110,3 +110,37 @@ def test_tcp_front_end_pipelined_clients():
     for k in (0, 1):
         bad = results[("bad", k)]
         assert not bad["ok"] and "malformed frame" in bad["error"]
+
+
+def test_spiral_jhy_frame_without_b1_b0():
+    """Tissue frames may omit B1 and B0 (defaults 1 and 0).  The finite
+    difference of an absent field is zero, as in the reference, whose perturbed
+    parameters keep has_b1/has_b0 (derivative.cu:43-45), so the B1 and B0 rows
+    of the six-parameter J^H.y are zero and the others equal those of the frame
+    that sends the defaults explicitly."""
+    banner("serving: spiral J^H.y frame without B1/B0")
+    context()
+    kind, t, seq, traj, coils, y = make_request(5)
+    assert kind == "jhy" and traj.__class__.__name__ == "SpiralTraj"
+    server = ca.Server(slots=2, memory_budget=8 << 30)
+    explicit = t.replace(B1=np.ones_like(t.B1), B0=np.zeros_like(t.B0))
+    frames = [
+        wire.encode_request(wire.KIND_JACOBIAN_HERMITIAN, t, seq, traj, coils, y, jacobian_method="gemm_fast",
+                            fields=("T1", "T2", "rho", "x", "y")),
+        wire.encode_request(wire.KIND_JACOBIAN_HERMITIAN, explicit, seq, traj, coils, y,
+                            jacobian_method="gemm_fast"),
+    ]
+    out = []
+    for f in frames:
+        tk = server.submit_wire(f)
+        assert server.wait_done(tk)
+        res = wire.decode_result(server.result_wire(tk))
+        server.release(tk)
+        assert res["ok"], res.get("error")
+        out.append(res["jhy"])
+    server.close()
+    absent, given = out
+    assert absent.shape == (6, t.nvoxels)
+    assert np.all(absent[2:4] == 0)
+    assert np.array_equal(absent[[0, 1, 4, 5]], given[[0, 1, 4, 5]])
+    assert np.any(given[2:4] != 0)

@@ -7,7 +7,10 @@ here by mathematics that a plausible mistake would break:
 
   FISP  : closed-form first echo; zero flip angle; brute-force isochromat
           ensemble (scipy rotations) for N > nTR; repetitions/TW; slice-profile
-          linearity; undersampling expansion; N-independence when N > nTR.
+          linearity; undersampling expansion; N-independence when N > nTR;
+          truncation (N < nTR): N = 1 equals the ideal-spoiling FLASH
+          recursion, and echoes 0..2N-1 are exact with echo 2N the first
+          changed (a pin that both off-by-one truncations fail).
   pSSFP : closed form of the relaxation-free alternating train; scipy
           rotation-vector ensemble; norm invariance.
   signal: reduction to a centred DFT (numpy.fft); contraction = complex matmul
@@ -100,15 +103,134 @@ def test_fisp_state_count_independent_above_ntr(oracle):
     assert rel(e1, e2) < 1e-14
 
 
-def test_fisp_truncation_drops_high_states(oracle):
-    # truncation at N states: the first N-1 echoes only see states k < N - 1 and are exact
-    t = small_tissue(16)
-    seq = W.make_fisp(40, 45.0, 8)
-    e_tr = oracle.fisp(t, seq, "f64")
-    seq.max_state = 41
-    e_full = oracle.fisp(t, seq, "f64")
-    assert np.max(np.abs(e_tr[:7] - e_full[:7])) < 1e-13
-    assert np.max(np.abs(e_tr[8:] - e_full[8:])) > 1e-8  # truncation is active afterwards
+def _random_train(nTR, seed):
+    """Complex RF train (degrees) whose every flip is nonzero (10-80 degrees, random phase)."""
+    rng = W.SplitMix64(seed)
+    return (rng.uniform(nTR, 10, 80) * np.exp(1j * rng.uniform(nTR, -np.pi, np.pi))).astype(np.complex64)
+
+
+def _truncation_seq(RF, N):
+    return W.FispSeq(RF=RF, profiles=np.ones((1, len(RF)), np.complex64), TR=0.009, TE=0.004,
+                     max_state=N, TI=0.02)
+
+
+def epg_numpy(t, RF, N, TR, TE, TI, rule="reference"):
+    """FP64 EPG over all voxels at once, one slice, inversion, one repetition.
+
+    Excitation is the textbook EPG rotation matrix (Weigel, J. Magn. Reson.
+    Imaging 41:266, 2015, eq. 15), not the oracle's element-wise form.  `rule`
+    selects the state truncation applied at each dephasing step:
+      "reference"  F-_k <- F-_{k+1}, F-_{N-1} <- 0, F+_k <- F+_{k-1} (F+_{N-1} lost),
+                   F+_0 <- conj(F-_0)   (epg.cuh:229-298)
+      "zero_from_N-2"  as reference but F- zeroed from index N-2 (an off-by-one)
+    A run with N+1 states is the other off-by-one (one state too many kept).
+    """
+    T1, T2, B1, B0 = (getattr(t, f).astype(np.float64) for f in ("T1", "T2", "B1", "B0"))
+    nv = T1.shape[0]
+    Fp = np.zeros((N, nv), complex)
+    Fm = np.zeros((N, nv), complex)
+    Z = np.zeros((N, nv), complex)
+    Z[0] = 1.0
+
+    def relax(dt):
+        E1, E2 = np.exp(-dt / T1), np.exp(-dt / T2)
+        r = np.where(B0 == 0, 1.0, np.exp(1j * 2 * np.pi * dt * B0))
+        Fp[:] *= E2 * r
+        Fm[:] *= E2 * np.conj(r)
+        Z[:] *= E1
+        Z[0] += 1 - E1
+
+    Z[:] *= -1  # inversion (ignores B1), then the TI interval
+    E1 = np.exp(-TI / T1)
+    Z[:] *= E1
+    Z[0] += 1 - E1
+    echo = np.zeros((len(RF), nv), complex)
+    for n, rf in enumerate(RF.astype(np.complex128)):
+        a = np.deg2rad(abs(rf)) * B1
+        ph = np.angle(rf) if abs(rf) > 0 else 0.0
+        c2, s2, sa, ca = np.cos(a / 2) ** 2, np.sin(a / 2) ** 2, np.sin(a), np.cos(a)
+        e1, e2 = np.exp(1j * ph), np.exp(2j * ph)
+        fp, fm, z = Fp.copy(), Fm.copy(), Z.copy()
+        Fp[:] = c2 * fp + e2 * s2 * fm - 1j * e1 * sa * z
+        Fm[:] = np.conj(e2) * s2 * fp + c2 * fm + 1j * np.conj(e1) * sa * z
+        Z[:] = -0.5j * np.conj(e1) * sa * fp + 0.5j * e1 * sa * fm + ca * z
+        relax(TE)
+        echo[n] = Fp[0]
+        relax(TR - TE)
+        Fm[:-1] = Fm[1:].copy()
+        Fm[N - 1] = 0
+        if rule == "zero_from_N-2":
+            Fm[max(N - 2, 0):] = 0
+        Fp[1:] = Fp[:-1].copy()
+        Fp[0] = np.conj(Fm[0])
+    return echo
+
+
+def truncation_horizon_holds(truncated, untruncated, N):
+    """With every flip nonzero, the first echo a truncation to N states can
+    change is echo 2N: F+ first reaches index N after TR N-1's shift (where the
+    truncated run drops it); TR N's excitation moves it into F-_N, which the
+    untruncated run shifts to F-_{N-1} (zeroed when truncated); F- then moves
+    down one index per TR, reaching F-_0 and F+_0 = conj(F-_0) after TR 2N-1's
+    shift, so TR 2N's echo is the first to differ.  Echoes 0..2N-1 are exact."""
+    d = np.max(np.abs(truncated - untruncated), axis=1)
+    return bool(np.all(d[: 2 * N] < 1e-13) and d[2 * N] > 1e-8)
+
+
+def test_fisp_single_state_is_ideal_spoiling(oracle):
+    """N = 1: the shift zeroes F-_0 and sets F+_0 = conj(F-_0) = 0 every TR, so the
+    transverse state is perfectly spoiled and the echo follows the FLASH
+    recursion echo_n = -i e^{i phi_n} sin(alpha_n) Z_n E2(TE) r(TE),
+    Z_{n+1} = cos(alpha_n) Z_n E1(TR) + 1 - E1(TR), Z_0 = 1 - 2 E1(TI) (inversion) or 1."""
+    nTR = 40
+    t = small_tissue(24, seed=21, b0=40.0)
+    RF = _random_train(nTR, 5)
+    T1, T2, B1, B0 = (getattr(t, f).astype(np.float64) for f in ("T1", "T2", "B1", "B0"))
+    for inversion in (True, False):
+        seq = W.FispSeq(RF=RF, profiles=np.ones((1, nTR), np.complex64), TR=0.0087, TE=0.0041,
+                        max_state=1, TI=0.025, inversion=inversion)
+        e = oracle.fisp(t, seq, "f64")
+        TR, TE, TI = f32(seq.TR), f32(seq.TE), f32(seq.TI)
+        Z = 1 - 2 * np.exp(-TI / T1) if inversion else np.ones_like(T1)
+        want = np.zeros_like(e)
+        for n, rf in enumerate(RF.astype(np.complex128)):
+            alpha = np.deg2rad(abs(rf)) * B1
+            want[n] = (-1j * np.exp(1j * np.angle(rf)) * np.sin(alpha) * Z * np.exp(-TE / T2)
+                       * np.exp(2j * np.pi * B0 * TE))
+            Z = np.cos(alpha) * Z * np.exp(-TR / T1) + 1 - np.exp(-TR / T1)
+        assert np.max(np.abs(e - want)) < 1e-13
+
+
+@pytest.mark.parametrize("N", [2, 5, 8, 16])
+def test_fisp_truncation_horizon(oracle, N):
+    """Truncation to N < nTR states (epg.cuh:229-298) leaves echoes 0..2N-1
+    exactly equal to the untruncated run (N > nTR, pinned by the isochromat
+    ensemble) and changes echo 2N.  Both off-by-one variants of the rule fail
+    this pin: zeroing F- from index N-2 changes echo 2N-2, and keeping N+1
+    states leaves echo 2N unchanged (shown on the numpy EPG below)."""
+    nTR = 2 * N + 6
+    t = small_tissue(16, seed=3 + N, b0=25.0)
+    RF = _random_train(nTR, 100 + N)
+    full = oracle.fisp(t, _truncation_seq(RF, nTR + 2), "f64")
+    trunc = oracle.fisp(t, _truncation_seq(RF, N), "f64")
+    assert truncation_horizon_holds(trunc, full, N)
+
+
+@pytest.mark.parametrize("N", [2, 5, 8])
+def test_fisp_truncation_pin_rejects_off_by_one(oracle, N):
+    """The horizon pin accepts the reference rule and rejects both off-by-one
+    truncations (mistakes planted in a CPU numpy EPG, never in product code)."""
+    nTR = 2 * N + 6
+    t = small_tissue(16, seed=3 + N, b0=25.0)
+    RF = _random_train(nTR, 100 + N)
+    TR, TE, TI = f32(0.009), f32(0.004), f32(0.02)
+    full = epg_numpy(t, RF, nTR + 2, TR, TE, TI)
+    ref_rule = epg_numpy(t, RF, N, TR, TE, TI)
+    # the numpy EPG with the reference rule is the oracle's truncated run
+    assert np.max(np.abs(ref_rule - oracle.fisp(t, _truncation_seq(RF, N), "f64"))) < 1e-12
+    assert truncation_horizon_holds(ref_rule, full, N)
+    assert not truncation_horizon_holds(epg_numpy(t, RF, N, TR, TE, TI, "zero_from_N-2"), full, N)
+    assert not truncation_horizon_holds(epg_numpy(t, RF, N + 1, TR, TE, TI), full, N)
 
 
 def test_fisp_slice_profiles_sum(oracle):
