@@ -114,10 +114,12 @@ def test_tcp_front_end_pipelined_clients():
 
 def test_spiral_jhy_frame_without_b1_b0():
     """Tissue frames may omit B1 and B0 (defaults 1 and 0).  The finite
-    difference of an absent field is zero, as in the reference, whose perturbed
-    parameters keep has_b1/has_b0 (derivative.cu:43-45), so the B1 and B0 rows
-    of the six-parameter J^H.y are zero and the others equal those of the frame
-    that sends the defaults explicitly."""
+    difference of the echoes with respect to an absent field is zero, as in the
+    reference, whose perturbed parameters keep has_b1/has_b0
+    (derivative.cu:43-45).  B1 enters the spiral signal only through the echoes,
+    so its row of the six-parameter J^H.y is zero; B0 also enters the readout
+    phase (dPhi/dB0 = (s - ns/2) i 2 pi dt Phi), so its row keeps that term.
+    The T1, T2 and rho rows equal those of the frame that sends the defaults."""
     banner("serving: spiral J^H.y frame without B1/B0")
     context()
     kind, t, seq, traj, coils, y = make_request(5)
@@ -141,6 +143,6 @@ def test_spiral_jhy_frame_without_b1_b0():
     server.close()
     absent, given = out
     assert absent.shape == (6, t.nvoxels)
-    assert np.all(absent[2:4] == 0)
+    assert np.all(absent[2] == 0) and np.any(given[2] != 0)
+    assert np.all(np.isfinite(absent[3])) and np.any(absent[3] != given[3])
     assert np.array_equal(absent[[0, 1, 4, 5]], given[[0, 1, 4, 5]])
-    assert np.any(given[2:4] != 0)
