@@ -73,6 +73,12 @@ _SIGNATURES = {
     "compas_clear_error": (None, []),
     "compas_make_fisp_sequence_ex": (_vp, [_vp, _vp, _f, _f, _f, _i, _f, _i, _i, _i, _i]),
     "compas_array_info": (_i, [_vp, C.POINTER(C.c_int), _i64p]),
+    "compas_make_array_bf16": (_vp, [_vp, _vp, _i, _i64p]),
+    "compas_read_array_bf16": (None, [_vp, _vp, _vp, C.c_int64]),
+    "compas_compute_gemm": (_i, [_vp, _vp, _vp, _vp, _f, _f, _i]),
+    "compas_compute_complex_gemm": (_i, [_vp, _vp, _vp, _vp, _f, _f, _i]),
+    "compas_convert_complex_to_planar": (_i, [_vp, _vp, _vp]),
+    "compas_convert_planar_to_complex": (_i, [_vp, _vp, _vp]),
     "compas_magnetization_to_signal_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _i]),
     "compas_compute_jacobian_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
     "compas_compute_jacobian_hermitian_method": (_vp, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
@@ -283,7 +289,9 @@ class CompasArray:
         dtype = C.c_int(0)
         sizes = (C.c_int64 * 3)()
         rank = _check(lib().compas_array_info(ptr, C.byref(dtype), sizes))
-        self.dtype = np.complex64 if dtype.value == 1 else np.float32
+        # bfloat16 arrays are exchanged as their raw bits (uint16)
+        self.bf16 = dtype.value == 2
+        self.dtype = {1: np.complex64, 2: np.uint16}.get(dtype.value, np.float32)
         self.shape = tuple(int(sizes[i]) for i in range(rank))
 
     @classmethod
@@ -298,6 +306,15 @@ class CompasArray:
             fn = lib().compas_make_array_float
         sizes = (C.c_int64 * a.ndim)(*a.shape)
         ptr = _check(fn(ctx.ptr, a.ctypes.data_as(C.c_void_p), a.ndim, sizes))
+        return cls(ptr, ctx)
+
+    @classmethod
+    def from_bf16_bits(cls, bits, context: Optional[Context] = None) -> "CompasArray":
+        """A bfloat16 array from raw bf16 bit patterns (uint16, see to_bf16_bits)."""
+        ctx = context or get_context()
+        b = np.ascontiguousarray(bits, dtype=np.uint16)
+        sizes = (C.c_int64 * b.ndim)(*b.shape)
+        ptr = _check(lib().compas_make_array_bf16(ctx.ptr, b.ctypes.data_as(C.c_void_p), b.ndim, sizes))
         return cls(ptr, ctx)
 
     def write(self, a) -> None:
@@ -318,6 +335,8 @@ class CompasArray:
         n = out.size
         if self.dtype == np.complex64:
             lib().compas_read_array_complex(self.context.ptr, self.ptr, out.ctypes.data_as(C.c_void_p), n)
+        elif self.bf16:
+            lib().compas_read_array_bf16(self.context.ptr, self.ptr, out.ctypes.data_as(C.c_void_p), n)
         else:
             lib().compas_read_array_float(self.context.ptr, self.ptr, out.ctypes.data_as(C.c_void_p), n)
         _check()
@@ -886,6 +905,56 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ GEMM ---
+
+GEMM_METHODS = {"pedantic": 0, "regular": 1, "fast": 2}  # GemmComputeMethod (utils/gemm.h:7-11)
+
+
+def to_bf16_bits(x) -> np.ndarray:
+    """float32 values rounded to the nearest bfloat16 (ties to even), as raw bits."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return r.astype(np.uint16)
+
+
+def from_bf16_bits(b) -> np.ndarray:
+    return (np.asarray(b, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def _gemm_call(fn, result, lhs, rhs, alpha, beta, method):
+    ctx = get_context()
+    ok = fn(ctx.ptr, result.ptr, lhs.ptr, rhs.ptr, float(alpha), float(beta), GEMM_METHODS[method])
+    _check()
+    if ok != 1:
+        raise CompasError("GEMM failed")
+
+
+def compute_gemm(result: CompasArray, lhs: CompasArray, rhs: CompasArray, alpha=1.0, beta=0.0,
+                 method="regular") -> None:
+    """result = alpha * lhs @ rhs.T + beta * result, in place (compute_gemm, utils/gemm.h:21-37)."""
+    _gemm_call(lib().compas_compute_gemm, result, lhs, rhs, alpha, beta, method)
+
+
+def compute_complex_gemm(result: CompasArray, lhs: CompasArray, rhs: CompasArray, alpha=1.0, beta=0.0,
+                         method="regular") -> None:
+    """Planar complex [2][m][n] = alpha * lhs @ rhs.T + beta * result (compute_complex_gemm, utils/gemm.h:39-55)."""
+    _gemm_call(lib().compas_compute_complex_gemm, result, lhs, rhs, alpha, beta, method)
+
+
+def convert_complex_to_planar(output: CompasArray, input: CompasArray) -> None:
+    ok = lib().compas_convert_complex_to_planar(get_context().ptr, output.ptr, input.ptr)
+    _check()
+    if ok != 1:
+        raise CompasError("conversion failed")
+
+
+def convert_planar_to_complex(output: CompasArray, input: CompasArray) -> None:
+    ok = lib().compas_convert_planar_to_complex(get_context().ptr, output.ptr, input.ptr)
+    _check()
+    if ok != 1:
+        raise CompasError("conversion failed")
 
 
 def phase_encoding(echos, parameters, trajectory) -> CompasArray:

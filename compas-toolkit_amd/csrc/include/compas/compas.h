@@ -333,7 +333,8 @@ Array<cfloat, 2> compute_jacobian_hermitian(
 // Jacobian is Cartesian with four): rows of `vector` and of the J^H.y result
 // are (T1, T2, B1, B0, rho_x, rho_y); the T rows are scaled by T1 and T2 as in
 // the Cartesian products.  vector is [6][nv] for J.v and [nc][nr][ns] for
-// J^H.y.  The spiral sums run on FP32 VALU for every method.
+// J^H.y.  The mixed-precision methods run the spiral sums on MFMA (BF16
+// hi + lo, FP32 accumulation); Naive, Direct and Gemm run them on FP32 VALU.
 Array<cfloat, 3> compute_jacobian(
     const CompasContext& ctx,
     Array<cfloat, 2> echos,
@@ -367,5 +368,61 @@ Array<cfloat, 3> compute_residual(
     Array<cfloat, 3> lhs,
     Array<cfloat, 3> rhs,
     float* objective_out = nullptr);
+
+// GEMM utility (include/compas/utils/gemm.h:7-71): result = alpha * lhs *
+// transpose(rhs) + beta * result with lhs [m][k], rhs [n][k], result [m][n]
+// (complex: planar [2][..][..], real plane first).  Arrays are contiguous and
+// the result is written in place.  Precision per method for FP32 operands:
+// Pedantic = FP32 FMA on VALU, Regular = BF16 hi + lo on MFMA (three products,
+// FP32-level), Fast = BF16 on MFMA (one product).  BF16 operands: exact
+// products, FP32 accumulation, for every method.
+enum struct GemmComputeMethod {
+    Pedantic,  // highest precision
+    Regular,  // balanced
+    Fast  // highest performance
+};
+
+using bfloat16_t = __bf16;
+
+void compute_gemm(
+    const CompasContext& ctx,
+    Array<float, 2> result,
+    Array<float, 2> lhs,
+    Array<float, 2> rhs,
+    float alpha,
+    float beta,
+    GemmComputeMethod kind = GemmComputeMethod::Regular);
+
+void compute_gemm(
+    const CompasContext& ctx,
+    Array<float, 2> result,
+    Array<bfloat16_t, 2> lhs,
+    Array<bfloat16_t, 2> rhs,
+    float alpha,
+    float beta,
+    GemmComputeMethod kind = GemmComputeMethod::Regular);
+
+void compute_complex_gemm(
+    const CompasContext& ctx,
+    Array<float, 3> result,
+    Array<float, 3> lhs,
+    Array<float, 3> rhs,
+    float alpha,
+    float beta,
+    GemmComputeMethod kind = GemmComputeMethod::Regular);
+
+void compute_complex_gemm(
+    const CompasContext& ctx,
+    Array<float, 3> result,
+    Array<bfloat16_t, 3> lhs,
+    Array<bfloat16_t, 3> rhs,
+    float alpha,
+    float beta,
+    GemmComputeMethod kind = GemmComputeMethod::Regular);
+
+// [n][m] complex <-> planar [2][n][m] (real plane first)
+void convert_complex_to_planar(const CompasContext& ctx, Array<float, 3> output, Array<cfloat, 2> input);
+void convert_complex_to_planar(const CompasContext& ctx, Array<bfloat16_t, 3> output, Array<cfloat, 2> input);
+void convert_planar_to_complex(const CompasContext& ctx, Array<cfloat, 2> output, Array<float, 3> input);
 
 }  // namespace compas

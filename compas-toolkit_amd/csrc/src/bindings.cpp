@@ -43,7 +43,20 @@ struct ObjectHandle: Handle, T {
     const char* name() const override { return typeid(T).name(); }
 };
 
-enum DType { DT_FLOAT = 0, DT_COMPLEX = 1 };
+enum DType { DT_FLOAT = 0, DT_COMPLEX = 1, DT_BF16 = 2 };
+
+template<typename T>
+constexpr int dtype_of() {
+    return std::is_same<T, cfloat>::value ? DT_COMPLEX : std::is_same<T, bfloat16_t>::value ? DT_BF16 : DT_FLOAT;
+}
+
+const char* dtype_name(int dtype) {
+    return dtype == DT_COMPLEX ? "complex" : dtype == DT_BF16 ? "bfloat16" : "float";
+}
+
+size_t dtype_size(int dtype) {
+    return dtype == DT_COMPLEX ? sizeof(cfloat) : dtype == DT_BF16 ? sizeof(uint16_t) : sizeof(float);
+}
 
 struct ArrayHandle: Handle {
     std::shared_ptr<Buffer> buf;
@@ -51,7 +64,9 @@ struct ArrayHandle: Handle {
     int rank = 0;
     int64_t sizes[3] = {0, 0, 0};
 
-    const char* name() const override { return dtype == DT_FLOAT ? "Array<float>" : "Array<cfloat>"; }
+    const char* name() const override {
+        return dtype == DT_FLOAT ? "Array<float>" : dtype == DT_BF16 ? "Array<bfloat16>" : "Array<cfloat>";
+    }
 
     int64_t numel() const {
         int64_t n = 1;
@@ -61,10 +76,10 @@ struct ArrayHandle: Handle {
 
     template<typename T, size_t N>
     Array<T, N> as(const char* what) const {
-        const int want = std::is_same<T, cfloat>::value ? DT_COMPLEX : DT_FLOAT;
+        const int want = dtype_of<T>();
         if (dtype != want || rank != int(N)) {
             std::stringstream ss;
-            ss << "invalid array for `" << what << "`: expecting " << (want == DT_COMPLEX ? "complex" : "float")
+            ss << "invalid array for `" << what << "`: expecting " << dtype_name(want)
                << " rank " << N << ", gotten " << name() << " rank " << rank;
             throw CompasException(ss.str());
         }
@@ -77,7 +92,7 @@ struct ArrayHandle: Handle {
     static ArrayHandle* wrap(const Array<T, N>& a) {
         auto* h = new ArrayHandle();
         h->buf = a.buffer();
-        h->dtype = std::is_same<T, cfloat>::value ? DT_COMPLEX : DT_FLOAT;
+        h->dtype = dtype_of<T>();
         h->rank = int(N);
         for (size_t i = 0; i < N; i++) h->sizes[i] = a.size(i);
         return h;
@@ -194,6 +209,11 @@ SimulateSignalMethod signal_method(int m) {
     return SimulateSignalMethod(m);
 }
 
+GemmComputeMethod gemm_method(int m) {
+    if (m < 0 || m > 2) COMPAS_ERROR("invalid value for `GemmComputeMethod`");
+    return GemmComputeMethod(m);
+}
+
 JacobianComputeMethod jacobian_method(int m) {
     if (m < 0 || m > 4) COMPAS_ERROR("invalid value for `JacobianComputeMethod`");
     return JacobianComputeMethod(m);
@@ -296,7 +316,7 @@ struct DLManagedTensor {
     void (*deleter)(DLManagedTensor*);
 };
 constexpr int32_t DL_ROCM = 10;
-constexpr uint8_t DL_FLOAT = 2, DL_COMPLEX = 5;
+constexpr uint8_t DL_FLOAT = 2, DL_BFLOAT = 4, DL_COMPLEX = 5;
 
 // An exported array: the DLPack view and a reference to the buffer
 struct ExportedArray {
@@ -388,6 +408,21 @@ void compas_read_array_complex(
     });
 }
 
+const CompasArrayHandle* compas_make_array_bf16(
+    const CompasContextHandle* ctx, const uint16_t* data, int rank, int64_t* sizes) {
+    return catch_exceptions([&] {
+        return reinterpret_cast<const CompasArrayHandle*>(static_cast<const Handle*>(
+            make_array_impl<bfloat16_t>(context(ctx), reinterpret_cast<const bfloat16_t*>(data), rank, sizes)));
+    });
+}
+
+void compas_read_array_bf16(const CompasContextHandle* ctx, const CompasArrayHandle* a, uint16_t* dest, int64_t length) {
+    catch_exceptions([&] {
+        (void)context(ctx);
+        read_array_impl(array(a, "array"), dest, length, sizeof(uint16_t), DT_BF16);
+    });
+}
+
 void compas_write_array_complex(
     const CompasContextHandle* ctx, const CompasArrayHandle* a, const compas_cfloat* src, int64_t length) {
     catch_exceptions([&] {
@@ -409,7 +444,7 @@ void compas_copy_array(const CompasContextHandle* ctx, const CompasArrayHandle* 
         if (d.dtype != s.dtype || d.numel() != s.numel()) throw CompasException("arrays differ in type or size");
         if (d.numel() == 0) return;
         c.activate();
-        const size_t elem = d.dtype == DT_COMPLEX ? sizeof(cfloat) : sizeof(float);
+        const size_t elem = dtype_size(d.dtype);
         COMPAS_HIP_CHECK(hipMemcpyAsync(d.buf->data(), s.buf->data(), size_t(d.numel()) * elem,
                                         hipMemcpyDeviceToDevice, c.stream()));
     });
@@ -493,7 +528,9 @@ void* compas_array_to_dlpack(const CompasArrayHandle* a) {
         t.data = h.buf->data();
         t.device = {DL_ROCM, h.buf->runtime()->device()};
         t.ndim = h.rank;
-        t.dtype = h.dtype == DT_COMPLEX ? DLDataType {DL_COMPLEX, 64, 1} : DLDataType {DL_FLOAT, 32, 1};
+        t.dtype = h.dtype == DT_COMPLEX ? DLDataType {DL_COMPLEX, 64, 1}
+                  : h.dtype == DT_BF16  ? DLDataType {DL_BFLOAT, 16, 1}
+                                        : DLDataType {DL_FLOAT, 32, 1};
         t.shape = e->shape;
         t.strides = nullptr;  // row-major, compact
         t.byte_offset = 0;
@@ -723,6 +760,65 @@ CompasObjectHandle* compas_make_fisp_sequence_ex(
         auto prof = array(sliceprofiles, "sliceprofiles").as<cfloat, 2>("sliceprofiles");
         return new_object(FISPSequence(rf, prof, TR, TE, TW, max_state, TI, undersampling_factor, repetitions,
                                        inversion_prepulse != 0, wait_spoiling != 0));
+    });
+}
+
+int compas_compute_gemm(
+    const CompasContextHandle* ctx, const CompasArrayHandle* result, const CompasArrayHandle* lhs,
+    const CompasArrayHandle* rhs, float alpha, float beta, int kind) {
+    return catch_exceptions([&] {
+        const auto method = gemm_method(kind);
+        auto out = array(result, "result").as<float, 2>("result");
+        const ArrayHandle& l = array(lhs, "lhs");
+        if (l.dtype == DT_BF16) {
+            compute_gemm(context(ctx), out, l.as<bfloat16_t, 2>("lhs"),
+                         array(rhs, "rhs").as<bfloat16_t, 2>("rhs"), alpha, beta, method);
+        } else {
+            compute_gemm(context(ctx), out, l.as<float, 2>("lhs"), array(rhs, "rhs").as<float, 2>("rhs"), alpha,
+                         beta, method);
+        }
+        return 1;
+    });
+}
+
+int compas_compute_complex_gemm(
+    const CompasContextHandle* ctx, const CompasArrayHandle* result, const CompasArrayHandle* lhs,
+    const CompasArrayHandle* rhs, float alpha, float beta, int kind) {
+    return catch_exceptions([&] {
+        const auto method = gemm_method(kind);
+        auto out = array(result, "result").as<float, 3>("result");
+        const ArrayHandle& l = array(lhs, "lhs");
+        if (l.dtype == DT_BF16) {
+            compute_complex_gemm(context(ctx), out, l.as<bfloat16_t, 3>("lhs"),
+                                 array(rhs, "rhs").as<bfloat16_t, 3>("rhs"), alpha, beta, method);
+        } else {
+            compute_complex_gemm(context(ctx), out, l.as<float, 3>("lhs"), array(rhs, "rhs").as<float, 3>("rhs"),
+                                 alpha, beta, method);
+        }
+        return 1;
+    });
+}
+
+int compas_convert_complex_to_planar(
+    const CompasContextHandle* ctx, const CompasArrayHandle* output, const CompasArrayHandle* input) {
+    return catch_exceptions([&] {
+        const ArrayHandle& o = array(output, "output");
+        auto in = array(input, "input").as<cfloat, 2>("input");
+        if (o.dtype == DT_BF16) {
+            convert_complex_to_planar(context(ctx), o.as<bfloat16_t, 3>("output"), in);
+        } else {
+            convert_complex_to_planar(context(ctx), o.as<float, 3>("output"), in);
+        }
+        return 1;
+    });
+}
+
+int compas_convert_planar_to_complex(
+    const CompasContextHandle* ctx, const CompasArrayHandle* output, const CompasArrayHandle* input) {
+    return catch_exceptions([&] {
+        convert_planar_to_complex(context(ctx), array(output, "output").as<cfloat, 2>("output"),
+                                  array(input, "input").as<float, 3>("input"));
+        return 1;
     });
 }
 

@@ -140,8 +140,39 @@ CompasObjectHandle* compas_make_fisp_sequence_ex(
     const CompasArrayHandle* RF_train, const CompasArrayHandle* sliceprofiles, float TR, float TE, float TW,
     int max_state, float TI, int undersampling_factor, int repetitions, int inversion_prepulse, int wait_spoiling);
 
-/* Shape of an array handle: dtype 0 = float32, 1 = complex64. Returns rank. */
+/* Shape of an array handle: dtype 0 = float32, 1 = complex64, 2 = bfloat16.
+ * Returns rank. */
 int compas_array_info(const CompasArrayHandle* array, int* dtype, int64_t* sizes3);
+
+/* GEMM utility (include/compas/utils/gemm.h:7-71 and src/utils/gemm.cu:8-254
+ * in the reference, a C++ API there with no C-ABI): result = alpha * lhs *
+ * transpose(rhs) + beta * result, written in place into `result`.
+ * compas_compute_gemm: result float [m][n], lhs [m][k], rhs [n][k], both
+ * float or both bfloat16.  compas_compute_complex_gemm: planar complex
+ * result float [2][m][n], lhs [2][m][k], rhs [2][n][k] (real plane first).
+ * kind = GemmComputeMethod {Pedantic, Regular, Fast} as 0..2: for float
+ * operands FP32 FMA on VALU, BF16 hi + lo on MFMA (FP32-level), BF16 on MFMA
+ * (one product); bfloat16 operands run exact products with FP32
+ * accumulation for every method.  beta = 0 does not read `result`.
+ * compas_convert_complex_to_planar: complex [n][m] into float or bfloat16
+ * [2][n][m]; compas_convert_planar_to_complex: float [2][n][m] into complex
+ * [n][m] (gemm_kernels.cuh:9-32).  bfloat16 arrays hold raw bf16 bits
+ * (compas_make_array_bf16 / compas_read_array_bf16).  Each returns 1, or 0 on
+ * error. */
+const CompasArrayHandle* compas_make_array_bf16(
+    const CompasContextHandle* ctx, const uint16_t* data, int rank, int64_t* sizes);
+void compas_read_array_bf16(
+    const CompasContextHandle* ctx, const CompasArrayHandle* array, uint16_t* dest, int64_t length);
+int compas_compute_gemm(
+    const CompasContextHandle* ctx, const CompasArrayHandle* result, const CompasArrayHandle* lhs,
+    const CompasArrayHandle* rhs, float alpha, float beta, int kind);
+int compas_compute_complex_gemm(
+    const CompasContextHandle* ctx, const CompasArrayHandle* result, const CompasArrayHandle* lhs,
+    const CompasArrayHandle* rhs, float alpha, float beta, int kind);
+int compas_convert_complex_to_planar(
+    const CompasContextHandle* ctx, const CompasArrayHandle* output, const CompasArrayHandle* input);
+int compas_convert_planar_to_complex(
+    const CompasContextHandle* ctx, const CompasArrayHandle* output, const CompasArrayHandle* input);
 
 /* Method-selecting variants of the signal and Jacobian entry points, exposing
  * the C++ enums SimulateSignalMethod {Naive, Direct, MatmulPedantic, Matmul,
