@@ -60,6 +60,7 @@ _SIGNATURES = {
     "compas_make_spiral_trajectory": (_vp, [_i, _i, _f, _vp, _vp]),
     "compas_simulate_magnetization_fisp": (_vp, [_vp, _vp, _vp]),
     "compas_simulate_magnetization_pssfp": (_vp, [_vp, _vp, _vp]),
+    "compas_simulate_magnetization_fisp_noalloc": (_i, [_vp, _vp, _vp, _vp, C.c_int64]),
     "compas_magnetization_to_signal_cartesian": (_vp, [_vp, _i, _vp, _vp, _vp, _vp, _i32]),
     "compas_magnetization_to_signal_spiral": (_vp, [_vp, _i, _vp, _vp, _vp, _vp]),
     "compas_compute_jacobian": (_vp, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]),

@@ -91,6 +91,14 @@ bool spiral_sum_mfma(
 // Per (slice, TR) RF table of the FISP kernels: (deg2rad |RF|, cos phi, sin phi, 0).
 Array<float4, 1> fisp_rf_table(const CompasContext& ctx, const FISPSequence& seq);
 
+// simulate_magnetization(FISP) into the caller's device buffer `echos`
+// ([nTR * usf][nvoxels], `capacity` complex values) without allocating: the
+// RF table goes to a static per-GPU device array of 65536 (slice, TR)
+// entries.  Calls that overlap on one GPU race on that array.
+void simulate_magnetization_into(
+    const CompasContext& ctx, const TissueParameters& parameters, const FISPSequence& seq, cfloat* echos,
+    int64_t capacity);
+
 // pSSFP launch layout: ZPL slice positions per lane, G = ceil(nz / ZPL) wave
 // groups per block of 64 voxels, at most per_launch groups per launch (the
 // partial echoes of groups 1.. are held in one buffer of <= 8 GiB).

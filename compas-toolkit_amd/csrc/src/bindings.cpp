@@ -634,6 +634,17 @@ CompasArrayHandle* compas_simulate_magnetization_fisp(
     });
 }
 
+int compas_simulate_magnetization_fisp_noalloc(
+    const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence,
+    compas_cfloat* echos, int64_t capacity) {
+    return catch_exceptions([&]() -> int {
+        simulate_magnetization_into(
+            context(ctx), unwrap<TissueParameters>(parameters, "parameters"),
+            unwrap<FISPSequence>(sequence, "sequence"), reinterpret_cast<cfloat*>(echos), capacity);
+        return 1;
+    });
+}
+
 CompasArrayHandle* compas_simulate_magnetization_pssfp(
     const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence) {
     return catch_exceptions([&] {

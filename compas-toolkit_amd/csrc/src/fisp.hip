@@ -24,6 +24,8 @@
 #include <cmath>
 
 #include <cstdlib>
+#include <mutex>
+#include <sstream>
 
 #include "compas/device/common.h"
 #include "compas/device/fisp_common.h"
@@ -364,17 +366,27 @@ static void choose_fisp_layout(int N, int nvoxels, int simds, int* spl, int* lpv
     }
 }
 
+static void check_fisp_sequence(const FISPSequence& seq) {
+    COMPAS_CHECK(seq.max_state >= 1);
+    if (seq.max_state > 128) COMPAS_ERROR("max_state cannot exceed 128");  // fisp.cu:100-102
+    COMPAS_CHECK(seq.sliceprofiles.size(1) == seq.RF_train.size(0));  // fisp.cu:22
+    COMPAS_CHECK(seq.undersampling_factor >= 1);
+    COMPAS_CHECK(seq.repetitions >= 1);
+}
+
+// Launches the recursion for every voxel: `table` holds the RF table of
+// `seq` (fisp_rf_table), `echos` the [nTR * usf][nvoxels] output.
+static void launch_fisp_epg(
+    const CompasContext& ctx, const TissueParameters& parameters, const FISPSequence& seq, const float4* table,
+    cfloat* echos, bool timed);
+
 Array<cfloat, 2> simulate_magnetization(
     const CompasContext& ctx,
     TissueParameters parameters,
     FISPSequence seq) {
+    check_fisp_sequence(seq);
     const int nTR = int(seq.RF_train.size(0));
     const int nv = parameters.nvoxels;
-    COMPAS_CHECK(seq.max_state >= 1);
-    if (seq.max_state > 128) COMPAS_ERROR("max_state cannot exceed 128");  // fisp.cu:100-102
-    COMPAS_CHECK(seq.sliceprofiles.size(1) == nTR);  // fisp.cu:22
-    COMPAS_CHECK(seq.undersampling_factor >= 1);
-    COMPAS_CHECK(seq.repetitions >= 1);
     const int nslices = int(seq.sliceprofiles.size(0));
     const int nr = nTR * seq.undersampling_factor;
     ctx.activate();
@@ -387,10 +399,85 @@ Array<cfloat, 2> simulate_magnetization(
     }
 
     auto table = fisp_rf_table(ctx, seq);
+    launch_fisp_epg(ctx, parameters, seq, table.data(), echos.data(), true);
+    return echos;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the RF table lives in a file-scope device array
+// instead of the context's pool, and the caller provides the echo buffer.
+//
+// Each GPU holds its own copy of `scratch` for the life of the process.  In
+// host code the name `scratch` is a host-side stand-in, never a device
+// address: the per-device address comes from hipGetSymbolAddress, looked up
+// once per device with that device current, and only that address is passed
+// to kernels.  Calls on one GPU share the array, so calls that overlap on one
+// GPU (different streams or threads without ordering) race; calls on one
+// context's stream are ordered and safe.
+
+static constexpr int64_t kRfTableScratchEntries = int64_t(1) << 16;  // (slice, TR) pairs: 1 MiB
+
+static __device__ __align__(256) float4 scratch[kRfTableScratchEntries];
+
+static float4* scratch_address(int device) {
+    static std::mutex mutex;
+    static float4* address[256] = {};
+    if (device < 0 || device >= 256) COMPAS_ERROR("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mutex);
+    if (address[device] == nullptr) {
+        void* p = nullptr;
+        COMPAS_HIP_CHECK(hipSetDevice(device));
+        COMPAS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) COMPAS_ERROR("no device address for the RF-table scratch array");
+        address[device] = static_cast<float4*>(p);
+    }
+    return address[device];
+}
+
+void simulate_magnetization_into(
+    const CompasContext& ctx, const TissueParameters& parameters, const FISPSequence& seq, cfloat* echos,
+    int64_t capacity) {
+    check_fisp_sequence(seq);
+    const int64_t nTR = seq.RF_train.size(0);
+    const int64_t nv = parameters.nvoxels;
+    const int64_t nslices = seq.sliceprofiles.size(0);
+    const int64_t nr = nTR * seq.undersampling_factor;
+    if (nslices * nTR > kRfTableScratchEntries) {
+        std::stringstream ss;
+        ss << "sequence too large for the allocation-free FISP simulation: " << nslices << " slice profiles x " << nTR
+           << " TRs = " << nslices * nTR << " RF-table entries, at most " << kRfTableScratchEntries;
+        throw CompasException(ss.str());
+    }
+    if (capacity < nr * nv) {
+        std::stringstream ss;
+        ss << "echo buffer too small: " << capacity << " complex values for [" << nr << "][" << nv << "]";
+        throw CompasException(ss.str());
+    }
+    if (nr * nv == 0) return;
+    if (echos == nullptr) COMPAS_ERROR("null echo buffer");
+    ctx.activate();
+    if (nslices == 0) {
+        ctx.fill_zero(echos, size_t(nr * nv) * sizeof(cfloat));
+        return;
+    }
+    float4* table = scratch_address(ctx.device());
+    const int total = int(nslices * nTR);
+    kernels::fisp_rf_table<<<dev::div_ceil(total, 256), 256, 0, ctx.stream()>>>(
+        seq.RF_train.data(), seq.sliceprofiles.data(), int(nTR), int(nslices), table);
+    check_launch("fisp_rf_table");
+    launch_fisp_epg(ctx, parameters, seq, table, echos, false);
+}
+
+static void launch_fisp_epg(
+    const CompasContext& ctx, const TissueParameters& parameters, const FISPSequence& seq, const float4* table,
+    cfloat* echos, bool timed) {
+    const int nTR = int(seq.RF_train.size(0));
+    const int nv = parameters.nvoxels;
+    const int nslices = int(seq.sliceprofiles.size(0));
 
     kernels::FispArgs a;
     a.tissue = dev::tissue_view(parameters);
-    a.table = table.data();
+    a.table = table;
     a.nTR = nTR;
     a.nslices = nslices;
     a.TR = seq.TR;
@@ -402,7 +489,7 @@ Array<cfloat, 2> simulate_magnetization(
     a.repetitions = seq.repetitions;
     a.inversion = seq.inversion_prepulse;
     a.wait_spoiling = seq.wait_spoiling;
-    a.echos = echos.data();
+    a.echos = echos;
 
     int spl = 0, lpv = 0;
     choose_fisp_layout(seq.max_state, nv, simd_count(ctx.device()), &spl, &lpv);
@@ -417,12 +504,14 @@ Array<cfloat, 2> simulate_magnetization(
     FispLauncher fn = lpv == 1 ? fisp_launcher_lpv1(spl) : lpv == 2 ? fisp_launcher_lpv2(spl) : fisp_launcher_lpv4(spl);
     COMPAS_CHECK(fn != nullptr);
     unsigned blocks = dev::div_ceil(int64_t(nv) * lpv, 64);
-    {
+    if (timed) {
         KernelScope ks(ctx, "fisp_epg");
         fn(a, blocks, ctx.stream());
         check_launch("fisp_epg");
+    } else {
+        fn(a, blocks, ctx.stream());
+        check_launch("fisp_epg");
     }
-    return echos;
 }
 
 }  // namespace compas

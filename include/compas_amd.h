@@ -90,6 +90,19 @@ CompasObjectHandle* compas_make_spiral_trajectory(
 /* bindings.cpp:230 -- returns echos [nreadouts][nvoxels] complex */
 CompasArrayHandle* compas_simulate_magnetization_fisp(
     const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): compas_simulate_magnetization_fisp into the caller's
+ * device buffer `echos` ([nTR * undersampling][nvoxels] complex64, room for
+ * `capacity` values, on the context's GPU), with no allocation.  The per-TR
+ * RF table goes to a static device array of the library holding at most
+ * 65536 (slice profile, TR) pairs per GPU; larger sequences are rejected
+ * with an error.  Every call on one GPU shares that array: calls ordered on
+ * one stream are safe, calls that overlap on one GPU race.  Returns 1, or 0
+ * on error (message through jl_errorf or compas_last_error). */
+int compas_simulate_magnetization_fisp_noalloc(
+    const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence,
+    compas_cfloat* echos, int64_t capacity);
 /* bindings.cpp:243 */
 CompasArrayHandle* compas_simulate_magnetization_pssfp(
     const CompasContextHandle* ctx, const CompasObjectHandle* parameters, const CompasObjectHandle* sequence);
