@@ -124,6 +124,7 @@ _SIGNATURES = {
     "compas_server_wait": (_i, [_vp, C.c_int64, C.c_double, C.POINTER(C.c_double)]),
     "compas_server_result": (_vp, [_vp, C.c_int64, _i]),
     "compas_server_release": (None, [_vp, C.c_int64]),
+    "compas_server_batch_stats": (_i, [_vp, _i64p]),
     "compas_server_now": (C.c_double, [_vp]),
     "compas_server_submit_wire": (C.c_int64, [_vp, C.c_char_p, C.c_int64]),
     "compas_server_result_wire": (C.c_int64, [_vp, C.c_int64, _vp, C.c_int64]),
@@ -816,9 +817,10 @@ def _check_quiet():
 
 
 class Server:
-    """Request server of the library (continuous batching on one GPU): `slots`
-    requests run concurrently on private streams, admitted in arrival order
-    within `memory_budget` bytes of estimated device memory."""
+    """Request server of the library (continuous batching on one GPU):
+    requests admitted within `memory_budget` bytes of estimated device memory
+    share work-table simulation launches; their contractions run on `slots`
+    private streams."""
 
     def __init__(self, slots: int = 4, memory_budget: int = 64 << 30):
         ctx = get_context()
@@ -849,8 +851,17 @@ class Server:
         return (times[0], times[1], times[2]) if rc == 1 else None
 
     def result(self, ticket: int, which: int = 1) -> CompasArray:
-        """which = 0: echos; 1: signal (simulate_signal) or J^H.y (jacobian_hermitian)."""
+        """which = 0: echos; 1: signal (simulate_signal) or J^H.y (jacobian_hermitian);
+        2..: the finite differences of a jacobian_hermitian request (T1, T2, then B1, B0 for spiral)."""
         return _wrap(lib().compas_server_result(self.ptr, int(ticket), int(which)))
+
+    def batch_stats(self) -> dict:
+        """Work-table launches so far, their work items, the most requests one
+        launch served, and the launches that served two or more requests."""
+        out = (C.c_int64 * 4)()
+        lib().compas_server_batch_stats(self.ptr, out)
+        _check()
+        return {"launches": out[0], "items": out[1], "max_requests": out[2], "shared_launches": out[3]}
 
     def release(self, ticket: int):
         lib().compas_server_release(self.ptr, int(ticket))

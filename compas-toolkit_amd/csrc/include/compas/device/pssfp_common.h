@@ -25,6 +25,13 @@ struct PssfpArgs {
     cfloat* echos;  // [nTR][nvoxels]
 };
 
+// A pSSFP simulation in a work table (request server): group 0 of a voxel
+// block writes a.echos, group g > 0 writes partial + (g - 1) * nTR * nv.
+struct PssfpTableJob {
+    PssfpArgs a;
+    cfloat* partial;
+};
+
 struct Vec3d {
     double x, y, z;
 };

@@ -8,6 +8,11 @@
 
 namespace compas {
 
+namespace kernels {
+struct FispArgs;
+struct PssfpTableJob;
+}  // namespace kernels
+
 // dst[i] += src[i] for i < n (complex)
 void add_into(const CompasContext& ctx, cfloat* dst, const cfloat* src, int64_t n);
 
@@ -126,6 +131,25 @@ int64_t pssfp_scratch_bytes(const CompasContext& ctx, int nz, int nv, int nTR);
 // partial + j * stride), in FP64, in order
 void pssfp_sum_groups(
     const CompasContext& ctx, cfloat* echos, const cfloat* partial, int count, int64_t n, int64_t stride);
+
+// Work-table launches of the request server: workgroup b runs items[b].
+//   FISP: item (job, voxel block of 64), every job with max_state == spl,
+//   fisp_table_supported(spl); one lane per voxel.
+//   pSSFP: item (job, voxel block, group, -), slice positions per lane
+//   zpl = pssfp_table_zpl(nz) of every job (a function of nz alone, so a
+//   request's result does not depend on what else shares the launch).
+bool fisp_table_supported(int max_state);
+void launch_fisp_table(const CompasContext& ctx, int spl, const kernels::FispArgs* jobs, const int2* items, int n);
+int pssfp_table_zpl(int nz);
+void launch_pssfp_table(
+    const CompasContext& ctx, int zpl, const kernels::PssfpTableJob* jobs, const int4* items, int n);
+
+// Finite-difference pieces of simulate_magnetization_derivative
+// (derivative.cu:11-94): the parameters with `field` moved by `delta` (the
+// presence flags kept), and (next - echos) / delta.
+TissueParameters perturbed_parameters(const CompasContext& ctx, const TissueParameters& p, int field, float delta);
+Array<cfloat, 2> finite_difference(
+    const CompasContext& ctx, const Array<cfloat, 2>& echos, const Array<cfloat, 2>& next, float delta);
 
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);

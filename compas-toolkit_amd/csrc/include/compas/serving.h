@@ -4,13 +4,25 @@
 // that calls the library one operation at a time).
 //
 // Requests reference objects that are already resident in HBM (tissue maps,
-// sequences, trajectories, coil maps, residuals).  A dispatcher thread admits
-// requests in arrival order while (a) a slot is free and (b) the estimated
-// device memory of the in-flight requests stays within the budget.  Each slot
-// is a private runtime (its own HIP stream and memory pool), so admitted
-// requests run concurrently on the device and a request that arrives while
-// others run starts as soon as a slot and memory free up.  Completion is
-// detected with an event per request; results stay in HBM until taken.
+// sequences, trajectories, coil maps, residuals).  A dispatcher thread:
+//   * admits pending requests while their estimated device memory fits the
+//     budget -- in arrival order, except that a request that does not fit
+//     lets later ones that do fit pass until it has waited 0.5 s, after which
+//     it holds the queue until it fits;
+//   * splits each admitted request's simulations (the echoes, and for J^H.y
+//     the perturbed simulations of its finite differences) into work items
+//     (voxel blocks, and slice-position groups for pSSFP) and runs them in
+//     work-table launches: each launch takes items from every active
+//     simulation in fair-share order (least GPU time used first), so one
+//     launch serves several requests, small requests finish within a launch
+//     or two, and large ones progress alongside instead of blocking the queue.
+//     Requests that arrive join the next launch; two launches are in flight
+//     at a time on two streams, so the tail of one overlaps the next;
+//   * when a request's simulations are complete (device events, no host
+//     wait), runs its signal or J^H.y contraction on one of S slot streams.
+// Completion is detected with an event per request; results stay in HBM
+// until taken.  A voxel's echoes are computed by the same arithmetic whatever
+// else shares a launch, so a request's result is the same alone and in a mix.
 
 #include <atomic>
 #include <chrono>
@@ -50,6 +62,7 @@ struct ServeResult {
     Array<cfloat, 2> echos;  // [nr][nv]
     Array<cfloat, 3> signal;  // SimulateSignal: [nc][nr][ns]
     Array<cfloat, 2> jhy;  // JacobianHermitian: [4 or 6][nv]
+    std::vector<Array<cfloat, 2>> derivatives;  // JacobianHermitian: finite differences T1, T2 (, B1, B0)
     double t_arrival = 0, t_start = 0, t_done = 0;  // seconds since the server started
     std::string error;
     bool return_echos = true;  // copied from the request
@@ -79,26 +92,63 @@ class Server {
     uint64_t id() const { return id_; }  // unique per server in the process
     int slots() const { return int(slots_.size()); }
 
+    // Work-table launches so far: launches, work items, the most requests
+    // one launch served, and the launches that served two or more requests.
+    struct BatchStats {
+        int64_t launches = 0, items = 0, max_requests = 0, shared_launches = 0;
+    };
+    BatchStats batch_stats();
+
+    struct SimJob;
+    struct Event;
+
   private:
     struct Entry {
         ServeRequest request;
         ServeResult result;
         size_t bytes = 0;
         int slot = -1;
+        bool table_path = false;  // simulations in work-table launches (else all calls on a slot)
+        std::vector<std::shared_ptr<SimJob>> jobs;  // [0]: the echoes; then one per finite difference
+        size_t next_job = 0;  // jobs activated so far
+        size_t jobs_done = 0;  // jobs whose completion step is enqueued
+        std::vector<Array<cfloat, 2>> derivs;  // finite differences, in the order of the jobs
+        hipEvent_t sim_done = nullptr;  // simulations complete (recorded on the simulation stream)
         hipEvent_t done_event = nullptr;
         hipEvent_t inputs_ready = nullptr;  // recorded on the base stream at submission
         bool done = false;
     };
 
     void dispatch_loop();
-    void launch(Entry& e, int slot);
+    void launch(Entry& e, int slot);  // every call of the request on a slot stream
+    void post_process(Entry& e, int slot);  // the request's contraction after its simulations
+    void admit(int64_t ticket, Entry& e);
+    void activate_jobs(int64_t ticket, Entry& e);
+    void build_and_launch_batch();
+    void complete_jobs();
+    void finish_entry(Entry& e, const std::string& error);
 
     CompasContext base_;
     uint64_t id_;
     std::vector<CompasContext> slots_;
     std::vector<bool> slot_busy_;
+    std::vector<CompasContext> sim_;  // two streams for the work-table launches; sim_[0] holds the job memory
     size_t budget_, in_flight_bytes_ = 0;
     std::chrono::steady_clock::time_point t0_;
+
+    // work tables: pinned host staging and device copies, a ring of slots
+    // each reusable once its launch's event has completed
+    static constexpr int kRing = 4;
+    static constexpr size_t kRingBytes = size_t(1) << 20;
+    void* host_ring_[kRing] = {};
+    Array<uint8_t, 1> dev_ring_[kRing];
+    std::shared_ptr<Event> ring_busy_[kRing];
+    int next_stream_ = 0;
+    hipEvent_t alloc_event_ = nullptr;  // recorded on sim_[0] after job memory is set up
+    std::vector<std::shared_ptr<SimJob>> active_jobs_;
+    std::vector<int64_t> admitted_;  // table-path requests whose simulations are not all complete
+    std::deque<int64_t> post_queue_;  // simulations complete or enqueued, waiting for a slot
+    BatchStats stats_;
 
     std::mutex mutex_;
     std::condition_variable cv_work_, cv_done_;

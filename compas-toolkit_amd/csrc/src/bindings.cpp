@@ -1120,7 +1120,21 @@ CompasArrayHandle* compas_server_result(CompasServerHandle* server, int64_t tick
             if (r.signal.buffer()) return new_array(r.signal);
             return new_array(r.jhy);
         }
-        throw CompasException("`which` must be 0 (echos) or 1 (signal or J^H.y)");
+        if (which >= 2 && size_t(which - 2) < r.derivatives.size()) return new_array(r.derivatives[size_t(which - 2)]);
+        throw CompasException(
+            "`which` must be 0 (echos), 1 (signal or J^H.y) or 2.. (finite differences of a J^H.y request)");
+    });
+}
+
+int compas_server_batch_stats(CompasServerHandle* server, int64_t* out4) {
+    return catch_exceptions([&] {
+        COMPAS_CHECK(out4 != nullptr);
+        const Server::BatchStats b = server_of(server).batch_stats();
+        out4[0] = b.launches;
+        out4[1] = b.items;
+        out4[2] = b.max_requests;
+        out4[3] = b.shared_launches;
+        return 1;
     });
 }
 

@@ -126,12 +126,9 @@ void add_into(const CompasContext& ctx, cfloat* dst, const cfloat* src, int64_t 
     check_launch("add_into");
 }
 
-template<typename Sequence>
-static Array<cfloat, 2> derivative_impl(
-    const CompasContext& ctx, int field, Array<cfloat, 2> echos, TissueParameters p, Sequence seq, float delta) {
+TissueParameters perturbed_parameters(const CompasContext& ctx, const TissueParameters& p, int field, float delta) {
     ctx.activate();
     const int nv = p.nvoxels;
-    COMPAS_CHECK(echos.size(1) == nv);  // derivative.cu:24
     COMPAS_CHECK(field >= 0 && field < TissueParameterField::NUM_FIELDS);  // derivative.cu:25
     auto data = ctx.empty<float, 2>({TissueParameterField::NUM_FIELDS, nv});
     if (nv > 0) {
@@ -146,8 +143,13 @@ static Array<cfloat, 2> derivative_impl(
     // the difference is (sim(p) - echos) / delta, zero for echoes of the same map
     TissueParameters q = p;
     q.data = data;
-    auto next = simulate_magnetization(ctx, q, seq);
-    COMPAS_CHECK(next.size(0) == echos.size(0));
+    return q;
+}
+
+Array<cfloat, 2> finite_difference(
+    const CompasContext& ctx, const Array<cfloat, 2>& echos, const Array<cfloat, 2>& next, float delta) {
+    ctx.activate();
+    COMPAS_CHECK(next.size(0) == echos.size(0) && next.size(1) == echos.size(1));
     auto out = ctx.empty<cfloat, 2>({echos.size(0), echos.size(1)});
     const int64_t n = out.numel();
     if (n > 0) {
@@ -157,6 +159,15 @@ static Array<cfloat, 2> derivative_impl(
         check_launch("finite_difference");
     }
     return out;
+}
+
+template<typename Sequence>
+static Array<cfloat, 2> derivative_impl(
+    const CompasContext& ctx, int field, Array<cfloat, 2> echos, TissueParameters p, Sequence seq, float delta) {
+    COMPAS_CHECK(echos.size(1) == p.nvoxels);  // derivative.cu:24
+    auto q = perturbed_parameters(ctx, p, field, delta);
+    auto next = simulate_magnetization(ctx, q, seq);
+    return finite_difference(ctx, echos, next, delta);
 }
 
 Array<cfloat, 2> simulate_magnetization_derivative(

@@ -29,6 +29,7 @@
 
 #include "compas/device/common.h"
 #include "compas/device/fisp_common.h"
+#include "compas/device/fisp_epg.h"
 #include "compas/internal.h"
 
 namespace compas {
@@ -54,167 +55,7 @@ __global__ void __launch_bounds__(64) fisp_epg(FispArgs a) {
     const int sub = (LPV > 1) ? int(threadIdx.x % LPV) : 0;
     const int voxel = (blockIdx.x * 64 + threadIdx.x) / LPV;
     if (voxel >= a.tissue.nvoxels) return;  // whole LPV groups leave together
-
-    const float T1 = a.tissue.T1(voxel);
-    const float T2 = a.tissue.T2(voxel);
-    const float B1 = a.tissue.B1(voxel);
-    const float B0 = a.tissue.B0(voxel);
-
-    const float E1_TE = relax(a.TE, T1), E2_TE = relax(a.TE, T2);
-    const float E1_TI = relax(a.TI, T1);
-    const float E1_TRmTE = relax(a.TR - a.TE, T1), E2_TRmTE = relax(a.TR - a.TE, T2);
-    const float E1_W = relax(a.TW, T1), E2_W = relax(a.TW, T2);
-
-    // transverse decay+precession factors d = E2 * exp(i 2 pi t B0)
-    const cfloat d_TE = off_resonance(a.TE, B0) * E2_TE;
-    const cfloat d_TRmTE = off_resonance(a.TR - a.TE, B0) * E2_TRmTE;
-    const cfloat d_W = off_resonance(a.TW, B0) * E2_W;
-    const cfloat d_TR = dev::cmul(d_TE, d_TRmTE);
-    const float E1_TR = E1_TE * E1_TRmTE;
-    // regrowth of Z_0 accumulated over TE and TR-TE
-    const float G = E1_TRmTE * (1.0f - E1_TE) + (1.0f - E1_TRmTE);
-
-    const bool owns_zero = (sub == 0);  // lane holding state k = 0
-    const int k0 = sub * SPL;           // global index of local state 0
-
-    // F+_0 at TE from the updated state: over TR - TE the transverse state
-    // only decays and precesses (F+ times d_TRmTE), so F+_0(TE) = F+_0(TR) /
-    // d_TRmTE, one complex multiply per TR instead of a second 2x3 operator.
-    // A wave with a lane whose d_TRmTE is too small to invert (T2 of a few
-    // microseconds) applies the operator D(TE) R to state 0 instead.
-    const float dd = d_TRmTE.re * d_TRmTE.re + d_TRmTE.im * d_TRmTE.im;
-    const bool direct_echo = __any(!(dd > 1e-30f));  // wave-uniform
-    const cfloat inv_TRmTE = direct_echo ? cfloat {0.0f, 0.0f}
-                                         : cfloat {d_TRmTE.re / dd, -d_TRmTE.im / dd};
-
-    for (int slice = 0; slice < a.nslices; slice++) {
-        const float4* tab = a.table + size_t(slice) * size_t(a.nTR);
-        cfloat P[SPL], Q[SPL], Z[SPL];
-#pragma unroll
-        for (int i = 0; i < SPL; i++) {
-            P[i] = {0, 0};
-            Q[i] = {0, 0};
-            Z[i] = {0, 0};
-        }
-        if (owns_zero) Z[0] = {1.0f, 0.0f};
-
-        for (int rep = 0; rep < a.repetitions; rep++) {
-            const bool sample = (rep == a.repetitions - 1);
-            if (a.inversion) {  // invert (ignores B1), spoil, decay(TI), regrow
-#pragma unroll
-                for (int i = 0; i < SPL; i++) {
-                    P[i] = {0, 0};
-                    Q[i] = {0, 0};
-                    Z[i] = Z[i] * (-E1_TI);
-                }
-                if (owns_zero) Z[0].re += 1.0f - E1_TI;
-            }
-
-            for (int n = 0; n < a.nTR; n++) {
-                const float4 t = tab[n];  // wave-uniform
-                const float alpha = t.x * B1;
-                float sx, cx;
-                dev::sincos_fast(0.5f * alpha, &sx, &cx);
-                const float sa = 2.0f * sx * cx;  // sin(alpha)
-                const float omc = 2.0f * sx * sx;  // 1 - cos(alpha), without cancellation
-                const float cp = t.y, sp = t.z;
-                // rotation about (cos phi, sin phi, 0) by alpha, real XYZ basis
-                const float R00 = 1.0f - omc * sp * sp, R01 = omc * cp * sp, R02 = sa * sp;
-                const float R10 = R01, R11 = 1.0f - omc * cp * cp, R12 = -sa * cp;
-                const float R20 = -sa * sp, R21 = sa * cp, R22 = 1.0f - omc;
-
-                // M = D(TR) R ; rows 0/1 mix with the complex decay d = dr + i di
-                const float dr = d_TR.re, di = d_TR.im;
-                const float M00 = dr * R00 - di * R10, M01 = dr * R01 - di * R11, M02 = dr * R02 - di * R12;
-                const float M10 = di * R00 + dr * R10, M11 = di * R01 + dr * R11, M12 = di * R02 + dr * R12;
-                const float M20 = E1_TR * R20, M21 = E1_TR * R21, M22 = E1_TR * R22;
-                // X = S/2, Y = -i D/2 with S = F+ + F-, D = F+ - F-
-                const float h00 = 0.5f * M00, h01 = 0.5f * M01;
-                const float h10 = 0.5f * M10, h11 = 0.5f * M11;
-                const float h20 = 0.5f * M20, h21 = 0.5f * M21;
-
-                // sample F+_0 at TE:  rows 0/1 of D(TE) R applied to state 0
-                if (direct_echo && sample && owns_zero) {
-                    const float er = d_TE.re, ei = d_TE.im;
-                    const float T00 = er * R00 - ei * R10, T01 = er * R01 - ei * R11, T02 = er * R02 - ei * R12;
-                    const float T10 = ei * R00 + er * R10, T11 = ei * R01 + er * R11, T12 = ei * R02 + er * R12;
-                    const cfloat S0 = P[0] + Q[0], D0 = P[0] - Q[0];
-                    const cfloat z0 = Z[0];
-                    const float xr = fmaf(0.5f * T00, S0.re, fmaf(0.5f * T01, D0.im, T02 * z0.re));
-                    const float xi = fmaf(0.5f * T00, S0.im, fmaf(-0.5f * T01, D0.re, T02 * z0.im));
-                    const float yr = fmaf(0.5f * T10, S0.re, fmaf(0.5f * T11, D0.im, T12 * z0.re));
-                    const float yi = fmaf(0.5f * T10, S0.im, fmaf(-0.5f * T11, D0.re, T12 * z0.im));
-                    const cfloat echo = {xr - yi, xi + yr};
-                    for (int u = 0; u < a.usf; u++) {
-                        cfloat* out = a.echos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
-                        *out = (slice == 0) ? echo : (*out + echo);
-                    }
-                }
-
-                cfloat Pn[SPL], Qn[SPL];
-#pragma unroll
-                for (int i = 0; i < SPL; i++) {
-                    const cfloat S = P[i] + Q[i], D = P[i] - Q[i];
-                    const cfloat z = Z[i];
-                    const float xr = fmaf(h00, S.re, fmaf(h01, D.im, M02 * z.re));
-                    const float xi = fmaf(h00, S.im, fmaf(-h01, D.re, M02 * z.im));
-                    const float yr = fmaf(h10, S.re, fmaf(h11, D.im, M12 * z.re));
-                    const float yi = fmaf(h10, S.im, fmaf(-h11, D.re, M12 * z.im));
-                    const float zr = fmaf(h20, S.re, fmaf(h21, D.im, M22 * z.re));
-                    const float zi = fmaf(h20, S.im, fmaf(-h21, D.re, M22 * z.im));
-                    Pn[i] = {xr - yi, xi + yr};
-                    Qn[i] = {xr + yi, xi - yr};
-                    Z[i] = {zr, zi};
-                }
-                if (owns_zero) Z[0].re += G;
-                if (!direct_echo && sample && owns_zero) {
-                    const cfloat echo = dev::cmul(Pn[0], inv_TRmTE);
-                    for (int u = 0; u < a.usf; u++) {
-                        cfloat* out = a.echos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
-                        *out = (slice == 0) ? echo : (*out + echo);
-                    }
-                }
-
-                // dephasing: F+_k <- F+_{k-1}, F-_k <- F-_{k+1}, F-_{N-1} <- 0, F+_0 <- conj(F-_0)
-                cfloat p_in = {0, 0}, q_in = {0, 0};
-                if constexpr (LPV > 1) {
-                    p_in.re = __shfl_up(Pn[SPL - 1].re, 1, LPV);
-                    p_in.im = __shfl_up(Pn[SPL - 1].im, 1, LPV);
-                    q_in.re = __shfl_down(Qn[0].re, 1, LPV);
-                    q_in.im = __shfl_down(Qn[0].im, 1, LPV);
-                    if (sub == LPV - 1) q_in = {0, 0};
-                }
-#pragma unroll
-                for (int i = SPL - 1; i >= 1; i--) P[i] = Pn[i - 1];
-                P[0] = p_in;
-#pragma unroll
-                for (int i = 0; i < SPL - 1; i++) Q[i] = Qn[i + 1];
-                Q[SPL - 1] = q_in;
-                if constexpr (LPV > 1) {
-#pragma unroll
-                    for (int i = 0; i < SPL; i++) {
-                        if (k0 + i >= a.N - 1) Q[i] = {0, 0};
-                    }
-                }
-                if (owns_zero) P[0] = conj(Q[0]);
-            }
-
-            if (a.wait_spoiling) {
-#pragma unroll
-                for (int i = 0; i < SPL; i++) {
-                    P[i] = {0, 0};
-                    Q[i] = {0, 0};
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < SPL; i++) {
-                P[i] = dev::cmul(P[i], d_W);
-                Q[i] = dev::cmul(Q[i], conj(d_W));
-                Z[i] = Z[i] * E1_W;
-            }
-            if (owns_zero) Z[0].re += 1.0f - E1_W;
-        }
-    }
+    fisp_epg_voxel<SPL, LPV>(a, voxel, sub);
 }
 
 }  // namespace kernels

@@ -42,19 +42,16 @@
 namespace compas {
 namespace kernels {
 
-// grid (ceil(nv / 64), groups in this launch), block 64.  Group g of the
-// launch is group0 + blockIdx.y; blockIdx.y == 0 writes out0, blockIdx.y = j
-// > 0 writes rest + (j - 1) * nTR * nv.
+// One wave: voxels 64 vblock .. 64 vblock + 63 (one per lane) times slice
+// positions ZPL group .. ZPL group + ZPL - 1; writes the sum over those
+// positions of every TR's echo to out[TR][voxel].  grf: LDS for RF_CHUNK
+// waveform samples.
 template<int ZPL>
-__global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat* out0, cfloat* rest) {
-    __shared__ double2 grf[RF_CHUNK];  // gamma_dt_RF as doubles (LDS broadcast reads)
+__device__ __forceinline__ void pssfp_iso_block(const PssfpArgs& a, int group, int vblock, cfloat* out, double2* grf) {
     const int lane = threadIdx.x;
-    const int group = group0 + int(blockIdx.y);
-    const int voxel = blockIdx.x * 64 + lane;
+    const int voxel = vblock * 64 + lane;
     const bool active = voxel < a.tissue.nvoxels;
     const int vx = active ? voxel : 0;
-    const size_t plane = size_t(a.nTR) * size_t(a.tissue.nvoxels);
-    cfloat* out = (blockIdx.y == 0) ? out0 : rest + size_t(blockIdx.y - 1) * plane;
 
     const double T1 = a.tissue.T1(vx), T2 = a.tissue.T2(vx);
     const double B1 = a.tissue.B1(vx), B0 = a.tissue.B0(vx);
@@ -182,6 +179,32 @@ __global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat*
     }
 }
 
+// grid (ceil(nv / 64), groups in this launch), block 64.  Group g of the
+// launch is group0 + blockIdx.y; blockIdx.y == 0 writes out0, blockIdx.y = j
+// > 0 writes rest + (j - 1) * nTR * nv.
+template<int ZPL>
+__global__ void __launch_bounds__(64) pssfp_iso(PssfpArgs a, int group0, cfloat* out0, cfloat* rest) {
+    __shared__ double2 grf[RF_CHUNK];  // gamma_dt_RF as doubles (LDS broadcast reads)
+    const size_t plane = size_t(a.nTR) * size_t(a.tissue.nvoxels);
+    cfloat* out = (blockIdx.y == 0) ? out0 : rest + size_t(blockIdx.y - 1) * plane;
+    pssfp_iso_block<ZPL>(a, group0 + int(blockIdx.y), int(blockIdx.x), out, grf);
+}
+
+// Work-table form (request server): workgroup b runs item (job, voxel block,
+// group) = items[b].xyz of jobs[job]; group 0 writes the job's echoes, group
+// g > 0 the partial plane g - 1.  The jobs of one launch may belong to
+// different requests with any voxel, TR and RF counts; a wave's arithmetic
+// is that of pssfp_iso for the same (voxel block, group).
+template<int ZPL>
+__global__ void __launch_bounds__(64) pssfp_iso_table(const PssfpTableJob* __restrict__ jobs, const int4* __restrict__ items) {
+    __shared__ double2 grf[RF_CHUNK];
+    const int4 it = items[blockIdx.x];
+    const PssfpTableJob job = jobs[it.x];
+    const size_t plane = size_t(job.a.nTR) * size_t(job.a.tissue.nvoxels);
+    cfloat* out = it.z == 0 ? job.a.echos : job.partial + size_t(it.z - 1) * plane;
+    pssfp_iso_block<ZPL>(job.a, it.z, it.y, out, grf);
+}
+
 // echos[i] += sum_j partial[j][i], accumulated in FP64 in slot order
 __global__ void pssfp_sum_groups_kernel(
     cfloat* __restrict__ echos, const cfloat* __restrict__ partial, int count, int64_t n, int64_t stride) {
@@ -274,6 +297,34 @@ int resident_waves() {
 }
 
 }  // namespace
+
+int pssfp_table_zpl(int nz) {
+    // the fewest groups of at most 7 positions, balanced: 35 -> 7 x 5, 10 -> 5 x 2
+    const int groups = std::max(1, (nz + 6) / 7);
+    return std::max(1, (nz + groups - 1) / groups);
+}
+
+template<int ZPL>
+static void launch_table(hipStream_t s, const kernels::PssfpTableJob* jobs, const int4* items, int n) {
+    kernels::pssfp_iso_table<ZPL><<<unsigned(n), 64, 0, s>>>(jobs, items);
+}
+
+void launch_pssfp_table(
+    const CompasContext& ctx, int zpl, const kernels::PssfpTableJob* jobs, const int4* items, int n) {
+    if (n <= 0) return;
+    KernelScope ks(ctx, "pssfp_iso_table");
+    switch (zpl) {
+        case 1: launch_table<1>(ctx.stream(), jobs, items, n); break;
+        case 2: launch_table<2>(ctx.stream(), jobs, items, n); break;
+        case 3: launch_table<3>(ctx.stream(), jobs, items, n); break;
+        case 4: launch_table<4>(ctx.stream(), jobs, items, n); break;
+        case 5: launch_table<5>(ctx.stream(), jobs, items, n); break;
+        case 6: launch_table<6>(ctx.stream(), jobs, items, n); break;
+        case 7: launch_table<7>(ctx.stream(), jobs, items, n); break;
+        default: COMPAS_ERROR("slice positions per lane outside 1..7");
+    }
+    check_launch("pssfp_iso_table");
+}
 
 static PssfpLayout primal_layout(const CompasContext& ctx, int nz, int nv, int nTR) {
     const int waves[8] = {0,

@@ -331,21 +331,29 @@ void* compas_array_to_dlpack(const CompasArrayHandle* array);
 
 /* Request server (no counterpart in the reference, whose host is a Julia
  * session calling one operation at a time): continuous batching on one GPU.
- * `slots` requests run concurrently, each on a private stream and memory
- * pool; pending requests are admitted in arrival order as soon as a slot is
- * free and the estimated device memory of the running requests fits
- * `memory_budget` bytes.  Requests reference resident objects (parameters,
- * sequence, trajectory, coils, residual handles).  Tickets are >= 0; -1 means
- * the submission failed (compas_last_error).
+ * Pending requests are admitted while the estimated device memory of the
+ * admitted ones fits `memory_budget` bytes (arrival order; a request that
+ * does not fit lets later ones pass for 0.5 s, then holds the queue).  The
+ * simulations of all admitted requests run as work items (voxel blocks,
+ * slice-position groups) of shared work-table launches in fair-share order,
+ * so one launch serves several requests; a request's contraction (signal or
+ * J^H.y) then runs on one of `slots` private streams.  Requests reference
+ * resident objects (parameters, sequence, trajectory, coils, residual
+ * handles).  Tickets are >= 0; -1 means the submission failed
+ * (compas_last_error).
  *   simulate_signal: simulate_magnetization + magnetization_to_signal
  *     (FISP or pSSFP, Cartesian or spiral), method = SimulateSignalMethod;
  *   jacobian_hermitian: simulation, finite-difference derivatives and J^H.y
  *     (Cartesian: 4 parameters; spiral: 6), method = JacobianComputeMethod.
  * compas_server_wait returns 1 when done (times = arrival, start, done in
- * seconds on the server clock), 0 on timeout, -1 if the request failed.
- * compas_server_result: which = 0 echos, 1 signal or J^H.y (new handle);
- * results stay in HBM until compas_server_release.  Destroying the server
- * waits for the running and pending requests. */
+ * seconds on the server clock; start = admission), 0 on timeout, -1 if the
+ * request failed.
+ * compas_server_result: which = 0 echos, 1 signal or J^H.y, 2.. the finite
+ * differences of a J^H.y request (T1, T2, then B1, B0 for spiral) (new
+ * handle); results stay in HBM until compas_server_release.
+ * compas_server_batch_stats: out4 = work-table launches, work items, the
+ * most requests one launch served, launches that served two or more.
+ * Destroying the server waits for the running and pending requests. */
 CompasServerHandle* compas_server_create(const CompasContextHandle* ctx, int slots, int64_t memory_budget);
 void compas_server_destroy(CompasServerHandle* server);
 int64_t compas_server_submit_simulate_signal(
@@ -358,6 +366,7 @@ int64_t compas_server_submit_jacobian_hermitian(
 int compas_server_wait(CompasServerHandle* server, int64_t ticket, double timeout_seconds, double* times);
 CompasArrayHandle* compas_server_result(CompasServerHandle* server, int64_t ticket, int which);
 void compas_server_release(CompasServerHandle* server, int64_t ticket);
+int compas_server_batch_stats(CompasServerHandle* server, int64_t* out4);
 double compas_server_now(CompasServerHandle* server);
 /* Per-kernel HIP-event timing on every slot stream of the server; the read
  * has the format of compas_read_kernel_timings (merged over the slots). */
