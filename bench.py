@@ -31,6 +31,11 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# The request server drives the base stream, its slot streams and three
+# simulation streams; with HIP's default of four hardware queues some of them
+# would share a queue and run one after another.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 sys.path.insert(0, os.path.join(ROOT, "compas-toolkit_amd"))
 
 import compas_amd as ca  # noqa: E402
@@ -832,6 +837,7 @@ def serve_bench(args):
             progress(f"submitted {i + 1} of {n_req}")
     th.join()
     progress("all requests done; re-running the checked ones alone")
+    batching = server.batch_stats()  # calibration burst + open loop
     server.close()
 
     ok = [s for s in stamps if s is not None]
@@ -875,6 +881,10 @@ def serve_bench(args):
         "calibration": f"{len(cal_ids)} requests submitted at once: {cal_span:.2f} s",
         "alone_vs_mix_max_rel_l2": worst,
         "alone_vs_mix_checked": len(check_ids),
+        "batching": dict(batching, requests=len(cal_ids) + n_req,
+                         note="work-table simulation launches (FISP voxel blocks, pSSFP voxel-block x "
+                              "slice-group items) over the calibration burst and the run; shared_launches "
+                              "served two or more requests; queue_ms is arrival -> admission"),
         "errors": errors[:5],
         "setup_s": t_setup,
     }

@@ -16,8 +16,9 @@
 //     simulation in fair-share order (least GPU time used first), so one
 //     launch serves several requests, small requests finish within a launch
 //     or two, and large ones progress alongside instead of blocking the queue.
-//     Requests that arrive join the next launch; two launches are in flight
-//     at a time on two streams, so the tail of one overlaps the next;
+//     Requests that arrive join the next launch; up to three launches are in
+//     flight on three streams, so the tail of one overlaps the others, and the
+//     items of a launch go longest first;
 //   * when a request's simulations are complete (device events, no host
 //     wait), runs its signal or J^H.y contraction on one of S slot streams.
 // Completion is detected with an event per request; results stay in HBM
@@ -132,18 +133,20 @@ class Server {
     uint64_t id_;
     std::vector<CompasContext> slots_;
     std::vector<bool> slot_busy_;
-    std::vector<CompasContext> sim_;  // two streams for the work-table launches; sim_[0] holds the job memory
+    std::vector<CompasContext> sim_;  // streams of the work-table launches; sim_[0] holds the job memory
     size_t budget_, in_flight_bytes_ = 0;
     std::chrono::steady_clock::time_point t0_;
 
-    // work tables: pinned host staging and device copies, a ring of slots
-    // each reusable once its launch's event has completed
-    static constexpr int kRing = 4;
+    // work-table launches: kSimStreams streams, at most one launch in flight
+    // on each; tables staged in pinned host memory and copied to a ring of
+    // device buffers, each reusable once its launch's event has completed
+    static constexpr int kSimStreams = 3;
+    static constexpr int kRing = 2 * kSimStreams;
     static constexpr size_t kRingBytes = size_t(1) << 20;
     void* host_ring_[kRing] = {};
     Array<uint8_t, 1> dev_ring_[kRing];
     std::shared_ptr<Event> ring_busy_[kRing];
-    int next_stream_ = 0;
+    std::shared_ptr<Event> stream_busy_[kSimStreams];  // each stream's last launch
     hipEvent_t alloc_event_ = nullptr;  // recorded on sim_[0] after job memory is set up
     std::vector<std::shared_ptr<SimJob>> active_jobs_;
     std::vector<int64_t> admitted_;  // table-path requests whose simulations are not all complete

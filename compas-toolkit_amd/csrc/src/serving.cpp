@@ -8,7 +8,7 @@
 //      FISP RF tables, pSSFP partial planes, perturbed tissue copies) comes
 //      from the pool of sim_[0], set up on its stream after the request's
 //      inputs are ready.
-//   2. Work-table launches.  Each launch takes up to kBatchWaves work items
+//   2. Work-table launches.  Each launch takes up to batch_waves() work items
 //      (one wave each) from the active jobs, least GPU time used first (each
 //      job's virtual time grows by its item cost), groups them by kernel
 //      instance and launches fisp_epg_table / pssfp_iso_table over a table
@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <queue>
 #include <set>
@@ -40,7 +41,23 @@ namespace compas {
 namespace {
 
 constexpr float kDelta = 1e-4f;  // finite-difference step (derivative.h:21-47 default)
-constexpr int kBatchWaves = 2048;  // work items (waves) per launch: two per SIMD of a 256-CU GPU
+// Work items (waves) per launch.  pSSFP: 2048, two waves per SIMD of a
+// 256-CU GPU (the FP64 recursion needs the second wave to hide its latency).
+// FISP: 1024, one wave per SIMD -- a FISP wave issues VALU on most of its
+// SIMD's cycles alone, and the contractions of other requests run beside it
+// (measured on the simulate + signal steps of 50176 voxels x 1120 TRs: 2.26
+// ms per step with 1024, 2.38 with 2048).  COMPAS_SERVER_BATCH_WAVES sets
+// both (measurement).
+int batch_waves(bool pssfp) {
+    static const int forced = [] {
+        const char* e = std::getenv("COMPAS_SERVER_BATCH_WAVES");
+        const int v = e ? std::atoi(e) : 0;
+        return v >= 64 && v <= 65536 ? v : 0;
+    }();
+    if (forced) return forced;
+    return pssfp ? 2048 : 1024;
+}
+
 constexpr int kMaxBatchJobs = 256;  // simulations per launch
 constexpr int kMaxActive = 64;  // admitted requests at once
 constexpr double kMaxPassSeconds = 0.5;  // a request that does not fit may be passed this long after arrival
@@ -129,7 +146,7 @@ struct Server::SimJob {
     kernels::PssfpTableJob pss {};
     Array<cfloat, 2> echos, partial;
     Array<float4, 1> table;
-    std::shared_ptr<Event> last[2];  // last launch holding items of this job, per stream
+    std::shared_ptr<Event> last[Server::kSimStreams];  // last launch holding items of this job, per stream
 };
 
 namespace {
@@ -226,7 +243,7 @@ Server::Server(const CompasContext& base, int slots, size_t memory_budget) :
         slots_.emplace_back(rt);
     }
     slot_busy_.assign(size_t(slots), false);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < kSimStreams; i++) {
         auto rt = std::make_shared<Runtime>(base.device());
         rt->set_autotune(false);
         sim_.emplace_back(rt);
@@ -418,7 +435,9 @@ void Server::complete_jobs() {
             SimJob& j = *e->jobs[k];
             if (j.completed || j.next < j.items) continue;
             if (k > 0 && !e->jobs[0]->completed) continue;
-            if (j.last[1]) COMPAS_HIP_CHECK(hipStreamWaitEvent(sim.stream(), j.last[1]->ev, 0));
+            for (int st = 1; st < kSimStreams; st++) {
+                if (j.last[st]) COMPAS_HIP_CHECK(hipStreamWaitEvent(sim.stream(), j.last[st]->ev, 0));
+            }
             if (j.pssfp && j.groups > 1) {
                 const int64_t plane = j.echos.numel();
                 pssfp_sum_groups(sim, j.echos.data(), j.partial.data(), j.groups - 1, plane, plane);
@@ -449,31 +468,43 @@ void Server::complete_jobs() {
     }
 }
 
-// One work-table launch over the active jobs, if a ring slot is free and
-// fewer than two launches are in flight.
+// One work-table launch over the active jobs, on a stream with no launch in
+// flight, if a ring slot is free.
 void Server::build_and_launch_batch() {
-    int ring = -1, busy = 0;
+    int ring = -1, s = -1;
     for (int r = 0; r < kRing; r++) {
         if (ring_busy_[r] && ring_busy_[r]->complete()) ring_busy_[r].reset();
-        if (ring_busy_[r]) {
-            busy++;
-        } else if (ring < 0) {
-            ring = r;
-        }
+        if (!ring_busy_[r] && ring < 0) ring = r;
     }
-    if (ring < 0 || busy >= 2) return;
+    for (int st = 0; st < kSimStreams; st++) {
+        if (stream_busy_[st] && stream_busy_[st]->complete()) stream_busy_[st].reset();
+        if (!stream_busy_[st] && s < 0) s = st;
+    }
+    if (ring < 0 || s < 0) return;
 
-    // fair share: repeatedly the job with the least virtual time gives one item
+    // One kernel instance per launch (a stream runs its kernels one after the
+    // other, so two instances in one launch would serialise): the instance
+    // of the least-served job.  Then fair share within it: repeatedly the job
+    // with the least virtual time gives one item.
+    int best = -1;
+    for (size_t i = 0; i < active_jobs_.size(); i++) {
+        const SimJob& j = *active_jobs_[i];
+        if (j.next < j.items && (best < 0 || j.vtime < active_jobs_[size_t(best)]->vtime)) best = int(i);
+    }
+    if (best < 0) return;
+    const bool want_pssfp = active_jobs_[size_t(best)]->pssfp;
+    const int want_bucket = active_jobs_[size_t(best)]->bucket;
     using Key = std::pair<double, size_t>;
     std::priority_queue<Key, std::vector<Key>, std::greater<Key>> heap;
     for (size_t i = 0; i < active_jobs_.size(); i++) {
-        if (active_jobs_[i]->next < active_jobs_[i]->items) heap.push({active_jobs_[i]->vtime, i});
+        const SimJob& j = *active_jobs_[i];
+        if (j.next < j.items && j.pssfp == want_pssfp && j.bucket == want_bucket) heap.push({j.vtime, i});
     }
-    if (heap.empty()) return;
     std::vector<int> local(active_jobs_.size(), -1);  // job -> index among this launch's jobs
     std::vector<size_t> jobs;  // active_jobs_ indices in the launch
     std::vector<std::pair<size_t, int64_t>> picks;  // (launch job, item)
-    while (!heap.empty() && int(picks.size()) < kBatchWaves) {
+    const int max_items = batch_waves(want_pssfp);
+    while (!heap.empty() && int(picks.size()) < max_items) {
         const size_t i = heap.top().second;
         heap.pop();
         SimJob& j = *active_jobs_[i];
@@ -486,6 +517,12 @@ void Server::build_and_launch_batch() {
         j.vtime += j.item_cost;
         if (j.next < j.items) heap.push({j.vtime, i});
     }
+
+    // longest items first, so a launch ends with its short items and the
+    // next launch's waves fill the SIMDs its long items leave
+    std::stable_sort(picks.begin(), picks.end(), [&](const auto& x, const auto& y) {
+        return active_jobs_[jobs[x.first]]->item_cost > active_jobs_[jobs[y.first]]->item_cost;
+    });
 
     // table layout: per kernel instance (FISP states / pSSFP positions per
     // lane) its job descriptors, then its items
@@ -550,8 +587,6 @@ void Server::build_and_launch_batch() {
         }
     }
 
-    const int s = next_stream_;
-    next_stream_ ^= 1;
     const CompasContext& ctx = sim_[size_t(s)];
     ctx.activate();
     if (s != 0) COMPAS_HIP_CHECK(hipStreamWaitEvent(ctx.stream(), alloc_event_, 0));
@@ -569,6 +604,7 @@ void Server::build_and_launch_batch() {
     auto ev = std::make_shared<Event>();
     COMPAS_HIP_CHECK(hipEventRecord(ev->ev, ctx.stream()));
     ring_busy_[ring] = ev;
+    stream_busy_[s] = ev;
     for (size_t lj = 0; lj < jobs.size(); lj++) active_jobs_[jobs[lj]]->last[s] = ev;
 
     std::lock_guard<std::mutex> lock(mutex_);
