@@ -186,6 +186,51 @@ __device__ inline int staged_b_row(int tid) {
     return tid / MfmaTile<P, WM, WN, BKC, TM, TN>::B_TPR;
 }
 
+// The MFMAs of one staged K step: each wave multiplies its TM row tiles of
+// A by its TN column tiles of B out of LDS.
+template<int P, int WM, int WN, int BKC, int TM, int TN>
+__device__ __forceinline__ void mfma_tile_step(f32x16 (&acc)[TM][TN], const MfmaSmem<P, WM, WN, BKC, TM, TN>& sm,
+                                               int lane, int wm, int wn, uint32_t sel, uint32_t flip) {
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
+    using V = typename Half16<P>::V;
+    constexpr bool LO = Tl::LO;
+    constexpr int LD = Tl::LD;
+#pragma unroll
+    for (int kk = 0; kk < BKC / 8; kk++) {
+        const int koff = kk * 16 + (lane >> 5) * 8;
+        V ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+        for (int i = 0; i < TM; i++) {
+            const int r = wm * (32 * TM) + i * 32 + (lane & 31);
+            ah[i] = *reinterpret_cast<const V*>(sm.a_hi + r * LD + koff);
+            if constexpr (LO) al[i] = *reinterpret_cast<const V*>(sm.a_lo + r * LD + koff);
+        }
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            const int c = wn * (16 * TN) + j * 16 + (lane & 31) / 2;  // complex column of this lane
+            bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_hi + c * LD + koff), sel, flip);
+            if constexpr (LO) bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_lo + c * LD + koff), sel, flip);
+        }
+        // hi*hi, then hi*lo, then lo*hi over the accumulators: each
+        // accumulator's products stay in that order, and TM x TN independent
+        // MFMAs separate two that chain on one accumulator
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
+        if constexpr (LO) {
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
+        }
+    }
+}
+
 // acc += A_tile[.., kb:ke] * B_tile[.., kb:ke]^T for the workgroup's tile.
 // a_src / b_src: start of the calling thread's staged A row / B row (complex,
 // K-contiguous), nullptr for a row outside the operand (staged as zeros);
@@ -198,8 +243,6 @@ __device__ inline void mfma_accumulate(
     const cfloat* __restrict__ b_src, float sb, bool vec_b,
     int kb, int ke) {
     using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
-    using T = typename Half16<P>::T;
-    using V = typename Half16<P>::V;
     constexpr bool LO = Tl::LO;
     constexpr int LD = Tl::LD;
 
@@ -261,41 +304,7 @@ __device__ inline void mfma_accumulate(
         }
         __syncthreads();
         if (k0 + BKC < ke) gload(k0 + BKC);  // in flight during the MFMAs below
-
-#pragma unroll
-        for (int kk = 0; kk < BKC / 8; kk++) {
-            const int koff = kk * 16 + (lane >> 5) * 8;
-            V ah[TM], al[TM], bh[TN], bl[TN];
-#pragma unroll
-            for (int i = 0; i < TM; i++) {
-                const int r = wm * (32 * TM) + i * 32 + (lane & 31);
-                ah[i] = *reinterpret_cast<const V*>(sm.a_hi + r * LD + koff);
-                if constexpr (LO) al[i] = *reinterpret_cast<const V*>(sm.a_lo + r * LD + koff);
-            }
-#pragma unroll
-            for (int j = 0; j < TN; j++) {
-                const int c = wn * (16 * TN) + j * 16 + (lane & 31) / 2;  // complex column of this lane
-                bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_hi + c * LD + koff), sel, flip);
-                if constexpr (LO) bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_lo + c * LD + koff), sel, flip);
-            }
-            // hi*hi, then hi*lo, then lo*hi over the accumulators: each
-            // accumulator's products stay in that order, and TM x TN independent
-            // MFMAs separate two that chain on one accumulator
-#pragma unroll
-            for (int i = 0; i < TM; i++)
-#pragma unroll
-                for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
-            if constexpr (LO) {
-#pragma unroll
-                for (int i = 0; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
-#pragma unroll
-                for (int i = 0; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
-            }
-        }
+        mfma_tile_step<P, WM, WN, BKC, TM, TN>(acc, sm, lane, wm, wn, sel, flip);
         __syncthreads();
     }
 }
@@ -412,6 +421,105 @@ __device__ inline void mfma_accumulate_db(
         if (more) stage(sm[cur ^ 1]);  // the other buffer: last read before the previous barrier
         __syncthreads();
         cur ^= 1;
+    }
+}
+
+// A B operand generated while staging instead of read from memory: complex
+// column n = c ns + s (coil c, sample s) is, for voxel v,
+//   B[n][v] = C[c][v] T[s / 4][v] q[v]^(s % 4),
+// with T the column factor of every fourth sample and q the factor between
+// consecutive samples (the Cartesian coil-weighted sample factors).  ns is a
+// multiple of 4, so the four columns a thread generates share c and s / 4.
+struct GenB {
+    const cfloat* C;  // [nc][nv]
+    const cfloat* T;  // [ns / 4][nv]
+    const cfloat* q;  // [nv]
+    int ns, nv, N;  // N = nc ns complex columns
+};
+
+// mfma_accumulate with the B tile generated: thread t stages voxel t % BKC of
+// the K step for the four consecutive columns 4 (t / BKC) .. + 3 of the tile
+// (three loads, four complex multiplies), where the loaded form stages one
+// column for BKC / B_TPR voxels.  The A staging and the MFMA loop are those
+// of mfma_accumulate.  n0: first complex column of the tile.
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
+__device__ inline void mfma_accumulate_genb(
+    f32x16 (&acc)[TM][TN], MfmaSmem<P, WM, WN, BKC, TM, TN>& sm,
+    const cfloat* __restrict__ a_src, float sa, bool vec_a,
+    const GenB& g, int n0, int kb, int ke) {
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
+    constexpr bool LO = Tl::LO;
+    constexpr int LD = Tl::LD;
+    static_assert(Tl::THREADS / BKC * 4 == Tl::BNC, "four generated columns per thread");
+    static_assert(!is_fp16(P), "generated operands carry no per-row scales");
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int a_row = tid / Tl::A_TPR, a_k = (tid % Tl::A_TPR) * Tl::A_PER;
+    const uint32_t sel = (lane & 1) ? 0x01000302u : 0x03020100u;
+    const uint32_t flip = (lane & 1) ? 0u : 0x80000000u;
+
+    // generated columns of this thread: 4 gc .. 4 gc + 3 of the tile, voxel gk of each K step
+    const int gk = tid % BKC, gc = tid / BKC;
+    const int n = n0 + 4 * gc;
+    const bool col_ok = n < g.N;
+    const int coil = col_ok ? n / g.ns : 0, quad = col_ok ? (n % g.ns) / 4 : 0;
+    const cfloat* __restrict__ gC = g.C + int64_t(coil) * g.nv;
+    const cfloat* __restrict__ gT = g.T + int64_t(quad) * g.nv;
+
+    cfloat ra[Tl::A_PER];
+    cfloat rC, rT, rq;
+    auto gload = [&](int k0) {
+        const int ka = k0 + a_k;
+        if (vec_a && a_src && ka + Tl::A_PER <= ke) {
+#pragma unroll
+            for (int i = 0; i < Tl::A_PER / 2; i++) {
+                const float4 v = *reinterpret_cast<const float4*>(a_src + ka + 2 * i);
+                ra[2 * i] = {v.x, v.y};
+                ra[2 * i + 1] = {v.z, v.w};
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < Tl::A_PER; i++) ra[i] = (a_src && ka + i < ke) ? a_src[ka + i] : cfloat {0, 0};
+        }
+        const int v = k0 + gk;
+        if (col_ok && v < ke) {
+            rC = gC[v];
+            rT = gT[v];
+            rq = g.q[v];
+        } else {
+            rC = {0, 0};
+            rT = {0, 0};
+            rq = {0, 0};
+        }
+    };
+
+    if (kb < ke) gload(kb);
+    for (int k0 = kb; k0 < ke; k0 += BKC) {
+        {
+            uint32_t h[Tl::A_PER], l[Tl::A_PER];
+#pragma unroll
+            for (int i = 0; i < Tl::A_PER; i++) split_scaled<P>(ra[i], sa, h[i], l[i]);
+            store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_hi + a_row * LD + 2 * a_k), h);
+            if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_lo + a_row * LD + 2 * a_k), l);
+        }
+        {
+            cfloat w = dev::cmul(rC, rT);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint32_t h, l;
+                split_pair<P>(w.re, w.im, h, l);
+                const int row = 4 * gc + i;
+                reinterpret_cast<uint32_t*>(sm.b_hi + row * LD)[gk] = h;
+                if constexpr (LO) reinterpret_cast<uint32_t*>(sm.b_lo + row * LD)[gk] = l;
+                if (i < 3) w = dev::cmul(w, rq);
+            }
+        }
+        __syncthreads();
+        if (k0 + BKC < ke) gload(k0 + BKC);  // in flight during the MFMAs below
+        mfma_tile_step<P, WM, WN, BKC, TM, TN>(acc, sm, lane, wm, wn, sel, flip);
+        __syncthreads();
     }
 }
 

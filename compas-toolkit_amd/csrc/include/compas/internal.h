@@ -53,6 +53,14 @@ void complex_contraction(
     ContractionShape shape,
     ContractionPrecision precision);
 
+// complex_contraction with the B operand generated in the kernel's staging
+// instead of read: B[c ns + s][v] = C[c][v] T[s / 4][v] q[v]^(s % 4) with C
+// [N / ns][K], T [ns / 4][K], q [K] (ns a multiple of 4).  BF16 precisions
+// only: returns false, doing nothing, for the others.
+bool complex_contraction_generated(
+    const CompasContext& ctx, const cfloat* A, const cfloat* C, const cfloat* T, const cfloat* q, int ns,
+    cfloat* out, ContractionShape shape, ContractionPrecision precision);
+
 // MFMA tile variants: force one (v < 0 restores autotuning), count them, and
 // report the variant chosen per tuned shape.
 void set_contraction_variant(int v);

@@ -27,6 +27,37 @@ namespace kernels {
 
 using namespace mfma;
 
+// Epilogue of a workgroup's tile: D[row][realcol], realcol = 2n + {re, im},
+// to the output (direct, column n at (n / nsub) out_outer + m nsub + n % nsub)
+// or to split ks of the partial sums.  Scales: FP16 per-row scales, else nullptr.
+template<int WM, int WN, int TM, int TN>
+__device__ inline void store_tile(const f32x16 (&acc)[TM][TN], int m0, int n0, int ks, int M, int N,
+                                       const float* __restrict__ row_scale_a, const float* __restrict__ row_scale_b,
+                                       float2* __restrict__ partial, cfloat* __restrict__ direct, int nsub,
+                                       int64_t out_outer) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = wm * (32 * TM) + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                const int rc = wn * (32 * TN) + j * 32 + (lane & 31);
+                const int m = m0 + row, n = n0 + (rc >> 1), part = rc & 1;
+                if (m >= M || n >= N) continue;
+                float v = acc[i][j][e];
+                if (row_scale_a) v /= row_scale_a[m] * row_scale_b[n];
+                if (direct) {
+                    const int64_t o = int64_t(n / nsub) * out_outer + int64_t(m) * nsub + (n % nsub);
+                    reinterpret_cast<float*>(direct)[2 * o + part] = v;
+                } else {
+                    reinterpret_cast<float*>(partial)[2 * ((int64_t(ks) * M + m) * N + n) + part] = v;
+                }
+            }
+}
+
 template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
 __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const cfloat* __restrict__ A,
@@ -51,9 +82,6 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     const int ke = min(K, kb + k_chunk);
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave / WN, wn = wave % WN;
-
     f32x16 acc[TM][TN];
     zero(acc);
 
@@ -72,26 +100,37 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
                                         bn < N ? B + int64_t(bn) * ldb : nullptr, sb, vec_b, kb, ke);
     }
 
-    // ---- epilogue: D[row][realcol], realcol = 2n + {re, im}
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++)
-#pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const int row = wm * (32 * TM) + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                const int rc = wn * (32 * TN) + j * 32 + (lane & 31);
-                const int m = m0 + row, n = n0 + (rc >> 1), part = rc & 1;
-                if (m >= M || n >= N) continue;
-                float v = acc[i][j][e];
-                if (row_scale_a) v /= row_scale_a[m] * row_scale_b[n];
-                if (direct) {
-                    const int64_t o = int64_t(n / nsub) * out_outer + int64_t(m) * nsub + (n % nsub);
-                    reinterpret_cast<float*>(direct)[2 * o + part] = v;
-                } else {
-                    reinterpret_cast<float*>(partial)[2 * ((int64_t(ks) * M + m) * N + n) + part] = v;
-                }
-            }
+    store_tile<WM, WN>(acc, m0, n0, ks, M, N, row_scale_a, row_scale_b, partial, direct, nsub, out_outer);
+}
+
+// C = A B^T with B generated in staging (GenB); the 2 x 4 x 16 tile, BF16
+// precisions only.  Split-K and the output forms are those of contraction_mfma.
+template<int P>
+__global__ void __launch_bounds__(512) contraction_mfma_genb(
+    const cfloat* __restrict__ A, GenB g, int M, int K, int64_t lda,
+    int k_chunk, int tiles_m, int tiles_n, int splits,
+    float2* __restrict__ partial, cfloat* __restrict__ direct, int nsub, int64_t out_outer) {
+    constexpr int WM = 2, WN = 4, BKC = 16;
+    using Tl = MfmaTile<P, WM, WN, BKC>;
+    __shared__ MfmaSmem<P, WM, WN, BKC> sm;
+
+    const int total = tiles_m * tiles_n * splits;
+    const int lid = xcd_logical_id(blockIdx.x, total);
+    if (lid >= total) return;
+    const int tn = lid % tiles_n;
+    const int tm = (lid / tiles_n) % tiles_m;
+    const int ks = lid / (tiles_n * tiles_m);
+    const int m0 = tm * Tl::BM, n0 = tn * Tl::BNC;
+    const int kb = ks * k_chunk;
+    const int ke = min(K, kb + k_chunk);
+
+    f32x16 acc[2][2];
+    zero(acc);
+    const int am = m0 + staged_a_row<P, WM, WN, BKC>(threadIdx.x);
+    const bool vec_a = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 2 == 0);
+    mfma_accumulate_genb<P, WM, WN, BKC>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, 1.0f, vec_a, g, n0,
+                                         kb, ke);
+    store_tile<WM, WN>(acc, m0, n0, ks, M, g.N, nullptr, nullptr, partial, direct, nsub, out_outer);
 }
 
 // Per-row power of two s with max_k |x[row][k]| * s ~ 2^14 (FP16 operands).
@@ -174,15 +213,12 @@ int resident_per_cu() {
 // above a multiple of the slots leaves a nearly empty last round.  The split
 // count minimises rounds x K-chunk (the critical path), with a small charge
 // per split for the partial-sum traffic.
-template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
-Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
-    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
+Plan make_plan_for(const ContractionShape& s, int BM, int BNC, int BKC, int threads, int64_t slots) {
     Plan p;
-    p.threads = Tl::THREADS;
-    p.tiles_m = int(dev::div_ceil(s.M, Tl::BM));
-    p.tiles_n = int(dev::div_ceil(s.N, Tl::BNC));
+    p.threads = threads;
+    p.tiles_m = int(dev::div_ceil(s.M, BM));
+    p.tiles_n = int(dev::div_ceil(s.N, BNC));
     const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
-    const int64_t slots = int64_t(resident_per_cu<P, WM, WN, BKC, DB, TM, TN>()) * cu_count(ctx.device());
     const int max_splits = std::max(1, std::min(64, s.K / (8 * BKC)));
     double best_cost = 1e300;
     p.splits = 1;
@@ -199,6 +235,13 @@ Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
         }
     }
     return p;
+}
+
+template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
+Plan make_plan(const CompasContext& ctx, const ContractionShape& s) {
+    using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
+    const int64_t slots = int64_t(resident_per_cu<P, WM, WN, BKC, DB, TM, TN>()) * cu_count(ctx.device());
+    return make_plan_for(s, Tl::BM, Tl::BNC, BKC, Tl::THREADS, slots);
 }
 
 template<int P, int WM, int WN, int BKC, bool DB, int TM, int TN>
@@ -393,6 +436,56 @@ void complex_contraction_mfma(
     }
     const int v = pick_variant(ctx, A, B, s, P, sa, sb);
     VARIANTS[v].fn[P](ctx, A, B, out, s, sa, sb);
+}
+
+namespace {
+
+template<int P>
+void launch_genb(const CompasContext& ctx, const cfloat* A, const GenB& g, cfloat* out, const ContractionShape& s) {
+    using Tl = MfmaTile<P, 2, 4, 16>;
+    static const int resident = [] {
+        int n = 0;
+        COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, contraction_mfma_genb<P>, Tl::THREADS, 0));
+        return std::max(1, n);
+    }();
+    const Plan p = make_plan_for(s, Tl::BM, Tl::BNC, 16, Tl::THREADS, int64_t(resident) * cu_count(ctx.device()));
+    const int total = p.tiles_m * p.tiles_n * p.splits;
+    const bool direct = (p.splits == 1 && !s.accumulate);
+    Array<float2, 1> partial;
+    if (!direct) partial = ctx.empty<float2, 1>({int64_t(p.splits) * s.M * s.N});
+    {
+        KernelScope ks(ctx, "contraction_mfma");
+        contraction_mfma_genb<P><<<total, p.threads, 0, ctx.stream()>>>(
+            A, g, s.M, s.K, s.lda, p.k_chunk, p.tiles_m, p.tiles_n, p.splits,
+            direct ? nullptr : partial.data(), direct ? out : nullptr, s.nsub, s.out_outer);
+        check_launch("contraction_mfma");
+    }
+    if (!direct) {
+        KernelScope ks(ctx, "contraction_reduce");
+        const int64_t n = int64_t(s.M) * s.N;
+        contraction_reduce_mf<<<dev::div_ceil(n, 256), 256, 0, ctx.stream()>>>(
+            partial.data(), p.splits, s.M, s.N, out, s.nsub, s.out_outer, s.accumulate);
+        check_launch("contraction_reduce");
+    }
+}
+
+}  // namespace
+
+bool complex_contraction_generated(
+    const CompasContext& ctx, const cfloat* A, const cfloat* C, const cfloat* T, const cfloat* q, int ns,
+    cfloat* out, ContractionShape s, ContractionPrecision precision) {
+    precision = effective_precision(precision);
+    if (precision != ContractionPrecision::BF16x3 && precision != ContractionPrecision::BF16x1) return false;
+    COMPAS_CHECK(ns > 0 && ns % 4 == 0 && s.N % ns == 0);
+    COMPAS_CHECK(s.M >= 0 && s.K >= 0 && s.nsub > 0);
+    if (s.M == 0 || s.N == 0) return true;
+    const GenB g {C, T, q, ns, s.K, s.N};
+    if (precision == ContractionPrecision::BF16x3) {
+        launch_genb<P_BF16X3>(ctx, A, g, out, s);
+    } else {
+        launch_genb<P_BF16X1>(ctx, A, g, out, s);
+    }
+    return true;
 }
 
 }  // namespace compas

@@ -10,11 +10,18 @@
 //   Theta_v = dk_re x + dk_im y + dt 2 pi B0,
 // so the raw echoes are the A operand of one contraction (M = nr, N = nc*ns,
 // K = nv) and the coil dimension is folded into N.  phi is formed in FP64 and
-// reduced to [-pi, pi) before the float sincos.
+// reduced to [-pi, pi) before the float sincos.  For the BF16 precisions W is not stored:
+// the contraction generates it while staging from W[c ns + 4a + b][v] =
+// C[c][v] T[a][v] q[v]^b (T: the coil-free weight of every fourth sample, q
+// = exp(-dt R2) exp(i Theta_v)), which replaces nc ns rows of W in HBM by
+// ns / 4 + 1 rows of factors.
 //
 // Spiral:  S[c][r][s] = sum_v A[r][v] C[c][v] E_rv^s with per-readout
 // exponents, evaluated by a direct kernel: FP64-anchored powers E^s0 every 8
 // samples, complex recurrence in between.
+
+#include <cstdlib>
+#include <string>
 
 #include "compas/device/common.h"
 #include "compas/internal.h"
@@ -39,6 +46,30 @@ __global__ void signal_cartesian_weights(
     for (int c = 0; c < nc; c++) {
         W[(size_t(c) * ns + s) * nv + v] = dev::cmul(e, coils[size_t(c) * nv + v]);
     }
+}
+
+// The factors of W in the form the contraction generates it from
+// (W[c ns + s][v] = C[c][v] T[s / 4][v] q[v]^(s % 4)): rows a < ns / 4 of T
+// are the coil-free weight at sample 4a, row ns / 4 is the ratio q of two
+// consecutive samples, exp(-dt R2) exp(i Theta_v).
+__global__ void signal_cartesian_anchors(
+    dev::TissueView tissue, int ns, float dt, float dk_re, float dk_im, cfloat* __restrict__ T) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    const int nv = tissue.nvoxels;
+    if (v >= nv) return;
+    const float R2 = 1.0f / tissue.T2(v);
+    const double B0 = tissue.B0(v), x = tissue.x(v), y = tissue.y(v);
+    const double off = double(dt) * dev::TWO_PI * B0;
+    const double theta = double(dk_re) * x + double(dk_im) * y + off;
+    if (a == ns / 4) {
+        T[size_t(a) * nv + v] = dev::polar_f(expf(-dt * R2), dev::reduce_phase(theta));
+        return;
+    }
+    const int s = 4 * a;
+    const double phi = double(s) * theta - 0.5 * double(ns) * (off + double(dk_re) * x);
+    const float mag = expf((0.5f * float(ns) - float(s)) * dt * R2);
+    T[size_t(a) * nv + v] = dev::cmul(dev::polar_f(mag, dev::reduce_phase(phi)), tissue.rho(v));
 }
 
 // A[r][v] = m[r][v] * rewind * prephaser * rho (spiral_view.cuh:13-57, signal_kernels.cuh:12-30)
@@ -197,6 +228,16 @@ static void check_signal_shapes(const Array<cfloat, 2>& echos, const TissueParam
     COMPAS_CHECK(t.samples_per_readout >= 0);
 }
 
+// The Cartesian weights are generated inside the contraction unless
+// COMPAS_CARTESIAN_WEIGHTS=stored asks for the stored form (for comparisons).
+static bool generated_weights() {
+    static const bool on = [] {
+        const char* e = std::getenv("COMPAS_CARTESIAN_WEIGHTS");
+        return !(e && std::string(e) == "stored");
+    }();
+    return on;
+}
+
 static Array<cfloat, 3> signal_cartesian(
     const CompasContext& ctx, const Array<cfloat, 2>& echos, const TissueParameters& p,
     const CartesianTrajectory& t, const Array<cfloat, 2>& coils, SimulateSignalMethod method) {
@@ -207,6 +248,25 @@ static Array<cfloat, 3> signal_cartesian(
         ctx.fill_zero(signal.data(), signal.nbytes());
         return signal;
     }
+    ContractionShape s {nr, nc * ns, nv, nv, nv, ns, int64_t(nr) * ns, false};
+    const ContractionPrecision prec = effective_precision(signal_precision(method));
+    if ((prec == ContractionPrecision::BF16x3 || prec == ContractionPrecision::BF16x1) && ns % 4 == 0 &&
+        generated_weights()) {
+        // W generated in the contraction's staging from ns / 4 + 1 rows of
+        // factors instead of written to HBM (nc ns rows)
+        auto T = ctx.empty<cfloat, 2>({ns / 4 + 1, nv});
+        {
+            KernelScope ks(ctx, "signal_cartesian_anchors");
+            dim3 grid(dev::div_ceil(nv, 256), ns / 4 + 1);
+            kernels::signal_cartesian_anchors<<<grid, 256, 0, ctx.stream()>>>(
+                dev::tissue_view(p), ns, t.delta_t, t.delta_k.re, t.delta_k.im, T.data());
+            check_launch("signal_cartesian_anchors");
+        }
+        const bool done = complex_contraction_generated(ctx, echos.data(), coils.data(), T.data(),
+                                                        T.data() + size_t(ns / 4) * nv, ns, signal.data(), s, prec);
+        COMPAS_CHECK(done);
+        return signal;
+    }
     auto W = ctx.empty<cfloat, 2>({int64_t(nc) * ns, nv});
     {
         KernelScope ks(ctx, "signal_cartesian_weights");
@@ -215,7 +275,6 @@ static Array<cfloat, 3> signal_cartesian(
             dev::tissue_view(p), coils.data(), nc, ns, t.delta_t, t.delta_k.re, t.delta_k.im, W.data());
         check_launch("signal_cartesian_weights");
     }
-    ContractionShape s {nr, nc * ns, nv, nv, nv, ns, int64_t(nr) * ns, false};
     complex_contraction(ctx, echos.data(), W.data(), signal.data(), s, signal_precision(method));
     return signal;
 }

@@ -256,6 +256,32 @@ def test_contraction_ragged_shapes(ctx, oracle, M, N, K):
         assert rel(got, want) < tol, method
 
 
+@pytest.mark.parametrize("M,nc,ns,K", [(1, 1, 4, 1), (5, 3, 8, 7), (129, 2, 12, 33), (261, 5, 28, 700),
+                                        (200, 1, 64, 1000), (70, 7, 132, 4099)])
+def test_cartesian_generated_weights_ragged(ctx, oracle, M, nc, ns, K):
+    # samples per readout a multiple of 4: the BF16 methods generate the weights
+    # C[c][v] T[s / 4][v] q[v]^(s % 4) in the contraction's staging; ragged
+    # readouts, coils and voxels against FP64 and against the FP32 VALU path
+    rng = W.SplitMix64(M * 7919 + nc * 31 + ns * 3 + K)
+    n = int(np.ceil(np.sqrt(K)))
+    t = W.make_tissue(rng, n, 20.0, 30.0).slice(0, K)
+    k0 = ((rng.normal(M) - 0.5) * 0.2 + 1j * (rng.normal(M) - 0.5) * 0.2).astype(np.complex64)
+    traj = W.CartesianTraj(M, ns, 5e-6, k0, float(np.float32(2 * np.pi / 20.0)))
+    coils = ((rng.normal(nc * K) + 1j * rng.normal(nc * K)) / 3).reshape(nc, K).astype(np.complex64)
+    echos = ((rng.normal(M * K) + 1j * rng.normal(M * K)) / 3).reshape(M, K).astype(np.complex64)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    ctraj = ca.CartesianTrajectory.from_trajectory(traj)
+    want = oracle.signal_cartesian(t, traj, coils, echos, "f64")
+    plain = ca.magnetization_to_signal(echos, params, ctraj, coils, method="direct").collect()
+    assert rel(plain, want) < 1e-5
+    for method, tol in (("matmul_fast", 1e-4), ("matmul", 1e-4), ("matmul_low", 2e-2)):
+        got = ca.magnetization_to_signal(echos, params, ctraj, coils, method=method).collect()
+        assert got.shape == (nc, M, ns)
+        assert rel(got, want) < tol, method
+        assert rel(got, plain) < tol, method
+
+
 def test_every_mfma_variant(ctx, oracle):
     # each tile variant the autotuner may pick, on a ragged shape, every precision
     rng = W.SplitMix64(77)
