@@ -860,6 +860,20 @@ def serve_bench(args):
     alone.close()
 
     counts = {k: sum(1 for sp in specs if sp["kind"] == k) for k in ("fisp", "pssfp", "jhy")}
+
+    def pct(idx):
+        la = np.array([stamps[i][2] - stamps[i][0] for i in idx])
+        qu = np.array([stamps[i][1] - stamps[i][0] for i in idx])
+        return {"n": len(idx), "p50_ms": 1e3 * float(np.percentile(la, 50)),
+                "p99_ms": 1e3 * float(np.percentile(la, 99)), "queue_p99_ms": 1e3 * float(np.percentile(qu, 99))}
+
+    done_ids = [i for i in range(n_req) if stamps[i] is not None]
+    by_kind = {k: pct([i for i in done_ids if specs[i]["kind"] == k]) for k in counts if counts[k]}
+    work = np.array([specs[i]["nv"] * specs[i]["nTR"] for i in done_ids], dtype=np.float64)
+    edges = np.percentile(work, [25, 50, 75]) if len(work) else []
+    quart = np.searchsorted(edges, work, side="right") if len(work) else []
+    by_size = {f"q{q + 1}": pct([i for i, qq in zip(done_ids, quart) if qq == q]) for q in range(4)
+               if any(qq == q for qq in quart)}
     line = {
         "metric": "serving: aggregate voxel*TR/s and k-space samples/s with p50/p99 latency (arrival -> result in HBM)",
         "value": vox_tr / span,
@@ -876,6 +890,8 @@ def serve_bench(args):
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99)),
                        "mean": 1e3 * float(lat.mean())},
         "queue_ms": {"p50": 1e3 * float(np.percentile(queue, 50)), "p99": 1e3 * float(np.percentile(queue, 99))},
+        "latency_by_kind": by_kind,
+        "latency_by_size": dict(by_size, note="quartiles of voxels x TRs per request"),
         "arrival_rate_per_s": lam,
         "capacity_per_s": capacity,
         "calibration": f"{len(cal_ids)} requests submitted at once: {cal_span:.2f} s",

@@ -129,7 +129,7 @@ struct MfmaTile {
     static constexpr int THREADS = 64 * WM * WN;
     static constexpr int BM = 32 * TM * WM;  // A rows
     static constexpr int BNC = 16 * TN * WN;  // complex B columns
-    static constexpr int LD = 2 * BKC + 8;  // LDS row stride, 16-bit elements (+16 bytes of padding)
+    static constexpr int LD = 2 * BKC;  // LDS row stride, 16-bit elements (no padding: rows are swizzled)
     static constexpr bool LO = has_lo(P);
     static constexpr int A_TPR = THREADS / BM;  // threads per A row
     static constexpr int A_PER = BKC / A_TPR;  // complex per thread per step (contiguous in K)
@@ -139,6 +139,19 @@ struct MfmaTile {
     static_assert(THREADS % BM == 0 && THREADS % BNC == 0, "staging rows");
     static_assert(A_PER >= 4 && A_PER % 4 == 0, "A staging granularity");
     static_assert(B_PER >= 2 && B_PER % 2 == 0, "B staging granularity");
+    static_assert(BKC == 16 || BKC == 32, "row swizzle");
+
+    // Offset, in 32-bit words, of word w (the packed pair of complex K index
+    // w) of LDS row r.  The 16-byte chunks of a row are permuted by the Gray
+    // code of the row's 128-byte line, so that a fragment read (one chunk of
+    // each of 8 or 16 consecutive rows) and a staging write (whole chunks of
+    // 2 or 4 consecutive rows) fall on distinct banks, whether the LDS serves
+    // 128 or 256 bytes per cycle.
+    __device__ static int word(int r, int w) {
+        const int line = r / (32 / BKC);
+        const int f = (line ^ (line >> 1)) & (BKC / 4 - 1);
+        return r * BKC + (((w >> 2) ^ f) << 2) + (w & 3);
+    }
 };
 
 // LDS of one workgroup: packed hi (and lo) parts of the A and B tiles
@@ -152,16 +165,25 @@ struct MfmaSmem {
     __attribute__((aligned(16))) T b_lo[Tl::LO ? Tl::BNC * Tl::LD : 8];
 };
 
-template<int N>
-__device__ inline void store_words(uint32_t* dst, const uint32_t* w) {
+// Stores the N packed words w[] of LDS row r from word w0 on (whole chunks
+// when N is a multiple of 4, else pairs within one chunk).
+template<typename Tl, int N>
+__device__ inline void store_row(void* plane, int r, int w0, const uint32_t* w) {
+    uint32_t* p = reinterpret_cast<uint32_t*>(plane);
     if constexpr (N % 4 == 0) {
 #pragma unroll
         for (int i = 0; i < N / 4; i++)
-            reinterpret_cast<uint4*>(dst)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+            *reinterpret_cast<uint4*>(p + Tl::word(r, w0 + 4 * i)) = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
     } else {
 #pragma unroll
-        for (int i = 0; i < N / 2; i++) reinterpret_cast<uint2*>(dst)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+        for (int i = 0; i < N / 2; i++) *reinterpret_cast<uint2*>(p + Tl::word(r, w0 + 2 * i)) = make_uint2(w[2 * i], w[2 * i + 1]);
     }
+}
+
+// A 16-byte fragment of LDS row r at 16-bit offset k (a multiple of 8)
+template<typename Tl, typename V, typename T>
+__device__ inline V load_frag(const T* plane, int r, int k) {
+    return *reinterpret_cast<const V*>(reinterpret_cast<const uint32_t*>(plane) + Tl::word(r, k >> 1));
 }
 
 // The B operand row of a lane from the packed (re, im) words of its complex
@@ -194,7 +216,6 @@ __device__ __forceinline__ void mfma_tile_step(f32x16 (&acc)[TM][TN], const Mfma
     using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
     using V = typename Half16<P>::V;
     constexpr bool LO = Tl::LO;
-    constexpr int LD = Tl::LD;
 #pragma unroll
     for (int kk = 0; kk < BKC / 8; kk++) {
         const int koff = kk * 16 + (lane >> 5) * 8;
@@ -202,14 +223,14 @@ __device__ __forceinline__ void mfma_tile_step(f32x16 (&acc)[TM][TN], const Mfma
 #pragma unroll
         for (int i = 0; i < TM; i++) {
             const int r = wm * (32 * TM) + i * 32 + (lane & 31);
-            ah[i] = *reinterpret_cast<const V*>(sm.a_hi + r * LD + koff);
-            if constexpr (LO) al[i] = *reinterpret_cast<const V*>(sm.a_lo + r * LD + koff);
+            ah[i] = load_frag<Tl, V>(sm.a_hi, r, koff);
+            if constexpr (LO) al[i] = load_frag<Tl, V>(sm.a_lo, r, koff);
         }
 #pragma unroll
         for (int j = 0; j < TN; j++) {
             const int c = wn * (16 * TN) + j * 16 + (lane & 31) / 2;  // complex column of this lane
-            bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_hi + c * LD + koff), sel, flip);
-            if constexpr (LO) bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(sm.b_lo + c * LD + koff), sel, flip);
+            bh[j] = b_row_form_t<V>(load_frag<Tl, uint4>(sm.b_hi, c, koff), sel, flip);
+            if constexpr (LO) bl[j] = b_row_form_t<V>(load_frag<Tl, uint4>(sm.b_lo, c, koff), sel, flip);
         }
         // hi*hi, then hi*lo, then lo*hi over the accumulators: each
         // accumulator's products stay in that order, and TM x TN independent
@@ -244,7 +265,6 @@ __device__ inline void mfma_accumulate(
     int kb, int ke) {
     using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
     constexpr bool LO = Tl::LO;
-    constexpr int LD = Tl::LD;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -291,16 +311,16 @@ __device__ inline void mfma_accumulate(
             uint32_t h[Tl::A_PER], l[Tl::A_PER];
 #pragma unroll
             for (int i = 0; i < Tl::A_PER; i++) split_scaled<P>(ra[i], sa, h[i], l[i]);
-            store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_hi + a_row * LD + 2 * a_k), h);
-            if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_lo + a_row * LD + 2 * a_k), l);
+            store_row<Tl, Tl::A_PER>(sm.a_hi, a_row, a_k, h);
+            if constexpr (LO) store_row<Tl, Tl::A_PER>(sm.a_lo, a_row, a_k, l);
         }
         // ---- stage B: (re, im) pairs, one row per complex column
         {
             uint32_t h[Tl::B_PER], l[Tl::B_PER];
 #pragma unroll
             for (int i = 0; i < Tl::B_PER; i++) split_scaled<P>(rb[i], sb, h[i], l[i]);
-            store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_hi + b_row * LD + 2 * b_k), h);
-            if constexpr (LO) store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(sm.b_lo + b_row * LD + 2 * b_k), l);
+            store_row<Tl, Tl::B_PER>(sm.b_hi, b_row, b_k, h);
+            if constexpr (LO) store_row<Tl, Tl::B_PER>(sm.b_lo, b_row, b_k, l);
         }
         __syncthreads();
         if (k0 + BKC < ke) gload(k0 + BKC);  // in flight during the MFMAs below
@@ -320,9 +340,7 @@ __device__ inline void mfma_accumulate_db(
     const cfloat* __restrict__ b_src, float sb, bool vec_b,
     int kb, int ke) {
     using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
-    using V = typename Half16<P>::V;
     constexpr bool LO = Tl::LO;
-    constexpr int LD = Tl::LD;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -363,50 +381,16 @@ __device__ inline void mfma_accumulate_db(
         uint32_t h[Tl::A_PER], l[Tl::A_PER];
 #pragma unroll
         for (int i = 0; i < Tl::A_PER; i++) split_scaled<P>(ra[i], sa, h[i], l[i]);
-        store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(buf.a_hi + a_row * LD + 2 * a_k), h);
-        if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(buf.a_lo + a_row * LD + 2 * a_k), l);
+        store_row<Tl, Tl::A_PER>(buf.a_hi, a_row, a_k, h);
+        if constexpr (LO) store_row<Tl, Tl::A_PER>(buf.a_lo, a_row, a_k, l);
         uint32_t hb[Tl::B_PER], lb[Tl::B_PER];
 #pragma unroll
         for (int i = 0; i < Tl::B_PER; i++) split_scaled<P>(rb[i], sb, hb[i], lb[i]);
-        store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_hi + b_row * LD + 2 * b_k), hb);
-        if constexpr (LO) store_words<Tl::B_PER>(reinterpret_cast<uint32_t*>(buf.b_lo + b_row * LD + 2 * b_k), lb);
+        store_row<Tl, Tl::B_PER>(buf.b_hi, b_row, b_k, hb);
+        if constexpr (LO) store_row<Tl, Tl::B_PER>(buf.b_lo, b_row, b_k, lb);
     };
     auto compute = [&](const MfmaSmem<P, WM, WN, BKC, TM, TN>& buf) {
-#pragma unroll
-        for (int kk = 0; kk < BKC / 8; kk++) {
-            const int koff = kk * 16 + (lane >> 5) * 8;
-            V ah[TM], al[TM], bh[TN], bl[TN];
-#pragma unroll
-            for (int i = 0; i < TM; i++) {
-                const int r = wm * (32 * TM) + i * 32 + (lane & 31);
-                ah[i] = *reinterpret_cast<const V*>(buf.a_hi + r * LD + koff);
-                if constexpr (LO) al[i] = *reinterpret_cast<const V*>(buf.a_lo + r * LD + koff);
-            }
-#pragma unroll
-            for (int j = 0; j < TN; j++) {
-                const int c = wn * (16 * TN) + j * 16 + (lane & 31) / 2;
-                bh[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(buf.b_hi + c * LD + koff), sel, flip);
-                if constexpr (LO)
-                    bl[j] = b_row_form_t<V>(*reinterpret_cast<const uint4*>(buf.b_lo + c * LD + koff), sel, flip);
-            }
-            // hi*hi, then hi*lo, then lo*hi over the accumulators: each
-            // accumulator's products stay in that order, and TM x TN independent
-            // MFMAs separate two that chain on one accumulator
-#pragma unroll
-            for (int i = 0; i < TM; i++)
-#pragma unroll
-                for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bh[j], acc[i][j]);
-            if constexpr (LO) {
-#pragma unroll
-                for (int i = 0; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(ah[i], bl[j], acc[i][j]);
-#pragma unroll
-                for (int i = 0; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++) acc[i][j] = mfma<P>(al[i], bh[j], acc[i][j]);
-            }
-        }
+        mfma_tile_step<P, WM, WN, BKC, TM, TN>(acc, buf, lane, wm, wn, sel, flip);
     };
 
     if (kb >= ke) return;  // uniform over the workgroup
@@ -438,10 +422,13 @@ struct GenB {
 };
 
 // mfma_accumulate with the B tile generated: thread t stages voxel t % BKC of
-// the K step for the four consecutive columns 4 (t / BKC) .. + 3 of the tile
-// (three loads, four complex multiplies), where the loaded form stages one
-// column for BKC / B_TPR voxels.  The A staging and the MFMA loop are those
-// of mfma_accumulate.  n0: first complex column of the tile.
+// the K step for the four consecutive columns 4 gc .. 4 gc + 3 of the tile,
+// gc = t / BKC (three loads, four complex multiplies), where the loaded form
+// stages one column for BKC / B_TPR voxels.  Column 4 gc + i goes to LDS row
+// gc + G i (G = THREADS / BKC), so that one store instruction covers
+// consecutive rows: the caller's epilogue maps LDS row rho back to column
+// 4 (rho % G) + rho / G.  The A staging and the MFMA loop are those of
+// mfma_accumulate.  n0: first complex column of the tile.
 template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
 __device__ inline void mfma_accumulate_genb(
     f32x16 (&acc)[TM][TN], MfmaSmem<P, WM, WN, BKC, TM, TN>& sm,
@@ -449,8 +436,8 @@ __device__ inline void mfma_accumulate_genb(
     const GenB& g, int n0, int kb, int ke) {
     using Tl = MfmaTile<P, WM, WN, BKC, TM, TN>;
     constexpr bool LO = Tl::LO;
-    constexpr int LD = Tl::LD;
-    static_assert(Tl::THREADS / BKC * 4 == Tl::BNC, "four generated columns per thread");
+    constexpr int G = Tl::THREADS / BKC;  // column groups of four
+    static_assert(4 * G == Tl::BNC, "four generated columns per thread");
     static_assert(!is_fp16(P), "generated operands carry no per-row scales");
 
     const int tid = threadIdx.x;
@@ -501,8 +488,8 @@ __device__ inline void mfma_accumulate_genb(
             uint32_t h[Tl::A_PER], l[Tl::A_PER];
 #pragma unroll
             for (int i = 0; i < Tl::A_PER; i++) split_scaled<P>(ra[i], sa, h[i], l[i]);
-            store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_hi + a_row * LD + 2 * a_k), h);
-            if constexpr (LO) store_words<Tl::A_PER>(reinterpret_cast<uint32_t*>(sm.a_lo + a_row * LD + 2 * a_k), l);
+            store_row<Tl, Tl::A_PER>(sm.a_hi, a_row, a_k, h);
+            if constexpr (LO) store_row<Tl, Tl::A_PER>(sm.a_lo, a_row, a_k, l);
         }
         {
             cfloat w = dev::cmul(rC, rT);
@@ -510,9 +497,9 @@ __device__ inline void mfma_accumulate_genb(
             for (int i = 0; i < 4; i++) {
                 uint32_t h, l;
                 split_pair<P>(w.re, w.im, h, l);
-                const int row = 4 * gc + i;
-                reinterpret_cast<uint32_t*>(sm.b_hi + row * LD)[gk] = h;
-                if constexpr (LO) reinterpret_cast<uint32_t*>(sm.b_lo + row * LD)[gk] = l;
+                const int row = gc + G * i;  // column 4 gc + i
+                reinterpret_cast<uint32_t*>(sm.b_hi)[Tl::word(row, gk)] = h;
+                if constexpr (LO) reinterpret_cast<uint32_t*>(sm.b_lo)[Tl::word(row, gk)] = l;
                 if (i < 3) w = dev::cmul(w, rq);
             }
         }

@@ -154,10 +154,11 @@ void launch_pssfp_table(
 
 // Finite-difference pieces of simulate_magnetization_derivative
 // (derivative.cu:11-94): the parameters with `field` moved by `delta` (the
-// presence flags kept), and (next - echos) / delta.
+// presence flags kept), and (next - echos) / delta, written over next (which
+// is returned: the difference needs no memory beyond the perturbed echoes).
 TissueParameters perturbed_parameters(const CompasContext& ctx, const TissueParameters& p, int field, float delta);
 Array<cfloat, 2> finite_difference(
-    const CompasContext& ctx, const Array<cfloat, 2>& echos, const Array<cfloat, 2>& next, float delta);
+    const CompasContext& ctx, const Array<cfloat, 2>& echos, Array<cfloat, 2> next, float delta);
 
 ContractionPrecision signal_precision(SimulateSignalMethod method);
 ContractionPrecision jacobian_precision(JacobianComputeMethod method);

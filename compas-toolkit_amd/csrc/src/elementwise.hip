@@ -30,11 +30,11 @@ __global__ void add_difference_to_parameters(
     out[size_t(field) * nv + v] += delta;
 }
 
-__global__ void finite_difference(
-    const cfloat* __restrict__ e0, const cfloat* __restrict__ e1, int64_t n, float inv_delta, cfloat* __restrict__ out) {
+// e1 <- (e1 - e0) / delta, in place
+__global__ void finite_difference(const cfloat* __restrict__ e0, cfloat* e1, int64_t n, float inv_delta) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
         const cfloat a = e0[i], b = e1[i];
-        out[i] = {(b.re - a.re) * inv_delta, (b.im - a.im) * inv_delta};
+        e1[i] = {(b.re - a.re) * inv_delta, (b.im - a.im) * inv_delta};
     }
 }
 
@@ -147,18 +147,18 @@ TissueParameters perturbed_parameters(const CompasContext& ctx, const TissuePara
 }
 
 Array<cfloat, 2> finite_difference(
-    const CompasContext& ctx, const Array<cfloat, 2>& echos, const Array<cfloat, 2>& next, float delta) {
+    const CompasContext& ctx, const Array<cfloat, 2>& echos, Array<cfloat, 2> next, float delta) {
     ctx.activate();
     COMPAS_CHECK(next.size(0) == echos.size(0) && next.size(1) == echos.size(1));
-    auto out = ctx.empty<cfloat, 2>({echos.size(0), echos.size(1)});
-    const int64_t n = out.numel();
+    COMPAS_CHECK(next.data() != echos.data() || next.numel() == 0);
+    const int64_t n = next.numel();
     if (n > 0) {
         KernelScope ks(ctx, "finite_difference");
-        kernels::finite_difference<<<grid_for(n), 256, 0, ctx.stream()>>>(
-            echos.data(), next.data(), n, 1.0f / delta, out.data());
+        kernels::finite_difference<<<grid_for(n), 256, 0, ctx.stream()>>>(echos.data(), next.data(), n,
+                                                                           1.0f / delta);
         check_launch("finite_difference");
     }
-    return out;
+    return next;
 }
 
 template<typename Sequence>

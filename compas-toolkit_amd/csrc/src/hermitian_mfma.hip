@@ -36,12 +36,25 @@ __device__ inline float2 conj_factor(cfloat z, int part) {
     return part ? make_float2(z.im, z.re) : make_float2(z.re, -z.im);
 }
 
+// Per-lane readout sums of the epilogue (lane = 2 g + part for voxel g of the
+// wave's 16, coil c): G conj(m), G conj(dm/dT1), G conj(dm/dT2), G' conj(m).
+// Record (wave (w, ow), part, g) starts at red_index; records are padded to
+// 8 NCP + 4 floats and the part planes offset by 16 more, so that the lanes'
+// 16-byte stores and the voxel threads' loads spread over the banks.
+template<int NCP>
+struct RedLayout {
+    static constexpr int RS = 8 * NCP + 4;  // floats per record
+    static constexpr int PLANE = 16 * RS + 16;  // floats per (wave, part) plane
+};
+template<int WN, int NCP>
+__device__ inline int red_index(int w, int ow, int g, int part) {
+    return ((w * WN + ow) * 2 + part) * RedLayout<NCP>::PLANE + g * RedLayout<NCP>::RS;
+}
+
 template<int P, int WM, int WN, int BKC, int NCP>
 union HermitianSmem {
     MfmaSmem<P, WM, WN, BKC> tile;
-    // per-lane readout sums (lane = 2 voxel + part, coil): G conj(m), G conj(dm/dT1),
-    // G conj(dm/dT2), G' conj(m)
-    float red[WM][WN][32][NCP][8];
+    __attribute__((aligned(16))) float red[WM * WN * 2 * RedLayout<NCP>::PLANE];
 };
 
 // Readout tiles per group of the launch order: a group sweeps every voxel tile
@@ -155,10 +168,11 @@ __global__ void __launch_bounds__(64 * WM * WN, 4) hermitian_mfma(
 #pragma unroll
         for (int k = 0; k < 8; k++) s[c][k] += __shfl_xor(s[c][k], 32);
     if (lane < 32) {
+        float* rec = sm.red + red_index<WN, NCP>(wm, wn, lane >> 1, lane & 1);
 #pragma unroll
         for (int c = 0; c < NCP; c++)
 #pragma unroll
-            for (int k = 0; k < 8; k++) sm.red[wm][wn][lane][c][k] = s[c][k];
+            for (int k = 0; k < 8; k++) rec[8 * c + k] = s[c][k];
     }
     __syncthreads();
 
@@ -175,7 +189,7 @@ __global__ void __launch_bounds__(64 * WM * WN, 4) hermitian_mfma(
         for (int w = 0; w < WM; w++)
 #pragma unroll
             for (int pt = 0; pt < 2; pt++) {
-                const float* e = sm.red[w][ow][2 * g + pt][c];
+                const float* e = sm.red + red_index<WN, NCP>(w, ow, g, pt) + 8 * c;
 #pragma unroll
                 for (int k = 0; k < 8; k++) t[k] += e[k];
             }

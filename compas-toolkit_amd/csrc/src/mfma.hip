@@ -29,8 +29,10 @@ using namespace mfma;
 
 // Epilogue of a workgroup's tile: D[row][realcol], realcol = 2n + {re, im},
 // to the output (direct, column n at (n / nsub) out_outer + m nsub + n % nsub)
-// or to split ks of the partial sums.  Scales: FP16 per-row scales, else nullptr.
-template<int WM, int WN, int TM, int TN>
+// or to split ks of the partial sums.  Scales: FP16 per-row scales, else
+// nullptr.  GPERM > 0: tile column rho holds column 4 (rho % GPERM) + rho /
+// GPERM (the generated operand's store order).
+template<int WM, int WN, int GPERM = 0, int TM, int TN>
 __device__ inline void store_tile(const f32x16 (&acc)[TM][TN], int m0, int n0, int ks, int M, int N,
                                        const float* __restrict__ row_scale_a, const float* __restrict__ row_scale_b,
                                        float2* __restrict__ partial, cfloat* __restrict__ direct, int nsub,
@@ -45,7 +47,8 @@ __device__ inline void store_tile(const f32x16 (&acc)[TM][TN], int m0, int n0, i
             for (int e = 0; e < 16; e++) {
                 const int row = wm * (32 * TM) + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
                 const int rc = wn * (32 * TN) + j * 32 + (lane & 31);
-                const int m = m0 + row, n = n0 + (rc >> 1), part = rc & 1;
+                const int rho = rc >> 1, part = rc & 1;
+                const int m = m0 + row, n = n0 + (GPERM ? 4 * (rho % GPERM) + rho / GPERM : rho);
                 if (m >= M || n >= N) continue;
                 float v = acc[i][j][e];
                 if (row_scale_a) v /= row_scale_a[m] * row_scale_b[n];
@@ -130,7 +133,8 @@ __global__ void __launch_bounds__(512) contraction_mfma_genb(
     const bool vec_a = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 2 == 0);
     mfma_accumulate_genb<P, WM, WN, BKC>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, 1.0f, vec_a, g, n0,
                                          kb, ke);
-    store_tile<WM, WN>(acc, m0, n0, ks, M, g.N, nullptr, nullptr, partial, direct, nsub, out_outer);
+    store_tile<WM, WN, Tl::THREADS / BKC>(acc, m0, n0, ks, M, g.N, nullptr, nullptr, partial, direct, nsub,
+                                          out_outer);
 }
 
 // Per-row power of two s with max_k |x[row][k]| * s ~ 2^14 (FP16 operands).

@@ -295,9 +295,9 @@ size_t Server::estimate_bytes(const ServeRequest& r) const {
         bytes += r.cartesian ? 8 * nc * ns * nv : echo;  // operand W, or the spiral amplitudes
     } else {
         const size_t nder = fd_count(r);
-        // derivatives, the perturbed simulations in flight (pSSFP: one at a time),
+        // derivatives (each computed in place over its perturbed simulation),
         // the decay operand and the partial sums of the contraction
-        bytes += nder * echo + (r.pssfp ? 1 : nder) * echo + 16 * nv * ns + 2 * echo;
+        bytes += nder * echo + 16 * nv * ns + 2 * echo;
     }
     return bytes;
 }
