@@ -795,16 +795,29 @@ def serve_bench(args):
                                               replace=False).tolist()
     server = ca.Server(slots=args.slots, memory_budget=budget)
     tickets = [serve_submit(server, reqs[i]) for i in cal_ids]
+    # results are taken in completion order: a finished request's results
+    # count against the server's memory budget until released
     times = []
-    for t in tickets:
-        times.append(server.wait(t))
-        server.release(t)
+    outstanding = list(tickets)
+    while outstanding:
+        still = []
+        for t in outstanding:
+            tm = server.wait(t, timeout=0.0)
+            if tm is None:
+                still.append(t)
+            else:
+                times.append(tm)
+                server.release(t)
+        if len(still) == len(outstanding):
+            time.sleep(0.001)
+        outstanding = still
     cal_span = max(d for _, _, d in times) - min(s for _, s, _ in times)
     capacity = len(cal_ids) / cal_span
-    lam = args.load * capacity
+    lam = args.rate if args.rate else args.load * capacity
     progress(f"capacity {capacity:.1f} requests/s; arrivals at {lam:.1f}/s")
 
-    # open loop: Poisson arrivals, a collector thread waits and releases in order
+    # open loop: Poisson arrivals; a collector thread takes results in
+    # completion order (held results count against the memory budget)
     gaps = np.random.default_rng(7).exponential(1.0 / lam, size=n_req)
     arrivals = np.cumsum(gaps)
     results = {}
@@ -814,15 +827,29 @@ def serve_bench(args):
     errors = []
 
     def collector():
-        for i in range(n_req):
-            submitted.acquire()
-            try:
-                stamps[i] = server.wait(tickets[i])
-                if i in check_ids:
-                    results[i] = server.result(tickets[i]).collect()
-                server.release(tickets[i])
-            except Exception as e:  # reported in the JSON line
-                errors.append(f"request {i}: {e}")
+        outstanding, received = [], 0
+        while received < n_req or outstanding:
+            while submitted.acquire(blocking=not outstanding):
+                outstanding.append(received)
+                received += 1
+                if received == n_req:
+                    break
+            still = []
+            for i in outstanding:
+                try:
+                    tm = server.wait(tickets[i], timeout=0.0)
+                    if tm is None:
+                        still.append(i)
+                        continue
+                    stamps[i] = tm
+                    if i in check_ids:
+                        results[i] = server.result(tickets[i]).collect()
+                    server.release(tickets[i])
+                except Exception as e:  # reported in the JSON line
+                    errors.append(f"request {i}: {e}")
+            if still and len(still) == len(outstanding):
+                time.sleep(0.001)
+            outstanding = still
 
     th = threading.Thread(target=collector)
     th.start()
@@ -882,8 +909,9 @@ def serve_bench(args):
         "higher_is_better": True,
         "dtype": "f32 (signal and J^H.y contractions BF16 hi+lo on MFMA)",
         "data": "synthetic (seeded request mix; resident tissue maps, sequences, trajectories, coils)",
-        "config": {"workload": f"{n_req} requests, Poisson arrivals at {args.load:.0%} of measured capacity, "
-                               f"{args.slots} slots, budget {args.budget_gib:g} GiB",
+        "config": {"workload": f"{n_req} requests, Poisson arrivals at "
+                               + (f"{lam:.1f}/s" if args.rate else f"{args.load:.0%} of measured capacity")
+                               + f", {args.slots} slots, budget {args.budget_gib:g} GiB",
                    "mix": counts, "samples_per_readout": SERVE_NS},
         "requests_per_s": len(ok) / span,
         "kspace_samples_per_s": kspace / span,
@@ -1126,6 +1154,8 @@ def main():
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--rate", type=float, default=None,
+                    help="with --serve: Poisson arrivals at this many requests/s instead of --load x capacity")
     ap.add_argument("--calibrate", type=int, default=None,
                     help="requests in the capacity burst (default: 10%% of --requests, 60 to 600)")
     ap.add_argument("--check", type=int, default=100)

@@ -53,6 +53,8 @@ struct CompasException: std::runtime_error {
     do {                                                                                     \
         hipError_t _e = (expr);                                                              \
         if (_e != hipSuccess) {                                                              \
+            (void)hipGetLastError(); /* a handled failure must not resurface at the next   \
+                                        launch check of this thread */                       \
             ::compas::raise_error(hipGetErrorString(_e), __FILE__, __LINE__, #expr);         \
         }                                                                                    \
     } while (0)

@@ -8,7 +8,8 @@
 //   * admits pending requests while their estimated device memory fits the
 //     budget -- in arrival order, except that a request that does not fit
 //     lets later ones that do fit pass until it has waited 0.5 s, after which
-//     it holds the queue until it fits;
+//     it holds the queue until it fits, except for small requests (at most
+//     1/64 of the budget), which still pass; at most 128 requests at once;
 //   * splits each admitted request's simulations (the echoes, and for J^H.y
 //     the perturbed simulations of its finite differences) into work items
 //     (voxel blocks, and slice-position groups for pSSFP) and runs them in
@@ -22,7 +23,7 @@
 //   * when a request's simulations are complete (device events, no host
 //     wait), runs its signal or J^H.y contraction on one of S slot streams.
 // Completion is detected with an event per request; results stay in HBM
-// until taken.  A voxel's echoes are computed by the same arithmetic whatever
+// until released, and count against the memory budget until then.  A voxel's echoes are computed by the same arithmetic whatever
 // else shares a launch, so a request's result is the same alone and in a mix.
 
 #include <atomic>
@@ -107,7 +108,8 @@ class Server {
     struct Entry {
         ServeRequest request;
         ServeResult result;
-        size_t bytes = 0;
+        size_t bytes = 0;  // estimated device memory, counted against the budget from admission
+        size_t held_bytes = 0;  // of which the results, counted until release
         int slot = -1;
         bool table_path = false;  // simulations in work-table launches (else all calls on a slot)
         std::vector<std::shared_ptr<SimJob>> jobs;  // [0]: the echoes; then one per finite difference
@@ -127,6 +129,7 @@ class Server {
     void activate_jobs(int64_t ticket, Entry& e);
     void build_and_launch_batch();
     void complete_jobs();
+    void fail_simulations(const std::string& error);  // every request of admitted_ ends with `error`
     void finish_entry(Entry& e, const std::string& error);
 
     CompasContext base_;

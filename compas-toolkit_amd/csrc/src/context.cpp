@@ -115,6 +115,10 @@ void* Runtime::allocate(size_t nbytes) {
         }
     }
     if (e != hipSuccess) {
+        // the caller may handle this (a request that does not fit ends with an
+        // error): clear the thread's last error, or the next launch check
+        // would report it as that launch's failure
+        (void)hipGetLastError();
         std::stringstream ss;
         ss << "device allocation of " << nbytes << " bytes failed: " << hipGetErrorString(e);
         throw CompasException(ss.str());

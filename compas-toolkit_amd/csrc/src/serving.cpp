@@ -59,8 +59,9 @@ int batch_waves(bool pssfp) {
 }
 
 constexpr int kMaxBatchJobs = 256;  // simulations per launch
-constexpr int kMaxActive = 64;  // admitted requests at once
+constexpr int kMaxActive = 128;  // admitted requests at once
 constexpr double kMaxPassSeconds = 0.5;  // a request that does not fit may be passed this long after arrival
+constexpr size_t kSmallShare = 64;  // requests of at most budget / kSmallShare pass a held queue
 
 const Trajectory& trajectory_of(const ServeRequest& r) {
     if (r.cartesian) return *r.cartesian;
@@ -629,12 +630,33 @@ void Server::finish_entry(Entry& e, const std::string& error) {
         *ev = nullptr;
     }
     if (e.slot >= 0) slot_busy_[size_t(e.slot)] = false;
-    in_flight_bytes_ -= e.bytes;
+    // the results stay in HBM until released: they keep their share of the
+    // budget until then, the rest of the request's estimate is free now
+    size_t held = 0;
+    auto add = [&](const auto& a) { held += size_t(a.numel()) * sizeof(cfloat); };
+    add(e.result.echos);
+    add(e.result.signal);
+    add(e.result.jhy);
+    for (auto& d : e.result.derivatives) add(d);
+    e.held_bytes = std::min(held, e.bytes);
+    in_flight_bytes_ -= e.bytes - e.held_bytes;
     e.request = ServeRequest {};  // drop the references to the inputs
     e.jobs.clear();
     e.derivs.clear();
     e.done = true;
     cv_done_.notify_all();
+}
+
+void Server::fail_simulations(const std::string& error) {
+    // the failed launch's neighbours may still run: let every simulation
+    // stream drain before the jobs' memory and the table ring are reused
+    for (auto& c : sim_) (void)hipStreamSynchronize(c.stream());
+    (void)hipGetLastError();
+    std::vector<int64_t> failed;
+    failed.swap(admitted_);
+    active_jobs_.clear();  // every active job belongs to a request of admitted_
+    std::lock_guard<std::mutex> lock(mutex_);
+    for (int64_t t : failed) finish_entry(*entries_.at(t), error);
 }
 
 void Server::dispatch_loop() {
@@ -653,16 +675,23 @@ void Server::dispatch_loop() {
             running_.erase(running_.begin() + std::ptrdiff_t(i));
         }
 
-        // ---- admissions by memory budget: arrival order, smaller requests may
-        // pass one that does not fit until it has waited long enough
+        // ---- admissions by memory budget: arrival order; later requests may
+        // pass one that does not fit until it has waited kMaxPassSeconds, after
+        // which it holds the queue until it fits -- except for small requests
+        // (at most budget / kSmallShare), which still pass: they hold their
+        // memory for tens of milliseconds, so they delay the held request
+        // little, and holding them would queue them behind its wait
         std::vector<std::pair<int64_t, Entry*>> admitted_now;
         {
             size_t active = admitted_.size() + post_queue_.size() + running_.size();
             const double t = now();
+            const size_t small = budget_ / kSmallShare;
+            bool held = false;
             for (auto it = pending_.begin(); it != pending_.end() && active < size_t(kMaxActive);) {
                 Entry& e = *entries_[*it];
                 const bool nothing_running = active == 0 && admitted_now.empty();
-                if (nothing_running || in_flight_bytes_ + e.bytes <= budget_) {
+                const bool may_pass = !held || e.bytes <= small;
+                if (may_pass && (nothing_running || in_flight_bytes_ + e.bytes <= budget_)) {
                     in_flight_bytes_ += e.bytes;
                     e.result.t_start = t;
                     admitted_now.push_back({*it, &e});
@@ -670,8 +699,7 @@ void Server::dispatch_loop() {
                     active++;
                     continue;
                 }
-                // a request that has waited long enough holds the queue until it fits
-                if (t - e.result.t_arrival > kMaxPassSeconds) break;
+                if (!held && t - e.result.t_arrival > kMaxPassSeconds) held = true;
                 ++it;
             }
         }
@@ -703,12 +731,18 @@ void Server::dispatch_loop() {
             }
         }
 
-        // ---- work-table launches and the completion steps they allow
-        for (int round = 0; round < 2; round++) {
+        // ---- work-table launches and the completion steps they allow; an
+        // error there (a failed launch, device memory exhausted) fails the
+        // requests whose simulations are not complete, and the server goes on
+        try {
+            for (int round = 0; round < 2; round++) {
+                complete_jobs();
+                build_and_launch_batch();
+            }
             complete_jobs();
-            build_and_launch_batch();
+        } catch (const std::exception& ex) {
+            fail_simulations(ex.what());
         }
-        complete_jobs();
 
         // ---- contractions (or whole requests) on free slots
         lock.lock();
@@ -796,6 +830,7 @@ void Server::release(int64_t ticket) {
     auto it = entries_.find(ticket);
     COMPAS_CHECK(it != entries_.end());
     COMPAS_CHECK(it->second->done);
+    in_flight_bytes_ -= it->second->held_bytes;
     entries_.erase(it);
 }
 

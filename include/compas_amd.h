@@ -333,7 +333,8 @@ void* compas_array_to_dlpack(const CompasArrayHandle* array);
  * session calling one operation at a time): continuous batching on one GPU.
  * Pending requests are admitted while the estimated device memory of the
  * admitted ones fits `memory_budget` bytes (arrival order; a request that
- * does not fit lets later ones pass for 0.5 s, then holds the queue).  The
+ * does not fit lets later ones pass for 0.5 s, then holds the queue for all
+ * but small requests of at most 1/64 of the budget; 128 admitted at most).  The
  * simulations of all admitted requests run as work items (voxel blocks,
  * slice-position groups) of shared work-table launches in fair-share order,
  * so one launch serves several requests; a request's contraction (signal or

@@ -203,6 +203,35 @@ def test_request_errors_are_reported():
     server.close()
 
 
+def test_request_beyond_device_memory_fails_alone():
+    """A request whose echoes cannot be allocated (1e6 voxels x 40000 TRs,
+    320 GB, above the budget: admitted when nothing else runs) ends with an
+    allocation error; the request submitted beside it and one submitted after
+    it complete with the plain library result (the failure does not leak into
+    later launches of the dispatcher)."""
+    banner("serving: a request beyond device memory")
+    context()
+    t = W.make_tissue(W.SplitMix64(0xC0A5_0079), 1000, 22.4, 40.0)
+    nTR = 40000
+    tr = W.CartesianTraj(nTR, 8, 5e-6, np.zeros(nTR, np.complex64), float(np.float32(2 * np.pi / 22.4)))
+    p = ca.TissueParameters.from_tissue(t)
+    s = ca.FispSequence.from_sequence(W.make_fisp(nTR, 60.0, 12))
+    traj = ca.CartesianTrajectory.from_trajectory(tr)
+    coils = ca.CompasArray.from_numpy(np.ones((1, t.nvoxels), np.complex64))
+    objs = [device_objects(make_request(i)) for i in (0, 1)]
+    server = ca.Server(slots=2, memory_budget=8 << 30)
+    big = server.submit_simulate_signal(p, s, traj, coils)
+    first = submit(server, objs[0])
+    assert server.wait(first, timeout=120) is not None
+    with pytest.raises(ca.CompasError, match="allocation"):
+        server.wait(big, timeout=120)
+    after = submit(server, objs[1])
+    assert server.wait(after, timeout=120) is not None
+    got = server.result(after).collect()
+    server.close()
+    assert rel(got, run_alone(objs[1])) <= 1e-6
+
+
 @pytest.mark.parametrize("i", [2, 5])  # Cartesian and spiral J^H.y
 def test_inputs_still_queued_on_the_base_stream(i):
     """A residual produced by library calls on the server's base context and
