@@ -106,14 +106,15 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     store_tile<WM, WN>(acc, m0, n0, ks, M, N, row_scale_a, row_scale_b, partial, direct, nsub, out_outer);
 }
 
-// C = A B^T with B generated in staging (GenB); the 2 x 4 x 16 tile, BF16
-// precisions only.  Split-K and the output forms are those of contraction_mfma.
-template<int P>
-__global__ void __launch_bounds__(512) contraction_mfma_genb(
+// C = A B^T with B generated in staging (GenB); WM x WN waves, 16 complex K
+// per step, BF16 precisions only.  Split-K and the output forms are those of
+// contraction_mfma.
+template<int P, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) contraction_mfma_genb(
     const cfloat* __restrict__ A, GenB g, int M, int K, int64_t lda,
     int k_chunk, int tiles_m, int tiles_n, int splits,
     float2* __restrict__ partial, cfloat* __restrict__ direct, int nsub, int64_t out_outer) {
-    constexpr int WM = 2, WN = 4, BKC = 16;
+    constexpr int BKC = 16;
     using Tl = MfmaTile<P, WM, WN, BKC>;
     __shared__ MfmaSmem<P, WM, WN, BKC> sm;
 
@@ -444,12 +445,16 @@ void complex_contraction_mfma(
 
 namespace {
 
+// tile of the generated-operand contraction (waves along M x along N)
+constexpr int GENB_WM = 2, GENB_WN = 4;
+
 template<int P>
 void launch_genb(const CompasContext& ctx, const cfloat* A, const GenB& g, cfloat* out, const ContractionShape& s) {
-    using Tl = MfmaTile<P, 2, 4, 16>;
+    using Tl = MfmaTile<P, GENB_WM, GENB_WN, 16>;
     static const int resident = [] {
         int n = 0;
-        COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, contraction_mfma_genb<P>, Tl::THREADS, 0));
+        COMPAS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, contraction_mfma_genb<P, GENB_WM, GENB_WN>, Tl::THREADS, 0));
         return std::max(1, n);
     }();
     const Plan p = make_plan_for(s, Tl::BM, Tl::BNC, 16, Tl::THREADS, int64_t(resident) * cu_count(ctx.device()));
@@ -459,7 +464,7 @@ void launch_genb(const CompasContext& ctx, const cfloat* A, const GenB& g, cfloa
     if (!direct) partial = ctx.empty<float2, 1>({int64_t(p.splits) * s.M * s.N});
     {
         KernelScope ks(ctx, "contraction_mfma");
-        contraction_mfma_genb<P><<<total, p.threads, 0, ctx.stream()>>>(
+        contraction_mfma_genb<P, GENB_WM, GENB_WN><<<total, p.threads, 0, ctx.stream()>>>(
             A, g, s.M, s.K, s.lda, p.k_chunk, p.tiles_m, p.tiles_n, p.splits,
             direct ? nullptr : partial.data(), direct ? out : nullptr, s.nsub, s.out_outer);
         check_launch("contraction_mfma");
