@@ -484,6 +484,36 @@ def test_jacobian_hermitian_coil_groups(ctx, oracle, nc, nr, ns, nv):
     assert rel(got, want) < 1e-5
 
 
+@pytest.mark.parametrize("nc,nr,ns,nv", [(1, 77, 4, 333), (3, 5, 8, 1), (2, 64, 52, 257), (5, 130, 48, 1000),
+                                         (8, 41, 36, 129), (4, 33, 51, 70)])
+def test_jacobian_product_ragged(ctx, oracle, nc, nr, ns, nv):
+    # J.v on ragged readouts, samples, coils and voxels (down to one voxel and
+    # four samples), every method against FP64 and the MFMA methods against the
+    # FP32 path
+    banner(f"J.v coils={nc} nr={nr} ns={ns} nv={nv}")
+    rng = W.SplitMix64(7000 * nc + nr + ns)
+    n = int(np.ceil(np.sqrt(nv)))
+    t = W.make_tissue(rng, n, 20.0, 40.0).slice(0, nv)
+    traj = W.CartesianTraj(nr, ns, 5e-6, np.zeros(nr, np.complex64), float(np.float32(2 * np.pi / 20.0)))
+
+    def cn(*shape, s=1.0):
+        k = int(np.prod(shape))
+        return ((rng.normal(k) + 1j * rng.normal(k)) * s).reshape(shape).astype(np.complex64)
+
+    coils, echos, d1, d2, v = cn(nc, nv, s=0.5), cn(nr, nv, s=0.3), cn(nr, nv, s=0.01), cn(nr, nv, s=0.1), cn(4, nv)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    ctraj = ca.CartesianTrajectory.from_trajectory(traj)
+    want = oracle.jacobian(t, traj, coils, echos, d1, d2, v, "f64")
+    plain = ca.compute_jacobian(echos, (d1, d2), params, ctraj, coils, v, method="direct").collect()
+    assert rel(plain, want) < 1e-5
+    for method, tol in (("gemm_fast", SIG_TOL), ("gemm_low", 2e-2)):
+        got = ca.compute_jacobian(echos, (d1, d2), params, ctraj, coils, v, method=method).collect()
+        assert got.shape == (nc, nr, ns)
+        assert rel(got, want) < tol, method
+        assert rel(got, plain) < tol, method
+
+
 PRECISION_TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "fp16x3": 1e-5, "bf16x1": 2e-2, "fp16x1": 5e-3}
 
 
