@@ -78,20 +78,33 @@ __global__ void __launch_bounds__(64) spiral_sum_mfma(
 
     const int s_lane = sb + n;  // the lane's sample in tile 0
     for (int vk = v_begin; vk < v_end; vk += 16) {
-        // the lane's four voxels of this K step (K elements 8 g .. 8 g + 7)
+        // the lane's four voxels of this K step (K elements 8 g .. 8 g + 7).
+        // Their phase slope, decay and 16-sample step do not depend on the
+        // lane's sample: lane n of the group evaluates them for voxel n % 4,
+        // and the group's lanes read them from there
+        double theta_own;
+        float decay_own;
+        cfloat st16_own;
+        {
+            const int v = vk + 4 * g + (n & 3);
+            const int vv = v < v_end ? v : v_begin;
+            const float R2 = 1.0f / tissue.T2(vv);
+            theta_own = double(dk.re) * double(tissue.x(vv)) + double(dk.im) * double(tissue.y(vv)) +
+                two_pi_dt * double(tissue.B0(vv));
+            decay_own = -dt * R2;
+            st16_own = dev::polar_reduced(expf(16.0f * decay_own), 16.0 * theta_own);
+        }
         cfloat ph[4], st16[4];
         uint32_t ahw[4], alw[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int v = vk + 4 * g + j;
             const bool ok = v < v_end;
-            const int vv = ok ? v : v_begin;
-            const float R2 = 1.0f / tissue.T2(vv);
-            const double theta = double(dk.re) * double(tissue.x(vv)) + double(dk.im) * double(tissue.y(vv)) +
-                two_pi_dt * double(tissue.B0(vv));
-            const float decay = -dt * R2;
+            const int src = (lane & ~15) | j;
+            const double theta = __shfl(theta_own, src);
+            const float decay = __shfl(decay_own, src);
+            st16[j] = {__shfl(st16_own.re, src), __shfl(st16_own.im, src)};
             ph[j] = ok ? dev::polar_reduced(expf(float(s_lane) * decay), double(s_lane) * theta) : cfloat {0, 0};
-            st16[j] = dev::polar_reduced(expf(16.0f * decay), 16.0 * theta);
             // A: W = C[coil][v] (A or Ab)[r][v] as a (re, -im) or (im, re) pair
             cfloat w = {0.0f, 0.0f};
             if (a_ok && ok) w = dev::cmul(coils[size_t(acoil) * nv + v], a_b ? Ab[row + v] : A[row + v]);
