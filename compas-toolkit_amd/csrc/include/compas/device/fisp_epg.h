@@ -115,10 +115,20 @@ __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int
                     }
                 }
 
-                cfloat Pn[SPL], Qn[SPL];
+                // the states are updated from the top down, and each state's
+                // outputs go straight to their dephased slots (F+ of state i to
+                // i + 1, F- to i - 1), written after the value they replace has
+                // been read: the sums of state i - 1 are formed before state i's
+                // outputs are stored, so no state moves between registers
+                cfloat pn_top = {0, 0}, qn_bottom = {0, 0}, pn0 = {0, 0};
+                cfloat S = P[SPL - 1] + Q[SPL - 1], D = P[SPL - 1] - Q[SPL - 1];
 #pragma unroll
-                for (int i = 0; i < SPL; i++) {
-                    const cfloat S = P[i] + Q[i], D = P[i] - Q[i];
+                for (int i = SPL - 1; i >= 0; i--) {
+                    cfloat Sn = {0, 0}, Dn = {0, 0};
+                    if (i > 0) {
+                        Sn = P[i - 1] + Q[i - 1];
+                        Dn = P[i - 1] - Q[i - 1];
+                    }
                     const cfloat z = Z[i];
                     const float xr = fmaf(h00, S.re, fmaf(h01, D.im, M02 * z.re));
                     const float xi = fmaf(h00, S.im, fmaf(-h01, D.re, M02 * z.im));
@@ -126,33 +136,43 @@ __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int
                     const float yi = fmaf(h10, S.im, fmaf(-h11, D.re, M12 * z.im));
                     const float zr = fmaf(h20, S.re, fmaf(h21, D.im, M22 * z.re));
                     const float zi = fmaf(h20, S.im, fmaf(-h21, D.re, M22 * z.im));
-                    Pn[i] = {xr - yi, xi + yr};
-                    Qn[i] = {xr + yi, xi - yr};
+                    const cfloat pn = {xr - yi, xi + yr}, qn = {xr + yi, xi - yr};
+                    if (i == 0) pn0 = pn;
+                    if (i + 1 < SPL) {
+                        P[i + 1] = pn;  // dephasing: F+_{i+1} <- F+_i
+                    } else {
+                        pn_top = pn;
+                    }
+                    if (i > 0) {
+                        Q[i - 1] = qn;  // dephasing: F-_{i-1} <- F-_i
+                    } else {
+                        qn_bottom = qn;
+                    }
                     Z[i] = {zr, zi};
+                    S = Sn;
+                    D = Dn;
                 }
                 if (owns_zero) Z[0].re += G;
                 if (!direct_echo && sample && owns_zero) {
-                    const cfloat echo = dev::cmul(Pn[0], inv_TRmTE);
+                    const cfloat echo = dev::cmul(pn0, inv_TRmTE);
                     for (int u = 0; u < a.usf; u++) {
                         cfloat* out = a.echos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
                         *out = (slice == 0) ? echo : (*out + echo);
                     }
                 }
 
-                // dephasing: F+_k <- F+_{k-1}, F-_k <- F-_{k+1}, F-_{N-1} <- 0, F+_0 <- conj(F-_0)
+                // the ends of the dephasing: F+ into state 0 and F- into the top
+                // state come from the neighbouring lane (or are zero), F-_{N-1} <- 0,
+                // F+_0 <- conj(F-_0)
                 cfloat p_in = {0, 0}, q_in = {0, 0};
                 if constexpr (LPV > 1) {
-                    p_in.re = __shfl_up(Pn[SPL - 1].re, 1, LPV);
-                    p_in.im = __shfl_up(Pn[SPL - 1].im, 1, LPV);
-                    q_in.re = __shfl_down(Qn[0].re, 1, LPV);
-                    q_in.im = __shfl_down(Qn[0].im, 1, LPV);
+                    p_in.re = __shfl_up(pn_top.re, 1, LPV);
+                    p_in.im = __shfl_up(pn_top.im, 1, LPV);
+                    q_in.re = __shfl_down(qn_bottom.re, 1, LPV);
+                    q_in.im = __shfl_down(qn_bottom.im, 1, LPV);
                     if (sub == LPV - 1) q_in = {0, 0};
                 }
-#pragma unroll
-                for (int i = SPL - 1; i >= 1; i--) P[i] = Pn[i - 1];
                 P[0] = p_in;
-#pragma unroll
-                for (int i = 0; i < SPL - 1; i++) Q[i] = Qn[i + 1];
                 Q[SPL - 1] = q_in;
                 if constexpr (LPV > 1) {
 #pragma unroll
