@@ -47,15 +47,12 @@ __device__ inline float reduce_phase(double phi) {
     return float(fma(-k, TWO_PI, phi));
 }
 
-// sin and cos of a float argument: Cody-Waite reduction by pi/2 with a
+// sin and cos of |x| <= 64: Cody-Waite reduction by pi/2 with a
 // two-constant split, then the Taylor series of sin and cos on [-pi/4, pi/4]
 // evaluated by Horner's rule (truncation below 2e-9), quadrant by the
-// reduction index.  Arguments beyond |x| = 64 use the library sincosf.
-__device__ inline void sincos_fast(float x, float* s, float* c) {
-    if (fabsf(x) > 64.0f) {
-        sincosf(x, s, c);
-        return;
-    }
+// reduction index.  No branch, so a caller that has bounded its arguments
+// keeps its loop body in one block.
+__device__ inline void sincos_poly(float x, float* s, float* c) {
     const float kf = rintf(x * 0.63661977236758134f);
     float r = fmaf(-kf, 1.5707963705062866f, x);  // float(pi/2)
     r = fmaf(-kf, -4.3711388286737929e-08f, r);  // pi/2 - float(pi/2)
@@ -76,27 +73,19 @@ __device__ inline void sincos_fast(float x, float* s, float* c) {
     *c = ((q + 1) & 2) ? -cc : cc;
 }
 
-// sin and cos of |x| <= pi + 1e-3 (a phase already reduced by reduce_phase):
-// the polynomial part of sincos_fast without its large-argument branch.
+// sin and cos of a float argument: sincos_poly, and the library sincosf
+// beyond |x| = 64.
+__device__ inline void sincos_fast(float x, float* s, float* c) {
+    if (fabsf(x) > 64.0f) {
+        sincosf(x, s, c);
+        return;
+    }
+    sincos_poly(x, s, c);
+}
+
+// sin and cos of |x| <= pi + 1e-3 (a phase already reduced by reduce_phase)
 __device__ inline void sincos_reduced(float x, float* s, float* c) {
-    const float kf = rintf(x * 0.63661977236758134f);
-    float r = fmaf(-kf, 1.5707963705062866f, x);
-    r = fmaf(-kf, -4.3711388286737929e-08f, r);
-    const float r2 = r * r;
-    float ps = fmaf(r2, 2.7557319e-6f, -1.9841270e-4f);
-    ps = fmaf(r2, ps, 8.3333333e-3f);
-    ps = fmaf(r2, ps, -1.6666667e-1f);
-    const float sr = fmaf(r * r2, ps, r);
-    float pc = fmaf(r2, -2.7557319e-7f, 2.4801587e-5f);
-    pc = fmaf(r2, pc, -1.3888889e-3f);
-    pc = fmaf(r2, pc, 4.1666668e-2f);
-    pc = fmaf(r2, pc, -0.5f);
-    const float cr = fmaf(r2, pc, 1.0f);
-    const int q = int(kf);
-    const float ss = (q & 1) ? cr : sr;
-    const float cc = (q & 1) ? sr : cr;
-    *s = (q & 2) ? -ss : ss;
-    *c = ((q + 1) & 2) ? -cc : cc;
+    sincos_poly(x, s, c);
 }
 
 // mag * exp(i phi) with the phase formed in FP64 and reduced before the float sincos

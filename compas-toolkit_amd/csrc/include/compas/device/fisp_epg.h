@@ -15,10 +15,23 @@
 namespace compas {
 namespace kernels {
 
+// Largest flip angle (deg2rad |RF|) of the launch's RF table, the same in
+// every lane of the wave.  Every lane of the wave must be active.
+__device__ inline float fisp_table_max_angle(const FispArgs& a) {
+    const int total = a.nslices * a.nTR;
+    float m = 0.0f;
+    for (int i = int(__lane_id()); i < total; i += 64) m = fmaxf(m, fabsf(a.table[i].x));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    return m;
+}
+
 // The recursion of one voxel (lane `sub` of its LPV lanes) over every slice
-// profile and repetition; writes the voxel's echoes.
-template<int SPL, int LPV>
-__device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int sub) {
+// profile and repetition; writes the voxel's echoes.  WIDE: the half flip
+// angles take sincos_fast (some may exceed 64 in magnitude), else the
+// branch-free sincos_poly.
+template<int SPL, int LPV, bool WIDE>
+__device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel, int sub) {
     const float T1 = a.tissue.T1(voxel);
     const float T2 = a.tissue.T2(voxel);
     const float B1 = a.tissue.B1(voxel);
@@ -53,6 +66,19 @@ __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int
 
     for (int slice = 0; slice < a.nslices; slice++) {
         const float4* tab = a.table + size_t(slice) * size_t(a.nTR);
+        // the first slice of a fully sampled sequence stores each echo once;
+        // otherwise every undersampled copy, and later slices add to it
+        const bool plain_store = (a.usf == 1) && (slice == 0);
+        auto store_echo = [&](int n, cfloat echo) {
+            if (plain_store) {
+                a.echos[size_t(n) * size_t(a.tissue.nvoxels) + voxel] = echo;
+            } else {
+                for (int u = 0; u < a.usf; u++) {
+                    cfloat* out = a.echos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
+                    *out = (slice == 0) ? echo : (*out + echo);
+                }
+            }
+        };
         cfloat P[SPL], Q[SPL], Z[SPL];
 #pragma unroll
         for (int i = 0; i < SPL; i++) {
@@ -74,11 +100,18 @@ __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int
                 if (owns_zero) Z[0].re += 1.0f - E1_TI;
             }
 
+            // the RF table entry of the next TR is loaded one TR ahead
+            float4 t_next = (a.nTR > 0) ? tab[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             for (int n = 0; n < a.nTR; n++) {
-                const float4 t = tab[n];  // wave-uniform
+                const float4 t = t_next;  // wave-uniform
+                t_next = tab[(n + 1 < a.nTR) ? n + 1 : n];
                 const float alpha = t.x * B1;
                 float sx, cx;
-                dev::sincos_fast(0.5f * alpha, &sx, &cx);
+                if constexpr (WIDE) {
+                    dev::sincos_fast(0.5f * alpha, &sx, &cx);
+                } else {
+                    dev::sincos_poly(0.5f * alpha, &sx, &cx);
+                }
                 const float sa = 2.0f * sx * cx;  // sin(alpha)
                 const float omc = 2.0f * sx * sx;  // 1 - cos(alpha), without cancellation
                 const float cp = t.y, sp = t.z;
@@ -109,10 +142,7 @@ __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int
                     const float yr = fmaf(0.5f * T10, S0.re, fmaf(0.5f * T11, D0.im, T12 * z0.re));
                     const float yi = fmaf(0.5f * T10, S0.im, fmaf(-0.5f * T11, D0.re, T12 * z0.im));
                     const cfloat echo = {xr - yi, xi + yr};
-                    for (int u = 0; u < a.usf; u++) {
-                        cfloat* out = a.echos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
-                        *out = (slice == 0) ? echo : (*out + echo);
-                    }
+                    store_echo(n, echo);
                 }
 
                 // the states are updated from the top down, and each state's
@@ -155,10 +185,7 @@ __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int
                 if (owns_zero) Z[0].re += G;
                 if (!direct_echo && sample && owns_zero) {
                     const cfloat echo = dev::cmul(pn0, inv_TRmTE);
-                    for (int u = 0; u < a.usf; u++) {
-                        cfloat* out = a.echos + size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
-                        *out = (slice == 0) ? echo : (*out + echo);
-                    }
+                    store_echo(n, echo);
                 }
 
                 // the ends of the dephasing: F+ into state 0 and F- into the top
@@ -198,6 +225,23 @@ __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int
             }
             if (owns_zero) Z[0].re += 1.0f - E1_W;
         }
+    }
+}
+
+// fisp_epg_voxel_form in the form the wave's angles need.  Whether some half
+// flip angle alpha/2 = t.x B1 / 2 of the wave can exceed 64 in magnitude,
+// where sincos_fast leaves its polynomial for the library sincosf, is decided
+// once: |fl(t.x B1)| <= fl(max_angle |B1|) since rounding is monotone, so a
+// wave without such lanes takes the polynomial for every TR, as sincos_fast
+// would, and its TR loop carries no branch for it.  max_angle: the wave's
+// fisp_table_max_angle.
+template<int SPL, int LPV>
+__device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int sub, float max_angle) {
+    const float B1 = a.tissue.B1(voxel);
+    if (__any(!(0.5f * (max_angle * fabsf(B1)) <= 64.0f))) {
+        fisp_epg_voxel_form<SPL, LPV, true>(a, voxel, sub);
+    } else {
+        fisp_epg_voxel_form<SPL, LPV, false>(a, voxel, sub);
     }
 }
 

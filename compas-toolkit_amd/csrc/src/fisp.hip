@@ -54,8 +54,9 @@ template<int SPL, int LPV>
 __global__ void __launch_bounds__(64) fisp_epg(FispArgs a) {
     const int sub = (LPV > 1) ? int(threadIdx.x % LPV) : 0;
     const int voxel = (blockIdx.x * 64 + threadIdx.x) / LPV;
+    const float max_angle = fisp_table_max_angle(a);  // while every lane is active
     if (voxel >= a.tissue.nvoxels) return;  // whole LPV groups leave together
-    fisp_epg_voxel<SPL, LPV>(a, voxel, sub);
+    fisp_epg_voxel<SPL, LPV>(a, voxel, sub, max_angle);
 }
 
 }  // namespace kernels

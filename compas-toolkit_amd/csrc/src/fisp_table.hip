@@ -17,8 +17,9 @@ __global__ void __launch_bounds__(64) fisp_epg_table(const FispArgs* __restrict_
     const int2 it = items[blockIdx.x];
     const FispArgs a = jobs[it.x];
     const int voxel = it.y * 64 + int(threadIdx.x);
+    const float max_angle = fisp_table_max_angle(a);  // while every lane is active
     if (voxel >= a.tissue.nvoxels) return;
-    fisp_epg_voxel<SPL, 1>(a, voxel, 0);
+    fisp_epg_voxel<SPL, 1>(a, voxel, 0, max_angle);
 }
 
 }  // namespace kernels

@@ -84,6 +84,25 @@ def test_fisp_b0_zero_branch(ctx, oracle):
     assert rel(run_fisp(t, seq), oracle.fisp(t, seq, "f64")) < MAG_TOL
 
 
+@pytest.mark.parametrize("peak_deg,nv", [(5000.0, 300), (8000.0, 300), (8000.0, 66000)])
+def test_fisp_wide_flip_angles(ctx, oracle, peak_deg, nv):
+    # A wave whose half flip angles alpha/2 = deg2rad |RF| B1 / 2 can pass 64
+    # takes the TR loop with the library sincosf branch, a wave whose largest
+    # stays below it the branch-free polynomial.  B1 in [0.8, 1.2]: 5000 deg
+    # keeps alpha/2 <= 52.4 (every wave polynomial, at large arguments);
+    # 8000 deg puts alpha/2 in [55.9, 83.8], both sides of 64 within waves.
+    # 300 voxels run four lanes per voxel, 66000 one.
+    banner(f"fisp flip angles up to {peak_deg} deg, nv={nv}")
+    t, seq = small_fisp(nv=nv, nTR=60, N=16)
+    RF = seq.RF.copy()
+    RF[7::11] = (peak_deg * np.exp(1j * np.linspace(-0.4, 0.4, RF[7::11].size))).astype(np.complex64)
+    seq.RF = RF
+    half = np.deg2rad(peak_deg) * t.B1 / 2
+    assert (half.max() > 64) == (peak_deg > 6000)
+    got = run_fisp(t, seq)
+    assert rel(got, oracle.fisp(t, seq, "f64")) < MAG_TOL
+
+
 @pytest.mark.parametrize("nv,options", [(300, False), (300, True), (5000, False)])
 def test_fisp_echo_paths(ctx, oracle, nv, options):
     # the kernel samples F+_0 at TE by undoing the decay over TR - TE on the
