@@ -73,6 +73,23 @@ __device__ inline void sincos_poly(float x, float* s, float* c) {
     *c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// sincos_poly of an argument whose reduction index rint(x 2/pi) is 0
+// (|fl(x 2/pi)| <= 1/2, |x| <= pi/4): the same polynomial operations on x
+// itself, without the reduction and the quadrant selection, so the same
+// results (a negative zero argument aside).
+__device__ inline void sincos_poly_q0(float x, float* s, float* c) {
+    const float r2 = x * x;
+    float ps = fmaf(r2, 2.7557319e-6f, -1.9841270e-4f);
+    ps = fmaf(r2, ps, 8.3333333e-3f);
+    ps = fmaf(r2, ps, -1.6666667e-1f);
+    *s = fmaf(x * r2, ps, x);
+    float pc = fmaf(r2, -2.7557319e-7f, 2.4801587e-5f);
+    pc = fmaf(r2, pc, -1.3888889e-3f);
+    pc = fmaf(r2, pc, 4.1666668e-2f);
+    pc = fmaf(r2, pc, -0.5f);
+    *c = fmaf(r2, pc, 1.0f);
+}
+
 // sin and cos of a float argument: sincos_poly, and the library sincosf
 // beyond |x| = 64.
 __device__ inline void sincos_fast(float x, float* s, float* c) {

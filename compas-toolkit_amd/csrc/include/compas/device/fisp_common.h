@@ -10,7 +10,7 @@ namespace kernels {
 
 struct FispArgs {
     dev::TissueView tissue;
-    const float4* table;  // [nslices][nTR]: (deg2rad*|RF|, cos phi, sin phi, 0)
+    const float4* table;  // [nslices][nTR]: (deg2rad*|RF|, cos phi, sin phi, 0), then one zero entry
     int nTR;
     int nslices;
     float TR, TE, TW, TI;

@@ -26,12 +26,44 @@ __device__ inline float fisp_table_max_angle(const FispArgs& a) {
     return m;
 }
 
+// Forms of the half flip angle's sincos in the TR loop
+enum FispHalfAngle : int {
+    HALF_Q0 = 0,    // every |alpha/2| has reduction index 0: sincos_poly_q0
+    HALF_POLY = 1,  // every |alpha/2| <= 64: sincos_poly
+    HALF_WIDE = 2,  // sincos_fast (the library sincosf beyond 64)
+};
+
+template<int FORM>
+__device__ __forceinline__ void half_angle_sincos(float x, float* s, float* c) {
+    if constexpr (FORM == HALF_WIDE) {
+        dev::sincos_fast(x, s, c);
+    } else if constexpr (FORM == HALF_POLY) {
+        dev::sincos_poly(x, s, c);
+    } else {
+        dev::sincos_poly_q0(x, s, c);
+    }
+}
+
+// F+_0 at TE is taken from the updated state: over TR - TE the transverse
+// state only decays and precesses (F+ times d_TRmTE), so F+_0(TE) =
+// F+_0(TR) / d_TRmTE, one complex multiply per TR instead of a second 2x3
+// operator.  A wave with a lane whose d_TRmTE is too small to invert (T2 of a
+// few microseconds) applies the operator D(TE) R to state 0 instead: this
+// tells whether the wave must (wave-uniform).
+__device__ inline bool fisp_direct_echo(const FispArgs& a, float T2, float B0) {
+    const cfloat d_TRmTE = off_resonance(a.TR - a.TE, B0) * relax(a.TR - a.TE, T2);
+    const float dd = d_TRmTE.re * d_TRmTE.re + d_TRmTE.im * d_TRmTE.im;
+    return __any(!(dd > 1e-30f));
+}
+
 // The recursion of one voxel (lane `sub` of its LPV lanes) over every slice
-// profile and repetition; writes the voxel's echoes.  WIDE: the half flip
-// angles take sincos_fast (some may exceed 64 in magnitude), else the
-// branch-free sincos_poly.
-template<int SPL, int LPV, bool WIDE>
-__device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel, int sub) {
+// profile and repetition; writes the voxel's echoes.  FORM: the sincos of
+// the half flip angles (FispHalfAngle).  direct_echo: fisp_direct_echo of
+// the wave.  SIMPLE: one slice profile, one repetition, no undersampling, no
+// direct echo and at most 2^28 voxels, so every TR stores its echo once at a
+// 32-bit byte offset from the TR's row, with no test in the loop.
+template<int SPL, int LPV, int FORM, bool SIMPLE>
+__device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel, int sub, bool direct_echo) {
     const float T1 = a.tissue.T1(voxel);
     const float T2 = a.tissue.T2(voxel);
     const float B1 = a.tissue.B1(voxel);
@@ -54,13 +86,8 @@ __device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel
     const bool owns_zero = (sub == 0);  // lane holding state k = 0
     const int k0 = sub * SPL;           // global index of local state 0
 
-    // F+_0 at TE from the updated state: over TR - TE the transverse state
-    // only decays and precesses (F+ times d_TRmTE), so F+_0(TE) = F+_0(TR) /
-    // d_TRmTE, one complex multiply per TR instead of a second 2x3 operator.
-    // A wave with a lane whose d_TRmTE is too small to invert (T2 of a few
-    // microseconds) applies the operator D(TE) R to state 0 instead.
+    // 1 / d_TRmTE for the echo from the updated state (fisp_direct_echo)
     const float dd = d_TRmTE.re * d_TRmTE.re + d_TRmTE.im * d_TRmTE.im;
-    const bool direct_echo = __any(!(dd > 1e-30f));  // wave-uniform
     const cfloat inv_TRmTE = direct_echo ? cfloat {0.0f, 0.0f}
                                          : cfloat {d_TRmTE.re / dd, -d_TRmTE.im / dd};
 
@@ -100,18 +127,18 @@ __device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel
                 if (owns_zero) Z[0].re += 1.0f - E1_TI;
             }
 
-            // the RF table entry of the next TR is loaded one TR ahead
+            // the RF table entry of the next TR is loaded one TR ahead (the
+            // table holds one entry more than the TRs of all slices)
             float4 t_next = (a.nTR > 0) ? tab[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // SIMPLE: the TR's echo row, and the voxel's byte offset in it
+            cfloat* row = a.echos;
+            const uint32_t voff = uint32_t(voxel) * uint32_t(sizeof(cfloat));
             for (int n = 0; n < a.nTR; n++) {
                 const float4 t = t_next;  // wave-uniform
-                t_next = tab[(n + 1 < a.nTR) ? n + 1 : n];
+                t_next = tab[n + 1];
                 const float alpha = t.x * B1;
                 float sx, cx;
-                if constexpr (WIDE) {
-                    dev::sincos_fast(0.5f * alpha, &sx, &cx);
-                } else {
-                    dev::sincos_poly(0.5f * alpha, &sx, &cx);
-                }
+                half_angle_sincos<FORM>(0.5f * alpha, &sx, &cx);
                 const float sa = 2.0f * sx * cx;  // sin(alpha)
                 const float omc = 2.0f * sx * sx;  // 1 - cos(alpha), without cancellation
                 const float cp = t.y, sp = t.z;
@@ -130,12 +157,28 @@ __device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel
                 const float h10 = 0.5f * M10, h11 = 0.5f * M11;
                 const float h20 = 0.5f * M20, h21 = 0.5f * M21;
 
+                // S = F+ + F- and D = F+ - F- of local state i.  With one lane
+                // per voxel (SPL = N) two of them are known: F+_0 = conj(F-_0)
+                // (S_0 = (2 Re F-_0, 0), D_0 = (0, -2 Im F-_0), the same values
+                // as the sums) and F-_{N-1} = 0, so F+_0 and F-_{N-1} are never
+                // stored.
+                auto sums = [&](int i, cfloat& Si, cfloat& Di) {
+                    if (LPV == 1 && i == SPL - 1 && SPL > 1) {
+                        Si = P[i];
+                        Di = P[i];
+                    } else {
+                        Si = P[i] + Q[i];
+                        Di = P[i] - Q[i];
+                    }
+                };
+
                 // sample F+_0 at TE:  rows 0/1 of D(TE) R applied to state 0
-                if (direct_echo && sample && owns_zero) {
+                if (!SIMPLE && direct_echo && sample && owns_zero) {
                     const float er = d_TE.re, ei = d_TE.im;
                     const float T00 = er * R00 - ei * R10, T01 = er * R01 - ei * R11, T02 = er * R02 - ei * R12;
                     const float T10 = ei * R00 + er * R10, T11 = ei * R01 + er * R11, T12 = ei * R02 + er * R12;
-                    const cfloat S0 = P[0] + Q[0], D0 = P[0] - Q[0];
+                    cfloat S0, D0;
+                    sums(0, S0, D0);
                     const cfloat z0 = Z[0];
                     const float xr = fmaf(0.5f * T00, S0.re, fmaf(0.5f * T01, D0.im, T02 * z0.re));
                     const float xi = fmaf(0.5f * T00, S0.im, fmaf(-0.5f * T01, D0.re, T02 * z0.im));
@@ -151,21 +194,26 @@ __device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel
                 // been read: the sums of state i - 1 are formed before state i's
                 // outputs are stored, so no state moves between registers
                 cfloat pn_top = {0, 0}, qn_bottom = {0, 0}, pn0 = {0, 0};
-                cfloat S = P[SPL - 1] + Q[SPL - 1], D = P[SPL - 1] - Q[SPL - 1];
+                cfloat S, D;
+                sums(SPL - 1, S, D);
 #pragma unroll
                 for (int i = SPL - 1; i >= 0; i--) {
                     cfloat Sn = {0, 0}, Dn = {0, 0};
-                    if (i > 0) {
-                        Sn = P[i - 1] + Q[i - 1];
-                        Dn = P[i - 1] - Q[i - 1];
-                    }
+                    if (i > 0) sums(i - 1, Sn, Dn);
                     const cfloat z = Z[i];
                     const float xr = fmaf(h00, S.re, fmaf(h01, D.im, M02 * z.re));
-                    const float xi = fmaf(h00, S.im, fmaf(-h01, D.re, M02 * z.im));
                     const float yr = fmaf(h10, S.re, fmaf(h11, D.im, M12 * z.re));
-                    const float yi = fmaf(h10, S.im, fmaf(-h11, D.re, M12 * z.im));
                     const float zr = fmaf(h20, S.re, fmaf(h21, D.im, M22 * z.re));
-                    const float zi = fmaf(h20, S.im, fmaf(-h21, D.re, M22 * z.im));
+                    float xi, yi, zi;
+                    if (false) {
+                        xi = M02 * z.im;
+                        yi = M12 * z.im;
+                        zi = M22 * z.im;
+                    } else {
+                        xi = fmaf(h00, S.im, fmaf(-h01, D.re, M02 * z.im));
+                        yi = fmaf(h10, S.im, fmaf(-h11, D.re, M12 * z.im));
+                        zi = fmaf(h20, S.im, fmaf(-h21, D.re, M22 * z.im));
+                    }
                     const cfloat pn = {xr - yi, xi + yr}, qn = {xr + yi, xi - yr};
                     if (i == 0) pn0 = pn;
                     if (i + 1 < SPL) {
@@ -183,7 +231,12 @@ __device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel
                     D = Dn;
                 }
                 if (owns_zero) Z[0].re += G;
-                if (!direct_echo && sample && owns_zero) {
+                if constexpr (SIMPLE) {
+                    if (owns_zero) {
+                        *reinterpret_cast<cfloat*>(reinterpret_cast<char*>(row) + voff) = dev::cmul(pn0, inv_TRmTE);
+                    }
+                    row += a.tissue.nvoxels;
+                } else if (!direct_echo && sample && owns_zero) {
                     const cfloat echo = dev::cmul(pn0, inv_TRmTE);
                     store_echo(n, echo);
                 }
@@ -228,20 +281,27 @@ __device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel
     }
 }
 
-// fisp_epg_voxel_form in the form the wave's angles need.  Whether some half
-// flip angle alpha/2 = t.x B1 / 2 of the wave can exceed 64 in magnitude,
-// where sincos_fast leaves its polynomial for the library sincosf, is decided
-// once: |fl(t.x B1)| <= fl(max_angle |B1|) since rounding is monotone, so a
-// wave without such lanes takes the polynomial for every TR, as sincos_fast
-// would, and its TR loop carries no branch for it.  max_angle: the wave's
-// fisp_table_max_angle.
+// fisp_epg_voxel_form in the form the wave needs, decided once per wave so
+// that the TR loop carries no test for it.  The half flip angles alpha/2 =
+// t.x B1 / 2 of the wave are bounded by h = 0.5 fl(max_angle |B1|), since
+// rounding is monotone (|fl(t.x B1)| <= fl(max_angle |B1|)): below 64 every
+// sincos_fast would take its polynomial, and where also fl(h 2/pi) <= 1/2
+// every reduction index is 0.  max_angle: the wave's fisp_table_max_angle.
 template<int SPL, int LPV>
 __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int sub, float max_angle) {
     const float B1 = a.tissue.B1(voxel);
-    if (__any(!(0.5f * (max_angle * fabsf(B1)) <= 64.0f))) {
-        fisp_epg_voxel_form<SPL, LPV, true>(a, voxel, sub);
+    const float h = 0.5f * (max_angle * fabsf(B1));
+    const bool direct_echo = fisp_direct_echo(a, a.tissue.T2(voxel), a.tissue.B0(voxel));
+    const bool simple = !direct_echo && a.nslices == 1 && a.repetitions == 1 && a.usf == 1 &&
+                        a.tissue.nvoxels <= (1 << 28);
+    if (__any(!(h <= 64.0f))) {
+        fisp_epg_voxel_form<SPL, LPV, HALF_WIDE, false>(a, voxel, sub, direct_echo);
+    } else if (!simple) {
+        fisp_epg_voxel_form<SPL, LPV, HALF_POLY, false>(a, voxel, sub, direct_echo);
+    } else if (__any(!(h * 0.63661977236758134f <= 0.5f))) {
+        fisp_epg_voxel_form<SPL, LPV, HALF_POLY, true>(a, voxel, sub, false);
     } else {
-        fisp_epg_voxel_form<SPL, LPV, false>(a, voxel, sub);
+        fisp_epg_voxel_form<SPL, LPV, HALF_Q0, true>(a, voxel, sub, false);
     }
 }
 

@@ -35,9 +35,16 @@
 namespace compas {
 namespace kernels {
 
+// table[i] for i < nslices nTR, then a zero entry at nslices nTR (the
+// recursion loads the entry of the TR after its last one); nslices nTR + 1
+// threads.
 __global__ void fisp_rf_table(const cfloat* RF, const cfloat* profiles, int nTR, int nslices, float4* table) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nTR * nslices) return;
+    if (i > nTR * nslices) return;
+    if (i == nTR * nslices) {
+        table[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     int n = i % nTR;
     cfloat p = profiles[i], r = RF[n];
     cfloat rf = {p.re * r.re - p.im * r.im, p.re * r.im + p.im * r.re};
@@ -153,10 +160,9 @@ static FispLauncher fisp_launcher_lpv4(int spl) {
 
 Array<float4, 1> fisp_rf_table(const CompasContext& ctx, const FISPSequence& seq) {
     const int nTR = int(seq.RF_train.size(0)), nslices = int(seq.sliceprofiles.size(0));
-    auto table = ctx.empty<float4, 1>({int64_t(nslices) * nTR});
-    if (table.numel() == 0) return table;
+    auto table = ctx.empty<float4, 1>({int64_t(nslices) * nTR + 1});
     KernelScope ks(ctx, "fisp_rf_table");
-    const int total = nslices * nTR;
+    const int total = nslices * nTR + 1;
     kernels::fisp_rf_table<<<dev::div_ceil(total, 256), 256, 0, ctx.stream()>>>(
         seq.RF_train.data(), seq.sliceprofiles.data(), nTR, nslices, table.data());
     check_launch("fisp_rf_table");
@@ -259,7 +265,7 @@ Array<cfloat, 2> simulate_magnetization(
 
 static constexpr int64_t kRfTableScratchEntries = int64_t(1) << 16;  // (slice, TR) pairs: 1 MiB
 
-static __device__ __align__(256) float4 scratch[kRfTableScratchEntries];
+static __device__ __align__(256) float4 scratch[kRfTableScratchEntries + 1];  // and the terminating zero entry
 
 static float4* scratch_address(int device) {
     static std::mutex mutex;
@@ -303,7 +309,7 @@ void simulate_magnetization_into(
         return;
     }
     float4* table = scratch_address(ctx.device());
-    const int total = int(nslices * nTR);
+    const int total = int(nslices * nTR) + 1;
     kernels::fisp_rf_table<<<dev::div_ceil(total, 256), 256, 0, ctx.stream()>>>(
         seq.RF_train.data(), seq.sliceprofiles.data(), int(nTR), int(nslices), table);
     check_launch("fisp_rf_table");
