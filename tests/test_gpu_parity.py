@@ -84,6 +84,21 @@ def test_fisp_b0_zero_branch(ctx, oracle):
     assert rel(run_fisp(t, seq), oracle.fisp(t, seq, "f64")) < MAG_TOL
 
 
+@pytest.mark.parametrize("nTR", [1, 2, 3])
+@pytest.mark.parametrize("nslices", [1, 2])
+def test_fisp_short_trains(ctx, oracle, nTR, nslices):
+    # the recursion reads the RF table entry of the TR after the current one
+    # (the next slice's first, or the table's terminating entry after the
+    # last TR of the last slice); one slice runs the loop without tests
+    banner(f"fisp nTR={nTR} nslices={nslices}")
+    t, seq = small_fisp(nv=130, nTR=nTR, N=8)
+    if nslices == 2:
+        seq.profiles = np.stack([np.linspace(0.6, 1.1, nTR), np.full(nTR, 0.9 - 0.2j)]).astype(np.complex64)
+    got = run_fisp(t, seq)
+    assert got.shape == (nTR, 130)
+    assert rel(got, oracle.fisp(t, seq, "f64")) < MAG_TOL
+
+
 @pytest.mark.parametrize("peak_deg,nv", [(5000.0, 300), (8000.0, 300), (8000.0, 66000)])
 def test_fisp_wide_flip_angles(ctx, oracle, peak_deg, nv):
     # A wave whose half flip angles alpha/2 = deg2rad |RF| B1 / 2 can pass 64
