@@ -15,47 +15,6 @@
 namespace compas {
 namespace kernels {
 
-// Largest flip angle (deg2rad |RF|) of the launch's RF table, the same in
-// every lane of the wave.  Every lane of the wave must be active.
-__device__ inline float fisp_table_max_angle(const FispArgs& a) {
-    const int total = a.nslices * a.nTR;
-    float m = 0.0f;
-    for (int i = int(__lane_id()); i < total; i += 64) m = fmaxf(m, fabsf(a.table[i].x));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    return m;
-}
-
-// Forms of the half flip angle's sincos in the TR loop
-enum FispHalfAngle : int {
-    HALF_Q0 = 0,    // every |alpha/2| has reduction index 0: sincos_poly_q0
-    HALF_POLY = 1,  // every |alpha/2| <= 64: sincos_poly
-    HALF_WIDE = 2,  // sincos_fast (the library sincosf beyond 64)
-};
-
-template<int FORM>
-__device__ __forceinline__ void half_angle_sincos(float x, float* s, float* c) {
-    if constexpr (FORM == HALF_WIDE) {
-        dev::sincos_fast(x, s, c);
-    } else if constexpr (FORM == HALF_POLY) {
-        dev::sincos_poly(x, s, c);
-    } else {
-        dev::sincos_poly_q0(x, s, c);
-    }
-}
-
-// F+_0 at TE is taken from the updated state: over TR - TE the transverse
-// state only decays and precesses (F+ times d_TRmTE), so F+_0(TE) =
-// F+_0(TR) / d_TRmTE, one complex multiply per TR instead of a second 2x3
-// operator.  A wave with a lane whose d_TRmTE is too small to invert (T2 of a
-// few microseconds) applies the operator D(TE) R to state 0 instead: this
-// tells whether the wave must (wave-uniform).
-__device__ inline bool fisp_direct_echo(const FispArgs& a, float T2, float B0) {
-    const cfloat d_TRmTE = off_resonance(a.TR - a.TE, B0) * relax(a.TR - a.TE, T2);
-    const float dd = d_TRmTE.re * d_TRmTE.re + d_TRmTE.im * d_TRmTE.im;
-    return __any(!(dd > 1e-30f));
-}
-
 // The recursion of one voxel (lane `sub` of its LPV lanes) over every slice
 // profile and repetition; writes the voxel's echoes.  FORM: the sincos of
 // the half flip angles (FispHalfAngle).  direct_echo: fisp_direct_echo of
@@ -281,24 +240,16 @@ __device__ __forceinline__ void fisp_epg_voxel_form(const FispArgs& a, int voxel
     }
 }
 
-// fisp_epg_voxel_form in the form the wave needs, decided once per wave so
-// that the TR loop carries no test for it.  The half flip angles alpha/2 =
-// t.x B1 / 2 of the wave are bounded by h = 0.5 fl(max_angle |B1|), since
-// rounding is monotone (|fl(t.x B1)| <= fl(max_angle |B1|)): below 64 every
-// sincos_fast would take its polynomial, and where also fl(h 2/pi) <= 1/2
-// every reduction index is 0.  max_angle: the wave's fisp_table_max_angle.
+// fisp_epg_voxel_form in the form the wave needs (fisp_wave_form), decided
+// once per wave so that the TR loop carries no test for it.
 template<int SPL, int LPV>
 __device__ __forceinline__ void fisp_epg_voxel(const FispArgs& a, int voxel, int sub, float max_angle) {
-    const float B1 = a.tissue.B1(voxel);
-    const float h = 0.5f * (max_angle * fabsf(B1));
-    const bool direct_echo = fisp_direct_echo(a, a.tissue.T2(voxel), a.tissue.B0(voxel));
-    const bool simple = !direct_echo && a.nslices == 1 && a.repetitions == 1 && a.usf == 1 &&
-                        a.tissue.nvoxels <= (1 << 28);
-    if (__any(!(h <= 64.0f))) {
-        fisp_epg_voxel_form<SPL, LPV, HALF_WIDE, false>(a, voxel, sub, direct_echo);
-    } else if (!simple) {
-        fisp_epg_voxel_form<SPL, LPV, HALF_POLY, false>(a, voxel, sub, direct_echo);
-    } else if (__any(!(h * 0.63661977236758134f <= 0.5f))) {
+    const FispWaveForm f = fisp_wave_form(a, voxel, max_angle);
+    if (f.wide) {
+        fisp_epg_voxel_form<SPL, LPV, HALF_WIDE, false>(a, voxel, sub, f.direct_echo);
+    } else if (!f.simple) {
+        fisp_epg_voxel_form<SPL, LPV, HALF_POLY, false>(a, voxel, sub, f.direct_echo);
+    } else if (!f.q0) {
         fisp_epg_voxel_form<SPL, LPV, HALF_POLY, true>(a, voxel, sub, false);
     } else {
         fisp_epg_voxel_form<SPL, LPV, HALF_Q0, true>(a, voxel, sub, false);

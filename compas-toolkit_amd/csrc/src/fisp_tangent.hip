@@ -96,14 +96,17 @@ __device__ inline Rows decay_rows(
     return r;
 }
 
-template<int SPL, int LPV, int MASK>
-__global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes out) {
+// The recursion and its tangents for one voxel (lane `sub` of its LPV
+// lanes).  FORM: the sincos of the half flip angles (FispHalfAngle);
+// SIMPLE: the simple-sequence form of fisp_wave_form (no test in the TR
+// loop, each TR's outputs stored once through running row pointers);
+// direct_echo: fisp_direct_echo of the wave.
+template<int SPL, int LPV, int MASK, int FORM, bool SIMPLE>
+__device__ __forceinline__ void fisp_tangents_voxel(const FispArgs& a, const FispPlanes& out, int voxel, int sub,
+                                                    bool direct_echo) {
     using TS = TangentSet<MASK>;
     constexpr int NT = TS::nt, NP = TS::planes, OFS = int(TS::echo);
     static_assert(NT >= 1, "a tangent pass carries at least one tangent");
-    const int sub = (LPV > 1) ? int(threadIdx.x % LPV) : 0;
-    const int voxel = (blockIdx.x * 64 + threadIdx.x) / LPV;
-    if (voxel >= a.tissue.nvoxels) return;  // whole LPV groups leave together
 
     const float T1 = a.tissue.T1(voxel);
     const float T2 = a.tissue.T2(voxel);
@@ -140,7 +143,6 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
     // A wave with a lane whose d_TRmTE cannot be inverted samples with the
     // operator D(TE) R as before.
     const float dd = d_TRmTE.re * d_TRmTE.re + d_TRmTE.im * d_TRmTE.im;
-    const bool direct_echo = __any(!(dd > 1e-30f));  // wave-uniform
     const cfloat inv_TRmTE = direct_echo ? cfloat {0.0f, 0.0f}
                                          : cfloat {d_TRmTE.re / dd, -d_TRmTE.im / dd};
     const float k2TRmTE = (a.TR - a.TE) * iT2s, k0TRmTE = dev::TWO_PI_F * (a.TR - a.TE);
@@ -176,11 +178,20 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
                 }
             }
 
+            // the RF table entry of the next TR is loaded one TR ahead (the
+            // table holds one entry more than the TRs of all slices)
+            float4 t_next = (a.nTR > 0) ? tab[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // SIMPLE: each output plane's row of the TR, and the voxel's byte offset in it
+            cfloat* rows[NP];
+#pragma unroll
+            for (int k = 0; k < NP; k++) rows[k] = out.p[k];
+            const uint32_t voff = uint32_t(voxel) * uint32_t(sizeof(cfloat));
             for (int n = 0; n < a.nTR; n++) {
-                const float4 t = tab[n];  // wave-uniform
+                const float4 t = t_next;  // wave-uniform
+                t_next = tab[n + 1];
                 const float alpha = t.x * B1;
                 float sx, cx;
-                dev::sincos_fast(0.5f * alpha, &sx, &cx);
+                half_angle_sincos<FORM>(0.5f * alpha, &sx, &cx);
                 const float sa = 2.0f * sx * cx;  // sin(alpha)
                 const float omc = 2.0f * sx * sx;  // 1 - cos(alpha)
                 const float cp = t.y, sp = t.z;
@@ -196,10 +207,10 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
                     const float dR10 = dR01, dR11 = -sa * cp * cp * t.x, dR12 = -ca * cp * t.x;
                     const float dR20 = -ca * sp * t.x, dR21 = ca * cp * t.x, dR22 = -sa * t.x;
                     N = decay_rows(d_TR, E1_TR, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
-                    if (direct_echo) Sb = decay_rows(d_TE, 0.0f, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
+                    if (!SIMPLE && direct_echo) Sb = decay_rows(d_TE, 0.0f, dR00, dR01, dR02, dR10, dR11, dR12, dR20, dR21, dR22);
                 }
 
-                if (direct_echo && sample && owns_zero) {
+                if (!SIMPLE && direct_echo && sample && owns_zero) {
                     const Rows T = decay_rows(d_TE, 0.0f, R00, R01, R02, R10, R11, R12, R20, R21, R22);
                     const cfloat S0 = P[0] + Q[0], D0 = P[0] - Q[0];
                     const cfloat echo = sample_rows(T, S0, D0, Z[0]);
@@ -256,7 +267,7 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
                     Z[0].re += G;
                     if constexpr (TS::t1) dZ[TS::it1][0].re += dG;
                 }
-                if (!direct_echo && sample && owns_zero) {
+                if ((SIMPLE || (!direct_echo && sample)) && owns_zero) {
                     const cfloat echo = dev::cmul(Pn[0], inv_TRmTE);
                     cfloat v[NP];
                     if constexpr (TS::echo) v[0] = echo;
@@ -267,11 +278,21 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
                         if (TS::b0 && k == TS::ib0) dv = dv - times_i(echo) * k0TRmTE;
                         v[OFS + k] = dv;
                     }
-                    for (int u = 0; u < a.usf; u++) {
-                        const size_t at = size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
+                    if constexpr (SIMPLE) {
 #pragma unroll
-                        for (int k = 0; k < NP; k++) out.p[k][at] = (slice == 0) ? v[k] : (out.p[k][at] + v[k]);
+                        for (int k = 0; k < NP; k++)
+                            *reinterpret_cast<cfloat*>(reinterpret_cast<char*>(rows[k]) + voff) = v[k];
+                    } else {
+                        for (int u = 0; u < a.usf; u++) {
+                            const size_t at = size_t(n * a.usf + u) * size_t(a.tissue.nvoxels) + voxel;
+#pragma unroll
+                            for (int k = 0; k < NP; k++) out.p[k][at] = (slice == 0) ? v[k] : (out.p[k][at] + v[k]);
+                        }
                     }
+                }
+                if constexpr (SIMPLE) {
+#pragma unroll
+                    for (int k = 0; k < NP; k++) rows[k] += a.tissue.nvoxels;
                 }
 
                 // dephasing: F+ moves up one state, F- down one, primal and tangents alike
@@ -337,6 +358,24 @@ __global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes o
                 if constexpr (TS::t1) dZ[TS::it1][0].re -= dE1_W;
             }
         }
+    }
+}
+
+// grid ceil(nvoxels LPV / 64), block 64: the recursion and tangents in the
+// form the wave needs (fisp_wave_form).
+template<int SPL, int LPV, int MASK>
+__global__ void __launch_bounds__(64) fisp_epg_tangents(FispArgs a, FispPlanes out) {
+    const int sub = (LPV > 1) ? int(threadIdx.x % LPV) : 0;
+    const int voxel = (blockIdx.x * 64 + threadIdx.x) / LPV;
+    const float max_angle = fisp_table_max_angle(a);  // while every lane is active
+    if (voxel >= a.tissue.nvoxels) return;  // whole LPV groups leave together
+    const FispWaveForm f = fisp_wave_form(a, voxel, max_angle);
+    if (f.wide) {
+        fisp_tangents_voxel<SPL, LPV, MASK, HALF_WIDE, false>(a, out, voxel, sub, f.direct_echo);
+    } else if (!f.simple) {
+        fisp_tangents_voxel<SPL, LPV, MASK, HALF_POLY, false>(a, out, voxel, sub, f.direct_echo);
+    } else {
+        fisp_tangents_voxel<SPL, LPV, MASK, HALF_POLY, true>(a, out, voxel, sub, false);
     }
 }
 
