@@ -42,6 +42,32 @@ struct Vec3d {
 constexpr double SERIES_LIMIT = 0.0625;
 constexpr double SERIES_LIMIT_WIDE = 0.25;
 
+// Shorter series for the small angles of most RF sub-steps: four terms up to
+// sqrt(SERIES_LIMIT_4) = 1/27 rad, five up to sqrt(SERIES_LIMIT_5) = 1/9 rad
+// (first omitted terms u^4/9! <= 9.8e-18 and u^5/11! <= 7.2e-18, relative).
+constexpr double SERIES_LIMIT_4 = 1.0 / 729.0;
+constexpr double SERIES_LIMIT_5 = 1.0 / 81.0;
+
+__device__ inline void rotation_series_4(double u, double& s1, double& s2) {
+    s1 = fma(u, -1.0 / 5040.0, 1.0 / 120.0);
+    s1 = fma(u, s1, -1.0 / 6.0);
+    s1 = fma(u, s1, 1.0);
+    s2 = fma(u, -1.0 / 40320.0, 1.0 / 720.0);
+    s2 = fma(u, s2, -1.0 / 24.0);
+    s2 = fma(u, s2, 0.5);
+}
+
+__device__ inline void rotation_series_5(double u, double& s1, double& s2) {
+    s1 = fma(u, 1.0 / 362880.0, -1.0 / 5040.0);
+    s1 = fma(u, s1, 1.0 / 120.0);
+    s1 = fma(u, s1, -1.0 / 6.0);
+    s1 = fma(u, s1, 1.0);
+    s2 = fma(u, 1.0 / 3628800.0, -1.0 / 40320.0);
+    s2 = fma(u, s2, 1.0 / 720.0);
+    s2 = fma(u, s2, -1.0 / 24.0);
+    s2 = fma(u, s2, 0.5);
+}
+
 // s1 = sin(t)/t and s2 = (1 - cos t)/t^2 of t = sqrt(u), u <= SERIES_LIMIT:
 // Taylor series in u, six terms each (first omitted terms below 1e-17), in
 // Horner form: five FMAs per series and no powers of u; the ZPL isochromats

@@ -23,8 +23,8 @@
 //     applies nRF Rodrigues rotations with little T2 damping.
 //   * Rodrigues' rotation is evaluated without sqrt/rcp/sin/cos for the small
 //     per-step angles: with u = |a|^2, s1(u) = sin(t)/t and s2(u) =
-//     (1 - cos t)/t^2 are Taylor polynomials in u (|t| <= 1/4: 6 terms, error
-//     < 1e-17), cos t = 1 - u s2(u), and
+//     (1 - cos t)/t^2 are Taylor polynomials in u (4, 5, 6 or 7 terms for |t|
+//     up to 1/27, 1/9, 1/4, 1/2; error < 1e-17), cos t = 1 - u s2(u), and
 //       m' = cos t m + s1 (a x m) + s2 (a . m) a,
 //     which equals the reference's normalised form.  Larger angles fall back
 //     to FP64 sincos; the choice is made once per RF sub-step for the whole
@@ -113,7 +113,23 @@ __device__ __forceinline__ void pssfp_iso_block(const PssfpArgs& a, int group, i
                 const double umax = axy2 + az2max;
                 // wave-uniform choice of the series: the common small-angle
                 // cases are straight-line code over the ZPL isochromats
-                if (!__any(umax > SERIES_LIMIT)) {
+                if (!__any(umax > SERIES_LIMIT_4)) {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        double s1, s2;
+                        rotation_series_4(u[i], s1, s2);
+                        apply_rotation(m[i], ax, ay, az[i], u[i], s1, s2);
+                        relax3(m[i], E1_ex, E2_ex, R1_ex);
+                    }
+                } else if (!__any(umax > SERIES_LIMIT_5)) {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        double s1, s2;
+                        rotation_series_5(u[i], s1, s2);
+                        apply_rotation(m[i], ax, ay, az[i], u[i], s1, s2);
+                        relax3(m[i], E1_ex, E2_ex, R1_ex);
+                    }
+                } else if (!__any(umax > SERIES_LIMIT)) {
 #pragma unroll
                     for (int i = 0; i < ZPL; i++) {
                         double s1, s2;
