@@ -81,7 +81,55 @@ __device__ inline void rotation_series_derivative_wide(double u, double& s1p, do
     s2p = fma(u6, -7.0 / 20922789888000.0, s2p);
 }
 
-enum { TIER_SERIES = 0, TIER_WIDE = 1, TIER_ANY = 2 };
+// s1, s2 and s1', s2' of the short tiers (u <= SERIES_LIMIT_4: four terms
+// of s1, s2, their term-wise derivatives with one more term; u <=
+// SERIES_LIMIT_5: five), summed over the powers of u; first omitted terms
+// below 1e-17 relative as for the longer tiers.
+__device__ inline void rotation_series_4_powers(double u, double& s1, double& s2) {
+    const double u2 = u * u, u3 = u2 * u;
+    s1 = fma(u, -1.0 / 6.0, 1.0);
+    s1 = fma(u2, 1.0 / 120.0, s1);
+    s1 = fma(u3, -1.0 / 5040.0, s1);
+    s2 = fma(u, -1.0 / 24.0, 0.5);
+    s2 = fma(u2, 1.0 / 720.0, s2);
+    s2 = fma(u3, -1.0 / 40320.0, s2);
+}
+
+__device__ inline void rotation_series_derivative_4(double u, double& s1p, double& s2p) {
+    const double u2 = u * u, u3 = u2 * u;
+    s1p = fma(u, 2.0 / 120.0, -1.0 / 6.0);
+    s1p = fma(u2, -3.0 / 5040.0, s1p);
+    s1p = fma(u3, 4.0 / 362880.0, s1p);
+    s2p = fma(u, 2.0 / 720.0, -1.0 / 24.0);
+    s2p = fma(u2, -3.0 / 40320.0, s2p);
+    s2p = fma(u3, 4.0 / 3628800.0, s2p);
+}
+
+__device__ inline void rotation_series_5_powers(double u, double& s1, double& s2) {
+    const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2;
+    s1 = fma(u, -1.0 / 6.0, 1.0);
+    s1 = fma(u2, 1.0 / 120.0, s1);
+    s1 = fma(u3, -1.0 / 5040.0, s1);
+    s1 = fma(u4, 1.0 / 362880.0, s1);
+    s2 = fma(u, -1.0 / 24.0, 0.5);
+    s2 = fma(u2, 1.0 / 720.0, s2);
+    s2 = fma(u3, -1.0 / 40320.0, s2);
+    s2 = fma(u4, 1.0 / 3628800.0, s2);
+}
+
+__device__ inline void rotation_series_derivative_5(double u, double& s1p, double& s2p) {
+    const double u2 = u * u, u3 = u2 * u, u4 = u2 * u2;
+    s1p = fma(u, 2.0 / 120.0, -1.0 / 6.0);
+    s1p = fma(u2, -3.0 / 5040.0, s1p);
+    s1p = fma(u3, 4.0 / 362880.0, s1p);
+    s1p = fma(u4, -5.0 / 39916800.0, s1p);
+    s2p = fma(u, 2.0 / 720.0, -1.0 / 24.0);
+    s2p = fma(u2, -3.0 / 40320.0, s2p);
+    s2p = fma(u3, 4.0 / 3628800.0, s2p);
+    s2p = fma(u4, -5.0 / 479001600.0, s2p);
+}
+
+enum { TIER_SERIES = 0, TIER_WIDE = 1, TIER_ANY = 2, TIER_4 = 3, TIER_5 = 4 };
 
 // Rot(a) as a matrix: c I + s1 [a]x + s2 a a^T
 struct RotationMatrix {
@@ -101,8 +149,9 @@ __device__ inline RotationMatrix rotation_matrix(double ax, double ay, double az
 
 // Rot(a) m and the tangents dm[k]: Rot(a) dm[k], plus (dRot/da . da/dp) m
 // for B1 (da = (zre, -zim, 0)) and B0 (da = (0, 0, daz)).
-// TIER: the six-term series (u <= SERIES_LIMIT), the seven-term one (u <=
-// SERIES_LIMIT_WIDE), or any u (the seven-term series, closed forms above).
+// TIER: the four-, five- or six-term series (u <= SERIES_LIMIT_4,
+// SERIES_LIMIT_5, SERIES_LIMIT), the seven-term one (u <= SERIES_LIMIT_WIDE),
+// or any u (the seven-term series, closed forms above).
 template<int MASK, int TIER>
 __device__ inline void rotate_all(
     Vec3d& m,
@@ -116,7 +165,13 @@ __device__ inline void rotate_all(
     double daz) {
     using S = TangentSet<MASK>;
     double s1, s2, s1p = 0.0, s2p = 0.0;
-    if (TIER == TIER_SERIES) {
+    if (TIER == TIER_4) {
+        rotation_series_4_powers(u, s1, s2);
+        if constexpr (S::axis) rotation_series_derivative_4(u, s1p, s2p);
+    } else if (TIER == TIER_5) {
+        rotation_series_5_powers(u, s1, s2);
+        if constexpr (S::axis) rotation_series_derivative_5(u, s1p, s2p);
+    } else if (TIER == TIER_SERIES) {
         rotation_series_powers(u, s1, s2);
         if constexpr (S::axis) rotation_series_derivative(u, s1p, s2p);
     } else if (TIER == TIER_WIDE || u <= SERIES_LIMIT_WIDE) {
@@ -346,7 +401,19 @@ __global__ void __launch_bounds__(64) pssfp_iso_tangents(PssfpArgs a, int group0
                 // the lane's largest u: a_z is fixed per slice position
                 const double umax = axy2 + az2max;
                 // wave-uniform choice of the series, as in the primal kernel
-                if (!__any(umax > SERIES_LIMIT)) {
+                if (!__any(umax > SERIES_LIMIT_4)) {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        rotate_all<MASK, TIER_4>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
+                        relax_all<MASK>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE1_ex, dE2_ex);
+                    }
+                } else if (!__any(umax > SERIES_LIMIT_5)) {
+#pragma unroll
+                    for (int i = 0; i < ZPL; i++) {
+                        rotate_all<MASK, TIER_5>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
+                        relax_all<MASK>(m[i], dm[i], E1_ex, E2_ex, R1_ex, dE1_ex, dE2_ex);
+                    }
+                } else if (!__any(umax > SERIES_LIMIT)) {
 #pragma unroll
                     for (int i = 0; i < ZPL; i++) {
                         rotate_all<MASK, TIER_SERIES>(m[i], dm[i], ax, ay, az[i], u[i], zre, zim, daz);
