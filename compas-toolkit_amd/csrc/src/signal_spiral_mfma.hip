@@ -47,8 +47,10 @@ __device__ inline mfma::bf16x8 pack_bf16x8(const uint32_t (&w)[4]) {
 }
 
 // partial[split][c][r][s] over voxels [v_begin, v_end) of split blockIdx.y;
-// grid (nr, splits, sample passes), block 64.
-template<bool TWO, bool LO, int SSM_TILES>
+// grid (nr, splits, sample passes), block 64.  FULL: every pass holds
+// SSM_TILES whole tiles (ns a multiple of 16 SSM_TILES), so the tile loop
+// carries no test.
+template<bool TWO, bool LO, int SSM_TILES, bool FULL>
 __global__ void __launch_bounds__(64) spiral_sum_mfma(
     dev::TissueView tissue, const cfloat* __restrict__ A, const cfloat* __restrict__ Ab,
     const cfloat* __restrict__ coils, int nc, const cfloat* __restrict__ delta_k, int nr, int ns, float dt,
@@ -116,7 +118,7 @@ __global__ void __launch_bounds__(64) spiral_sum_mfma(
         const mfma::bf16x8 alo = pack_bf16x8(alw);
 #pragma unroll
         for (int t = 0; t < SSM_TILES; t++) {
-            if (t < ntiles) {
+            if (FULL || t < ntiles) {
                 uint32_t bhw[4], blw[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) mfma::split_pair<P>(ph[j].re, ph[j].im, bhw[j], blw[j]);
@@ -141,7 +143,7 @@ __global__ void __launch_bounds__(64) spiral_sum_mfma(
 #pragma unroll
     for (int t = 0; t < SSM_TILES; t++) {
         const int s = sb + 16 * t + n;
-        if (t < ntiles && s < ns) {
+        if (FULL || (t < ntiles && s < ns)) {
             if constexpr (TWO) {
                 const int c = g;
                 if (c < nc) {
@@ -215,12 +217,17 @@ bool spiral_sum_mfma(
         const dim3 grid(nr, splits, passes);
         const auto tv = dev::tissue_view(p);
         hipStream_t s = ctx.stream();
+        const bool full = ns % (16 * tiles) == 0;
         auto launch = [&](auto two_c, auto lo_c) {
             constexpr bool TWO = decltype(two_c)::value;
             constexpr bool LO = decltype(lo_c)::value;
-            kernels::spiral_sum_mfma<TWO, LO, tiles><<<grid, 64, 0, s>>>(tv, A, TWO ? Ab : nullptr, coils, nc,
-                                                                        t.delta_k.data(), nr, ns, t.delta_t,
-                                                                        v_chunk, partial.data());
+            if (full) {
+                kernels::spiral_sum_mfma<TWO, LO, tiles, true><<<grid, 64, 0, s>>>(
+                    tv, A, TWO ? Ab : nullptr, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, v_chunk, partial.data());
+            } else {
+                kernels::spiral_sum_mfma<TWO, LO, tiles, false><<<grid, 64, 0, s>>>(
+                    tv, A, TWO ? Ab : nullptr, coils, nc, t.delta_k.data(), nr, ns, t.delta_t, v_chunk, partial.data());
+            }
         };
         if (two) {
             if (lo) launch(std::true_type {}, std::true_type {});

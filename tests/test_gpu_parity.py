@@ -389,7 +389,21 @@ def spiral6_inputs(nc, nr, ns, nv, seed):
 # more than eight (the VALU form), samples across the 256-sample LDS chunks and
 # not a multiple of 16, voxels not a multiple of the 32-voxel wave tile
 SPIRAL_SHAPES = [(4, 70, 64, 784), (1, 33, 50, 333), (5, 41, 37, 1000), (8, 16, 130, 257), (3, 9, 600, 97),
-                 (10, 7, 40, 70)]
+                 (10, 7, 40, 70), (4, 5, 512, 300), (2, 3, 1024, 130)]
+
+
+@pytest.mark.parametrize("nc,nr,ns,nv", [(4, 6, 512, 300), (8, 3, 1024, 200), (3, 4, 496, 150)])
+def test_spiral_signal_sample_passes(ctx, oracle, nc, nr, ns, nv):
+    # sample counts that fill every 512-sample pass of the MFMA spiral sum
+    # (its tile loop without tests) and one that does not
+    banner(f"spiral signal coils={nc} nr={nr} ns={ns} nv={nv}")
+    t, traj, coils, echos, _, _, _ = spiral6_inputs(nc, nr, ns, nv, 11 * nc + ns)
+    context()
+    params = ca.TissueParameters.from_tissue(t)
+    straj = ca.SpiralTrajectory.from_trajectory(traj)
+    got = ca.magnetization_to_signal(echos, params, straj, coils).collect()
+    assert got.shape == (nc, nr, ns)
+    assert rel(got, oracle.signal_spiral(t, traj, coils, echos, "f64")) < SIG_TOL
 
 
 @pytest.mark.parametrize("method", ["direct", "gemm_fast"])
