@@ -428,8 +428,10 @@ struct GenB {
 // gc + G i (G = THREADS / BKC), so that one store instruction covers
 // consecutive rows: the caller's epilogue maps LDS row rho back to column
 // 4 (rho % G) + rho / G.  The A staging and the MFMA loop are those of
-// mfma_accumulate.  n0: first complex column of the tile.
-template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2>
+// mfma_accumulate.  n0: first complex column of the tile.  FULL: every row
+// and column of the tile exists, the K range is whole steps and the A rows
+// take 16-byte loads, so the staging loads carry no bounds tests.
+template<int P, int WM, int WN, int BKC, int TM = 2, int TN = 2, bool FULL = false>
 __device__ inline void mfma_accumulate_genb(
     f32x16 (&acc)[TM][TN], MfmaSmem<P, WM, WN, BKC, TM, TN>& sm,
     const cfloat* __restrict__ a_src, float sa, bool vec_a,
@@ -459,7 +461,7 @@ __device__ inline void mfma_accumulate_genb(
     cfloat rC, rT, rq;
     auto gload = [&](int k0) {
         const int ka = k0 + a_k;
-        if (vec_a && a_src && ka + Tl::A_PER <= ke) {
+        if (FULL || (vec_a && a_src && ka + Tl::A_PER <= ke)) {
 #pragma unroll
             for (int i = 0; i < Tl::A_PER / 2; i++) {
                 const float4 v = *reinterpret_cast<const float4*>(a_src + ka + 2 * i);
@@ -471,7 +473,7 @@ __device__ inline void mfma_accumulate_genb(
             for (int i = 0; i < Tl::A_PER; i++) ra[i] = (a_src && ka + i < ke) ? a_src[ka + i] : cfloat {0, 0};
         }
         const int v = k0 + gk;
-        if (col_ok && v < ke) {
+        if (FULL || (col_ok && v < ke)) {
             rC = gC[v];
             rT = gT[v];
             rq = g.q[v];

@@ -106,9 +106,29 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma(
     store_tile<WM, WN>(acc, m0, n0, ks, M, N, row_scale_a, row_scale_b, partial, direct, nsub, out_outer);
 }
 
+// One output tile of contraction_mfma_genb over K range [kb, ke); FULL as
+// in mfma_accumulate_genb.
+template<int P, int WM, int WN, bool FULL>
+__device__ __forceinline__ void genb_tile(
+    MfmaSmem<P, WM, WN, 16>& sm, const cfloat* __restrict__ A, const GenB& g, int M, int64_t lda, bool vec_a,
+    int m0, int n0, int ks, int kb, int ke, float2* __restrict__ partial, cfloat* __restrict__ direct, int nsub,
+    int64_t out_outer) {
+    constexpr int BKC = 16;
+    using Tl = MfmaTile<P, WM, WN, BKC>;
+    f32x16 acc[2][2];
+    zero(acc);
+    const int am = m0 + staged_a_row<P, WM, WN, BKC>(threadIdx.x);
+    mfma_accumulate_genb<P, WM, WN, BKC, 2, 2, FULL>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, 1.0f, vec_a,
+                                                     g, n0, kb, ke);
+    store_tile<WM, WN, Tl::THREADS / BKC>(acc, m0, n0, ks, M, g.N, nullptr, nullptr, partial, direct, nsub,
+                                          out_outer);
+}
+
 // C = A B^T with B generated in staging (GenB); WM x WN waves, 16 complex K
 // per step, BF16 precisions only.  Split-K and the output forms are those of
-// contraction_mfma.
+// contraction_mfma.  A workgroup whose tile lies inside the output, whose K
+// range is whole steps and whose A rows take 16-byte loads runs the staging
+// without bounds tests.
 template<int P, int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma_genb(
     const cfloat* __restrict__ A, GenB g, int M, int K, int64_t lda,
@@ -128,14 +148,13 @@ __global__ void __launch_bounds__(64 * WM * WN) contraction_mfma_genb(
     const int kb = ks * k_chunk;
     const int ke = min(K, kb + k_chunk);
 
-    f32x16 acc[2][2];
-    zero(acc);
-    const int am = m0 + staged_a_row<P, WM, WN, BKC>(threadIdx.x);
     const bool vec_a = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 2 == 0);
-    mfma_accumulate_genb<P, WM, WN, BKC>(acc, sm, am < M ? A + int64_t(am) * lda : nullptr, 1.0f, vec_a, g, n0,
-                                         kb, ke);
-    store_tile<WM, WN, Tl::THREADS / BKC>(acc, m0, n0, ks, M, g.N, nullptr, nullptr, partial, direct, nsub,
-                                          out_outer);
+    const bool full = vec_a && m0 + Tl::BM <= M && n0 + Tl::BNC <= g.N && (ke - kb) % BKC == 0;
+    if (full) {
+        genb_tile<P, WM, WN, true>(sm, A, g, M, lda, vec_a, m0, n0, ks, kb, ke, partial, direct, nsub, out_outer);
+    } else {
+        genb_tile<P, WM, WN, false>(sm, A, g, M, lda, vec_a, m0, n0, ks, kb, ke, partial, direct, nsub, out_outer);
+    }
 }
 
 // Per-row power of two s with max_k |x[row][k]| * s ~ 2^14 (FP16 operands).
